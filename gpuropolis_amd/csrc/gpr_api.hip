@@ -1,0 +1,512 @@
+// C ABI of libgpuropolis.so (declared in include/gpuropolis.h).
+//
+// A context owns one HIP stream on one device and every device buffer of a run:
+// the datasets, the chain state in structure-of-arrays layout, the work-tile tables
+// and the counters.  Host copies happen only in gpr_set_data, gpr_write_state and
+// gpr_read_state; a block of MH steps plus resampling stays on the device.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "gpr_internal.h"
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess) return fail(GPR_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+    } while (0)
+
+struct KernelBucket {
+    int kc;
+    bool hi;
+    int tile0, ntiles;
+};
+
+struct gpr_ctx {
+    gpr_config cfg;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nd = 0, N = 0, rows = 0;
+    std::vector<GprDataset> hds;
+    GprDataset *d_ds = nullptr;
+    float *d_x = nullptr, *d_y = nullptr, *d_w = nullptr;
+    uint8_t *d_ops = nullptr;
+    float *d_cs = nullptr;
+    int32_t *d_len = nullptr;
+    double *d_prior = nullptr, *d_ll = nullptr;
+    float *d_grad = nullptr;
+    std::vector<GprTile> htiles;
+    std::vector<KernelBucket> buckets;
+    GprTile *d_tiles = nullptr;
+    int *d_tile_counter = nullptr;  // one per bucket
+    unsigned long long *d_counters = nullptr;
+    int32_t *d_good = nullptr, *d_ngood = nullptr, *d_map_chain = nullptr, *d_nres = nullptr;
+    double *d_map_post = nullptr, *d_cutoff = nullptr;
+    uint32_t step = 0, block = 0;
+    bool have_data = false, have_state = false;
+    double last_ms = 0.0;
+    int sched_used = GPR_SCHED_WAVE;
+    int cus = 256;
+};
+
+extern "C" const char *gpr_last_error(void) { return g_err.c_str(); }
+extern "C" const char *gpr_version(void) { return "gpuropolis-mi355x 0.1 (gfx950)"; }
+
+extern "C" int gpr_config_default(gpr_config *c) {
+    if (!c) return fail(GPR_E_ARG, "null config");
+    memset(c, 0, sizeof(*c));
+    c->device = 0;
+    c->semantics = GPR_SEM_CONFIG;
+    c->cap = 32;
+    c->dmax = GPR_MAX_DEPTH;
+    c->nsym = 10;
+    c->proposal = GPR_PROP_RW;
+    c->sigma = 0.1f;
+    c->eps = 0.05f;
+    c->struct_every = 5;
+    c->fixed_structure = 0;
+    c->schedule = GPR_SCHED_AUTO;
+    c->rng = GPR_RNG_PHILOX;
+    c->resample = GPR_RS_MAP;
+    c->resample_nats = 100.0;
+    c->resample_quantile = 0.5f;
+    c->seed = 0;
+    return GPR_OK;
+}
+
+static void free_state(gpr_ctx *c) {
+    void *ps[] = {c->d_ds, c->d_x, c->d_y, c->d_w, c->d_ops, c->d_cs, c->d_len, c->d_prior, c->d_ll, c->d_grad,
+                  c->d_tiles, c->d_tile_counter, c->d_counters, c->d_good, c->d_ngood, c->d_map_chain,
+                  c->d_nres, c->d_map_post, c->d_cutoff};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    c->d_ds = nullptr; c->d_x = c->d_y = c->d_w = nullptr; c->d_ops = nullptr; c->d_cs = nullptr;
+    c->d_len = nullptr; c->d_prior = c->d_ll = nullptr; c->d_grad = nullptr; c->d_tiles = nullptr;
+    c->d_tile_counter = nullptr; c->d_counters = nullptr; c->d_good = c->d_ngood = c->d_map_chain = nullptr;
+    c->d_nres = nullptr; c->d_map_post = c->d_cutoff = nullptr;
+    c->have_data = c->have_state = false;
+}
+
+extern "C" int gpr_create(const gpr_config *cfg, gpr_ctx **out) {
+    if (!cfg || !out) return fail(GPR_E_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->semantics != GPR_SEM_CONFIG && cfg->semantics != GPR_SEM_REFERENCE)
+        return fail(GPR_E_ARG, "semantics must be GPR_SEM_CONFIG or GPR_SEM_REFERENCE");
+    if (cfg->semantics == GPR_SEM_CONFIG && (cfg->cap < 1 || cfg->cap > GPR_MAX_CAP))
+        return fail(GPR_E_ARG, "cap must be in [1, 128]");
+    if (cfg->dmax < 1 || cfg->dmax > GPR_MAX_DEPTH) return fail(GPR_E_ARG, "dmax must be in [1, 16]");
+    if (cfg->struct_every < 1) return fail(GPR_E_ARG, "struct_every must be >= 1");
+    if (cfg->nsym < 2) return fail(GPR_E_ARG, "nsym must be >= 2");
+    if (cfg->rng == GPR_RNG_LCG && cfg->semantics != GPR_SEM_REFERENCE)
+        return fail(GPR_E_ARG, "the LCG stream exists only with reference semantics");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (cfg->device < 0 || cfg->device >= ndev)
+        return fail(GPR_E_ARG, "invalid GPU device " + std::to_string(cfg->device) + " (" + std::to_string(ndev) +
+                                   " present)");
+    HIP_TRY(hipSetDevice(cfg->device));
+    gpr_ctx *c = new gpr_ctx();
+    c->cfg = *cfg;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess) c->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(GPR_E_HIP, "stream/event creation failed");
+    }
+    *out = c;
+    return GPR_OK;
+}
+
+extern "C" int gpr_destroy(gpr_ctx *c) {
+    if (!c) return GPR_OK;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_state(c);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GPR_OK;
+}
+
+static int kc_for(int n) { return n <= 64 ? 1 : (n <= 128 ? 2 : 4); }
+
+template <class T>
+static int dev_alloc(T **p, size_t count) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(GPR_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return GPR_OK;
+}
+#define ALLOC(p, n) do { int _r = dev_alloc(&(p), (n)); if (_r) return _r; } while (0)
+
+extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x, const float *y, const float *sd,
+                            const int32_t *chains, const int32_t *caps) {
+    if (!c || nd < 1 || !n || !x || !y || !sd || !chains) return fail(GPR_E_ARG, "null or empty dataset arguments");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    free_state(c);
+    const bool ref = c->cfg.semantics == GPR_SEM_REFERENCE;
+    long long total_pts = 0, total_chains = 0;
+    for (int d = 0; d < nd; d++) {
+        if (n[d] < 1) return fail(GPR_E_ARG, "dataset " + std::to_string(d) + " has no points");
+        if (chains[d] < 1) return fail(GPR_E_ARG, "dataset " + std::to_string(d) + " has no chains");
+        if (caps && !ref && (caps[d] < 1 || caps[d] > c->cfg.cap))
+            return fail(GPR_E_ARG, "per-dataset cap out of [1, cfg.cap]");
+        total_pts += n[d];
+        total_chains += chains[d];
+    }
+    if (total_chains > (1ll << 30)) return fail(GPR_E_ARG, "too many chains");
+    std::vector<float> w((size_t)total_pts);
+    c->hds.assign(nd, GprDataset{});
+    long long off = 0, ch = 0;
+    for (int d = 0; d < nd; d++) {
+        GprDataset &g = c->hds[d];
+        g.n = n[d];
+        g.off = (int)off;
+        g.chain0 = (int)ch;
+        g.nchains = chains[d];
+        g.cap = ref ? 63 : (caps ? caps[d] : c->cfg.cap);
+        g.kc = kc_for(g.n);
+        double lc = 0.0;
+        for (int i = 0; i < n[d]; i++) {
+            const double s = sd[off + i];
+            if (!(s > 0.0) || !std::isfinite(s) || !std::isfinite(x[off + i]) || !std::isfinite(y[off + i]))
+                return fail(GPR_E_ARG, "dataset " + std::to_string(d) + " point " + std::to_string(i) +
+                                           ": need finite x, y and sd > 0 (main.cu:332)");
+            w[off + i] = (float)(1.0 / (2.0 * s * s));
+            lc += -log(s) - GPR_HALF_LOG2PI_D;
+        }
+        g.lconst = lc;
+        off += n[d];
+        ch += chains[d];
+    }
+    c->nd = nd;
+    c->N = (int)total_chains;
+    c->rows = ref ? GPR_REF_SLOTS : c->cfg.cap;
+    // tiles of GPR_WAVES_PER_WG chains of one dataset, grouped by kernel bucket
+    c->htiles.clear();
+    c->buckets.clear();
+    for (int kc : {4, 2, 1})
+        for (int hi = 1; hi >= 0; hi--) {
+            KernelBucket b{kc, hi != 0, (int)c->htiles.size(), 0};
+            // larger datasets first within a bucket: longest-processing-time order
+            std::vector<int> order;
+            for (int d = 0; d < nd; d++)
+                if (c->hds[d].kc == kc && ((c->hds[d].cap > 64) == (hi != 0))) order.push_back(d);
+            std::stable_sort(order.begin(), order.end(), [&](int a, int bb) { return c->hds[a].n > c->hds[bb].n; });
+            for (int d : order)
+                for (int s = 0; s < c->hds[d].nchains; s += GPR_WAVES_PER_WG)
+                    c->htiles.push_back(GprTile{d, c->hds[d].chain0 + s, std::min(GPR_WAVES_PER_WG, c->hds[d].nchains - s), 0});
+            b.ntiles = (int)c->htiles.size() - b.tile0;
+            if (b.ntiles) c->buckets.push_back(b);
+        }
+    const size_t N = (size_t)c->N;
+    ALLOC(c->d_ds, (size_t)nd);
+    ALLOC(c->d_x, (size_t)total_pts);
+    ALLOC(c->d_y, (size_t)total_pts);
+    ALLOC(c->d_w, (size_t)total_pts);
+    ALLOC(c->d_ops, (size_t)c->rows * N);
+    ALLOC(c->d_cs, (size_t)c->rows * N);
+    ALLOC(c->d_len, N);
+    ALLOC(c->d_prior, N);
+    ALLOC(c->d_ll, N);
+    if (!ref && c->cfg.proposal == GPR_PROP_MALA) ALLOC(c->d_grad, (size_t)c->rows * N);
+    ALLOC(c->d_tiles, c->htiles.size());
+    ALLOC(c->d_tile_counter, std::max<size_t>(1, c->buckets.size()));
+    ALLOC(c->d_counters, (size_t)GPR_NCNT);
+    ALLOC(c->d_good, N);
+    ALLOC(c->d_ngood, (size_t)nd);
+    ALLOC(c->d_map_chain, (size_t)nd);
+    ALLOC(c->d_map_post, (size_t)nd);
+    ALLOC(c->d_cutoff, (size_t)nd);
+    ALLOC(c->d_nres, 1);
+    HIP_TRY(hipMemcpy(c->d_ds, c->hds.data(), sizeof(GprDataset) * nd, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_x, x, sizeof(float) * total_pts, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_y, y, sizeof(float) * total_pts, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_w, w.data(), sizeof(float) * total_pts, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_tiles, c->htiles.data(), sizeof(GprTile) * c->htiles.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(c->d_counters, 0, sizeof(unsigned long long) * GPR_NCNT));
+    HIP_TRY(hipMemset(c->d_ops, 0, (size_t)c->rows * N));
+    HIP_TRY(hipMemset(c->d_cs, 0, sizeof(float) * c->rows * N));
+    HIP_TRY(hipMemset(c->d_len, 0, sizeof(int32_t) * N));
+    HIP_TRY(hipMemset(c->d_prior, 0, sizeof(double) * N));
+    HIP_TRY(hipMemset(c->d_ll, 0, sizeof(double) * N));
+    c->step = 0;
+    c->block = 0;
+    c->have_data = true;
+    return GPR_OK;
+}
+
+static GprMHArgs mh_args(gpr_ctx *c, int steps, int32_t block_seed) {
+    GprMHArgs a{};
+    a.ops = c->d_ops; a.cs = c->d_cs; a.len = c->d_len; a.prior = c->d_prior; a.ll = c->d_ll; a.grad = c->d_grad;
+    a.nchains = c->N;
+    a.dx = c->d_x; a.dy = c->d_y; a.dw = c->d_w;
+    a.ds = c->d_ds;
+    a.tiles = c->d_tiles;
+    a.ntiles = (int)c->htiles.size();
+    a.tile_counter = c->d_tile_counter;
+    a.steps = steps;
+    a.step0 = c->step;
+    a.seed = c->cfg.seed;
+    a.dmax = c->cfg.dmax;
+    a.lnsym = log((double)c->cfg.nsym);
+    a.sigma = c->cfg.sigma;
+    a.eps = c->cfg.eps;
+    a.struct_every = c->cfg.struct_every;
+    a.fixed_structure = c->cfg.fixed_structure;
+    a.counters = c->d_counters;
+    a.block_seed = block_seed;
+    a.rng = c->cfg.rng;
+    return a;
+}
+
+extern "C" int gpr_init_chains(gpr_ctx *c) {
+    if (!c || !c->have_data) return fail(GPR_E_STATE, "gpr_set_data must come first");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    GprMHArgs a = mh_args(c, 0, 0);
+    if (c->cfg.semantics == GPR_SEM_CONFIG) {
+        HIP_TRY(gpr_launch_init_config(a, c->stream));
+    } else {
+        return fail(GPR_E_UNSUPPORTED, "reference-semantics init not built yet");
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return GPR_OK;
+}
+
+extern "C" int gpr_write_state(gpr_ctx *c, const uint8_t *ops, const float *cs, const int32_t *len) {
+    if (!c || !c->have_data) return fail(GPR_E_STATE, "gpr_set_data must come first");
+    if (!ops || !cs) return fail(GPR_E_ARG, "null state arrays");
+    const bool ref = c->cfg.semantics == GPR_SEM_REFERENCE;
+    const size_t N = (size_t)c->N;
+    if (!ref) {
+        if (!len) return fail(GPR_E_ARG, "null len");
+        for (size_t i = 0; i < N; i++) {
+            int ds = 0;
+            while (ds + 1 < c->nd && c->hds[ds + 1].chain0 <= (int)i) ds++;
+            const int L = len[i];
+            if (L < 1 || L > c->hds[ds].cap) return fail(GPR_E_ARG, "chain " + std::to_string(i) + ": len out of [1, cap]");
+            int depth = 0, dm = 0;
+            for (int t = 0; t < L; t++) {
+                const int o = ops[(size_t)t * N + i];
+                if (o >= GPR_NUM_OPS) return fail(GPR_E_ARG, "chain " + std::to_string(i) + ": bad op code");
+                const int ar = gpr_arity(o);
+                if (depth < ar) return fail(GPR_E_ARG, "chain " + std::to_string(i) + ": malformed postfix");
+                depth += 1 - ar;
+                dm = std::max(dm, depth);
+            }
+            if (depth != 1) return fail(GPR_E_ARG, "chain " + std::to_string(i) + ": malformed postfix");
+            if (dm > c->cfg.dmax) return fail(GPR_E_ARG, "chain " + std::to_string(i) + ": stack depth above dmax");
+        }
+    } else {
+        for (size_t i = 0; i < N * GPR_REF_SLOTS; i++)
+            if (ops[i] >= GPR_REF_NOPS) return fail(GPR_E_ARG, "bad heap op code");
+    }
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    HIP_TRY(hipMemcpyAsync(c->d_ops, ops, (size_t)c->rows * N, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_cs, cs, sizeof(float) * c->rows * N, hipMemcpyHostToDevice, c->stream));
+    if (!ref) HIP_TRY(hipMemcpyAsync(c->d_len, len, sizeof(int32_t) * N, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return GPR_OK;
+}
+
+extern "C" int gpr_read_state(gpr_ctx *c, uint8_t *ops, float *cs, int32_t *len, double *prior, double *ll) {
+    if (!c || !c->have_data) return fail(GPR_E_STATE, "no state");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    const size_t N = (size_t)c->N;
+    if (ops) HIP_TRY(hipMemcpyAsync(ops, c->d_ops, (size_t)c->rows * N, hipMemcpyDeviceToHost, c->stream));
+    if (cs) HIP_TRY(hipMemcpyAsync(cs, c->d_cs, sizeof(float) * c->rows * N, hipMemcpyDeviceToHost, c->stream));
+    if (len) HIP_TRY(hipMemcpyAsync(len, c->d_len, sizeof(int32_t) * N, hipMemcpyDeviceToHost, c->stream));
+    if (prior) HIP_TRY(hipMemcpyAsync(prior, c->d_prior, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+    if (ll) HIP_TRY(hipMemcpyAsync(ll, c->d_ll, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
+}
+
+static int launch_block(gpr_ctx *c, int steps, int32_t block_seed) {
+    GprMHArgs a = mh_args(c, steps, block_seed);
+    const bool ref = c->cfg.semantics == GPR_SEM_REFERENCE;
+    int sched = c->cfg.schedule == GPR_SCHED_AUTO ? GPR_SCHED_WAVE : c->cfg.schedule;
+    if (ref) return fail(GPR_E_UNSUPPORTED, "reference-semantics MH kernel not built yet");
+    if (sched == GPR_SCHED_LANE) return fail(GPR_E_UNSUPPORTED, "lane schedule MH kernel not built yet");
+    HIP_TRY(gpr_launch_fill_i32(c->d_tile_counter, (int)c->buckets.size(), 0, c->stream));
+    for (size_t b = 0; b < c->buckets.size(); b++) {
+        const KernelBucket &kb = c->buckets[b];
+        GprMHArgs ab = a;
+        ab.tiles = c->d_tiles + kb.tile0;
+        ab.ntiles = kb.ntiles;
+        ab.tile_counter = c->d_tile_counter + b;
+        const int grid = std::min(kb.ntiles, c->cus * 8);
+        HIP_TRY(gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, grid, c->stream));
+    }
+    c->sched_used = sched;
+    c->step += (uint32_t)steps;
+    c->block++;
+    return GPR_OK;
+}
+
+extern "C" int gpr_run_block_async(gpr_ctx *c, int steps, int32_t block_seed) {
+    if (!c || !c->have_state) return fail(GPR_E_STATE, "no chain state: call gpr_init_chains or gpr_write_state");
+    if (steps < 1) return fail(GPR_E_ARG, "steps must be > 0 (main.cu:1028)");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    return launch_block(c, steps, block_seed);
+}
+
+extern "C" int gpr_run_block(gpr_ctx *c, int steps, int32_t block_seed) {
+    if (!c || !c->have_state) return fail(GPR_E_STATE, "no chain state: call gpr_init_chains or gpr_write_state");
+    if (steps < 1) return fail(GPR_E_ARG, "steps must be > 0 (main.cu:1028)");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    int r = launch_block(c, steps, block_seed);
+    if (r) return r;
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return GPR_OK;
+}
+
+static GprResampleArgs rs_args(gpr_ctx *c) {
+    GprResampleArgs r{};
+    r.ops = c->d_ops; r.cs = c->d_cs;
+    r.len = c->cfg.semantics == GPR_SEM_CONFIG ? c->d_len : nullptr;
+    r.prior = c->d_prior; r.ll = c->d_ll; r.grad = c->d_grad;
+    r.nchains = c->N;
+    r.cap_rows = c->rows;
+    r.ds = c->d_ds;
+    r.ndatasets = c->nd;
+    r.good = c->d_good; r.ngood = c->d_ngood; r.map_chain = c->d_map_chain; r.map_post = c->d_map_post;
+    r.cutoff = c->d_cutoff;
+    r.seed = c->cfg.seed;
+    r.block = c->block;
+    r.nats = c->cfg.resample_nats;
+    r.quantile = c->cfg.resample_quantile;
+    r.semantics = c->cfg.semantics;
+    r.nresampled = c->d_nres;
+    return r;
+}
+
+extern "C" int gpr_resample(gpr_ctx *c) {
+    if (!c || !c->have_state) return fail(GPR_E_STATE, "no chain state");
+    if (c->cfg.resample == GPR_RS_NONE) return GPR_OK;
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    if (c->cfg.resample != GPR_RS_MAP) return fail(GPR_E_UNSUPPORTED, "quantile resampling not built yet");
+    HIP_TRY(gpr_launch_fill_i32(c->d_nres, 1, 0, c->stream));
+    HIP_TRY(gpr_launch_resample(rs_args(c), c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
+}
+
+extern "C" int gpr_run_blocks_graph(gpr_ctx *c, int nblocks, int steps) {
+    return fail(GPR_E_UNSUPPORTED, "graph capture not built yet");
+}
+
+extern "C" int gpr_get_stats(const gpr_ctx *c, gpr_stats *o) {
+    if (!c || !o) return fail(GPR_E_ARG, "null argument");
+    memset(o, 0, sizeof(*o));
+    if (!c->have_data) return GPR_OK;
+    unsigned long long h[GPR_NCNT];
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    HIP_TRY(hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    int32_t mc = -1;
+    double mp = 0.0;
+    HIP_TRY(hipMemcpyAsync(&mc, c->d_map_chain, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&mp, c->d_map_post, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    o->struct_proposed = (long long)h[GPR_CNT_SPROP];
+    o->struct_accepted = (long long)h[GPR_CNT_SACC];
+    o->const_proposed = (long long)h[GPR_CNT_CPROP];
+    o->const_accepted = (long long)h[GPR_CNT_CACC];
+    o->node_evals = (long long)h[GPR_CNT_EVALS];
+    o->steps = (long long)h[GPR_CNT_STEPS];
+    o->last_block_ms = c->last_ms;
+    o->schedule_used = c->sched_used;
+    o->map_chain = mc;
+    o->map_posterior = mp;
+    return GPR_OK;
+}
+
+extern "C" int gpr_reset_stats(gpr_ctx *c) {
+    if (!c || !c->have_data) return GPR_OK;
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    HIP_TRY(hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * GPR_NCNT, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
+}
+
+extern "C" int gpr_sync(gpr_ctx *c) {
+    if (!c) return fail(GPR_E_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
+}
+
+extern "C" void *gpr_stream(gpr_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int gpr_eval_loglik(gpr_ctx *c, int dataset, const uint8_t *d_ops, const float *d_cs, const int32_t *d_len,
+                               int nchains, int cap, int schedule, double *d_ll) {
+    if (!c || !c->have_data) return fail(GPR_E_STATE, "gpr_set_data must come first");
+    if (dataset < 0 || dataset >= c->nd) return fail(GPR_E_ARG, "dataset index out of range");
+    if (!d_ops || !d_cs || !d_len || !d_ll || nchains < 1) return fail(GPR_E_ARG, "null or empty tape arrays");
+    if (cap < 1 || cap > GPR_MAX_CAP) return fail(GPR_E_ARG, "cap must be in [1, 128]");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    GprEvalArgs a{};
+    a.ops = d_ops; a.cs = d_cs; a.len = d_len; a.nchains = nchains; a.cap = cap;
+    a.dx = c->d_x; a.dy = c->d_y; a.dw = c->d_w;
+    a.d = c->hds[dataset];
+    a.ll = d_ll;
+    a.lnsym = log((double)c->cfg.nsym);
+    if (schedule == GPR_SCHED_LANE) HIP_TRY(gpr_launch_eval_lane(a, c->stream));
+    else HIP_TRY(gpr_launch_eval_wave(a, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
+}
+
+extern "C" int gpr_grad_logpost(gpr_ctx *c, int dataset, const uint8_t *d_ops, const float *d_cs,
+                                const int32_t *d_len, int nchains, int cap, float *d_grad, double *d_lp) {
+    return fail(GPR_E_UNSUPPORTED, "adjoint kernel not built yet");
+}
+
+// heap -> postfix of the reachable tree (host), main.cu:371-393 semantics
+static const int8_t kRefToTape[GPR_REF_NOPS] = {
+    GPR_OP_ONE, GPR_OP_CONST, GPR_OP_INV, -1, -1, GPR_OP_ADD, GPR_OP_SUB, GPR_OP_RSUB, GPR_OP_CADD, GPR_OP_MUL,
+    GPR_OP_CMUL, GPR_OP_DIV, GPR_OP_RDIV, GPR_OP_LOG, GPR_OP_EXP, GPR_OP_POW, GPR_OP_CPOW, GPR_OP_RPOW, GPR_OP_CRPOW,
+    GPR_OP_SQRT, GPR_OP_SIN, GPR_OP_ASIN, GPR_OP_COS, GPR_OP_ACOS, GPR_OP_TAN, GPR_OP_ATAN, GPR_OP_ABS};
+
+static int decode_heap(const uint8_t *hop, const float *hc, size_t stride, int slot, int pos, uint8_t *ops, float *cs) {
+    if (slot > GPR_REF_SLOTS) { ops[pos] = GPR_OP_X; cs[pos] = 0.0f; return pos + 1; }
+    const int o = hop[(size_t)(slot - 1) * stride];
+    if (o >= GPR_REF_NOPS) return -1;
+    if (o == GPR_REF_A) return decode_heap(hop, hc, stride, 2 * slot, pos, ops, cs);
+    if (o == GPR_REF_B) return decode_heap(hop, hc, stride, 2 * slot + 1, pos, ops, cs);
+    const int t = kRefToTape[o];
+    const int ar = gpr_arity(t);
+    if (ar >= 1) pos = decode_heap(hop, hc, stride, 2 * slot, pos, ops, cs);
+    if (pos < 0) return pos;
+    if (ar == 2) pos = decode_heap(hop, hc, stride, 2 * slot + 1, pos, ops, cs);
+    if (pos < 0) return pos;
+    ops[pos] = (uint8_t)t;
+    cs[pos] = gpr_has_const(t) ? hc[(size_t)(slot - 1) * stride] : 0.0f;
+    return pos + 1;
+}
+
+extern "C" int gpr_ref_decode(const uint8_t *hop, const float *hc, size_t stride, uint8_t *ops, float *cs) {
+    if (!hop || !hc || !ops || !cs || stride < 1) return fail(GPR_E_ARG, "null argument");
+    const int n = decode_heap(hop, hc, stride, 1, 0, ops, cs);
+    if (n < 0) return fail(GPR_E_ARG, "bad heap op code");
+    return n;
+}

@@ -1,0 +1,107 @@
+// Internal structures shared by the kernels (gpr_kernels.hip) and the C ABI
+// (gpr_api.hip).  Not installed.
+#pragma once
+#include "gpr_common.h"
+
+// One dataset as the kernels see it.  x/y/w are offsets into the concatenated device
+// arrays; w = 1/(2 sd^2); lconst = sum_i (-log sd_i - log(2 pi)/2) in FP64.
+struct GprDataset {
+    int n;          // points
+    int off;        // first point in the concatenated arrays
+    int chain0;     // first chain
+    int nchains;    // chains of this dataset
+    int cap;        // token cap of this dataset's programs
+    int kc;         // points per lane per chunk chosen for the wave schedule
+    double lconst;
+};
+
+// A unit of work for the wave kernels: up to GPR_WAVES_PER_WG chains of one dataset.
+struct GprTile {
+    int dataset;
+    int chain0;
+    int count;
+    int pad;
+};
+
+#define GPR_WAVES_PER_WG 4
+#define GPR_WG 256
+
+// Counters accumulated by the MH kernels (device, u64).
+enum { GPR_CNT_SPROP = 0, GPR_CNT_SACC, GPR_CNT_CPROP, GPR_CNT_CACC, GPR_CNT_EVALS, GPR_CNT_STEPS, GPR_NCNT };
+
+struct GprMHArgs {
+    uint8_t *ops;           // [cap][N]
+    float *cs;              // [cap][N]
+    int32_t *len;           // [N]
+    double *prior;          // [N]
+    double *ll;             // [N]
+    float *grad;            // [cap][N] MALA gradient cache (may be null)
+    int nchains;            // N, the stride
+    const float *dx, *dy, *dw;
+    const GprDataset *ds;
+    const GprTile *tiles;
+    int ntiles;
+    int *tile_counter;      // persistent work queue (reset by the host before launch)
+    int steps;
+    uint32_t step0;
+    uint64_t seed;
+    int dmax;
+    double lnsym;
+    float sigma;
+    float eps;
+    int struct_every;
+    int fixed_structure;
+    unsigned long long *counters;
+    int32_t block_seed;
+    int rng;
+};
+
+struct GprEvalArgs {
+    const uint8_t *ops;
+    const float *cs;
+    const int32_t *len;
+    int nchains;
+    int cap;
+    const float *dx, *dy, *dw;
+    GprDataset d;
+    double *ll;
+    float *grad;            // gradient output (grad kernel)
+    double lnsym;
+};
+
+struct GprResampleArgs {
+    uint8_t *ops;
+    float *cs;
+    int32_t *len;
+    double *prior;
+    double *ll;
+    float *grad;
+    int nchains;
+    int cap_rows;           // rows of ops/cs to copy
+    const GprDataset *ds;
+    int ndatasets;
+    int32_t *good;          // [N] scratch: good chains per dataset, packed at chain0
+    int32_t *ngood;         // [ndatasets]
+    int32_t *map_chain;     // [ndatasets]
+    double *map_post;       // [ndatasets]
+    double *cutoff;         // [ndatasets] quantile rule
+    uint64_t seed;
+    uint32_t block;
+    double nats;
+    float quantile;
+    int semantics;
+    int32_t *nresampled;    // [1]
+};
+
+// launchers (gpr_kernels.hip)
+hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, int grid, hipStream_t s);
+hipError_t gpr_launch_mh_config_lane(const GprMHArgs &a, int cap_max, hipStream_t s);
+hipError_t gpr_launch_mh_ref_lane(const GprMHArgs &a, hipStream_t s);
+hipError_t gpr_launch_mh_ref_wave(const GprMHArgs &a, int kc, int grid, hipStream_t s);
+hipError_t gpr_launch_eval_wave(const GprEvalArgs &a, hipStream_t s);
+hipError_t gpr_launch_eval_lane(const GprEvalArgs &a, hipStream_t s);
+hipError_t gpr_launch_grad_wave(const GprEvalArgs &a, hipStream_t s);
+hipError_t gpr_launch_init_config(const GprMHArgs &a, hipStream_t s);
+hipError_t gpr_launch_init_ref(const GprMHArgs &a, hipStream_t s);
+hipError_t gpr_launch_resample(const GprResampleArgs &a, hipStream_t s);
+hipError_t gpr_launch_fill_i32(int *p, int n, int v, hipStream_t s);

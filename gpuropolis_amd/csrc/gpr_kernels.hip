@@ -1,0 +1,729 @@
+// HIP kernels of the GPUropolis MI355X sampler (gfx950, wave64).
+//
+// Hot path: the fused MH step (MH_simple_kernel, main.cu:679-813, of the reference)
+// with its evaluator and Gaussian log-likelihood (main.cu:371-468), prior
+// (main.cu:529-544) and accept test, plus the resampling between blocks
+// (main.cu:1215-1234), here on the device.
+//
+// Two evaluator schedules:
+//   wave  one wavefront per chain.  The program is wave-uniform: lane t holds token
+//         t (and t+64) of the tape, the interpreter reads each op with v_readlane into
+//         a scalar register and branches on it without divergence, and the 64 lanes
+//         run over data points, KC points per lane per chunk, with a per-lane value
+//         stack in VGPRs indexed by the uniform stack pointer.  Per-point terms are
+//         Kahan-summed per lane and tree-reduced across the wave.
+//   lane  one lane per chain, as the reference.  Every lane interprets its own tape;
+//         the tape and the value stack live in LDS ([slot][lane], conflict-free), data
+//         points are broadcast from LDS.
+#include "gpr_internal.h"
+
+#define GPR_INF (__builtin_inf())
+
+// ---------------------------------------------------------------- wave primitives
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rl_f(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double rl_d(double v, int l) {
+    uint64_t b = __builtin_bit_cast(uint64_t, v);
+    int lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    int hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ int wl_i(int old, int v, int l) { return lane_id() == l ? v : old; }
+__device__ __forceinline__ float wl_f(float old, float v, int l) { return lane_id() == l ? v : old; }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_scan_i(int v) {  // inclusive
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(v, o);
+        if (l >= o) v += u;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------- op semantics (FP32)
+// Values of dispatch_eval (main.cu:397-444) with IEEE division and accurate libm.
+__device__ __forceinline__ float op_bin(int op, float a, float b) {
+    switch (op) {
+    case GPR_OP_ADD: return a + b;
+    case GPR_OP_SUB: return a - b;
+    case GPR_OP_MUL: return a * b;
+    case GPR_OP_DIV: return a / b;
+    case GPR_OP_POW: return powf(a, b);
+    case GPR_OP_RSUB: return b - a;
+    case GPR_OP_RDIV: return b / a;
+    default: return powf(b, a);  // RPOW
+    }
+}
+__device__ __forceinline__ float op_un(int op, float a, float c) {
+    switch (op) {
+    case GPR_OP_NEG: return -a;
+    case GPR_OP_EXP: return expf(a);
+    case GPR_OP_LOG: return logf(a);
+    case GPR_OP_INV: return 1.0f / a;
+    case GPR_OP_SQRT: return sqrtf(a);
+    case GPR_OP_SIN: return sinf(a);
+    case GPR_OP_ASIN: return asinf(a);
+    case GPR_OP_COS: return cosf(a);
+    case GPR_OP_ACOS: return acosf(a);
+    case GPR_OP_TAN: return tanf(a);
+    case GPR_OP_ATAN: return atanf(a);
+    case GPR_OP_ABS: return fabsf(a);
+    case GPR_OP_CADD: return a + c;
+    case GPR_OP_CMUL: return c * a;
+    case GPR_OP_CPOW: return powf(a, c);
+    case GPR_OP_CRPOW: return powf(c, a);
+    default: return __builtin_nanf("");
+    }
+}
+
+// ---------------------------------------------------------------- wave program state
+// Lane l holds token l in (o0, c0) and token l+64 in (o1, c1); tokens >= len are 0.
+struct WaveProg {
+    int o0, o1;
+    float c0, c1;
+    int len;
+};
+
+// Per-lane value stack: KC vector values of GPR_MAX_DEPTH floats.  Element access
+// with the uniform stack pointer lowers to s_set_gpr_idx moves; nothing goes through
+// an alloca, so nothing is spilled to scratch.
+typedef float gpr_f16v __attribute__((ext_vector_type(GPR_MAX_DEPTH)));
+template <int KC>
+struct VStack {
+    gpr_f16v s0, s1, s2, s3;
+    __device__ __forceinline__ float get(int k, int i) const {
+        return k == 0 ? s0[i] : (k == 1 ? s1[i] : (k == 2 ? s2[i] : s3[i]));
+    }
+    __device__ __forceinline__ void put(int k, int i, float v) {
+        if (k == 0) s0[i] = v; else if (k == 1) s1[i] = v; else if (k == 2) s2[i] = v; else s3[i] = v;
+    }
+};
+
+#define GPR_KLOOP(expr)                              \
+    do {                                             \
+        _Pragma("unroll") for (int k = 0; k < KC; k++) T[k] = (expr); \
+    } while (0)
+
+// Interpret the tape at KC points per lane.  op is wave-uniform: every branch below is
+// a scalar branch and only the selected operation executes.
+template <int KC, bool HI>
+__device__ __forceinline__ void wave_eval(const WaveProg &p, const float (&xs)[KC], float (&T)[KC]) {
+    VStack<KC> S;
+    int sp = 0;
+#pragma unroll
+    for (int k = 0; k < KC; k++) T[k] = 0.0f;
+    for (int t = 0; t < p.len; t++) {
+        const int l = t & 63;
+        const int op = (HI && t >= 64) ? rl_i(p.o1, l) : rl_i(p.o0, l);
+        if (op <= GPR_OP_ONE) {
+            // the write index is bounded for malformed tapes; valid ones (depth <= 16) never hit it
+            const int wi = min(sp, GPR_MAX_DEPTH - 1);
+#pragma unroll
+            for (int k = 0; k < KC; k++) S.put(k, wi, T[k]);
+            sp = min(sp + 1, GPR_MAX_DEPTH);
+            if (op == GPR_OP_X) {
+                GPR_KLOOP(xs[k]);
+            } else {
+                const float c = op == GPR_OP_CONST ? ((HI && t >= 64) ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 1.0f;
+                GPR_KLOOP(c);
+            }
+            continue;
+        }
+        if (op <= GPR_OP_RPOW) {
+            sp = max(sp - 1, 0);
+            switch (op) {
+            case GPR_OP_ADD: GPR_KLOOP(S.get(k, sp) + T[k]); break;
+            case GPR_OP_SUB: GPR_KLOOP(S.get(k, sp) - T[k]); break;
+            case GPR_OP_MUL: GPR_KLOOP(S.get(k, sp) * T[k]); break;
+            case GPR_OP_DIV: GPR_KLOOP(S.get(k, sp) / T[k]); break;
+            case GPR_OP_RSUB: GPR_KLOOP(T[k] - S.get(k, sp)); break;
+            case GPR_OP_RDIV: GPR_KLOOP(T[k] / S.get(k, sp)); break;
+            case GPR_OP_POW: GPR_KLOOP(powf(S.get(k, sp), T[k])); break;
+            default: GPR_KLOOP(powf(T[k], S.get(k, sp))); break;
+            }
+            continue;
+        }
+        const float c = op >= GPR_OP_CADD ? ((HI && t >= 64) ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 0.0f;
+        switch (op) {
+        case GPR_OP_NEG: GPR_KLOOP(-T[k]); break;
+        case GPR_OP_EXP: GPR_KLOOP(expf(T[k])); break;
+        case GPR_OP_LOG: GPR_KLOOP(logf(T[k])); break;
+        case GPR_OP_INV: GPR_KLOOP(1.0f / T[k]); break;
+        case GPR_OP_SQRT: GPR_KLOOP(sqrtf(T[k])); break;
+        case GPR_OP_SIN: GPR_KLOOP(sinf(T[k])); break;
+        case GPR_OP_ASIN: GPR_KLOOP(asinf(T[k])); break;
+        case GPR_OP_COS: GPR_KLOOP(cosf(T[k])); break;
+        case GPR_OP_ACOS: GPR_KLOOP(acosf(T[k])); break;
+        case GPR_OP_TAN: GPR_KLOOP(tanf(T[k])); break;
+        case GPR_OP_ATAN: GPR_KLOOP(atanf(T[k])); break;
+        case GPR_OP_ABS: GPR_KLOOP(fabsf(T[k])); break;
+        case GPR_OP_CADD: GPR_KLOOP(T[k] + c); break;
+        case GPR_OP_CMUL: GPR_KLOOP(c * T[k]); break;
+        case GPR_OP_CPOW: GPR_KLOOP(powf(T[k], c)); break;
+        case GPR_OP_CRPOW: GPR_KLOOP(powf(c, T[k])); break;
+        default: GPR_KLOOP(__builtin_nanf("")); break;
+        }
+    }
+}
+
+// Dataset points of one chunk: point index base + k*64 + lane.
+template <int KC>
+struct Chunk {
+    float x[KC], y[KC], w[KC];
+    __device__ __forceinline__ void load(const float *dx, const float *dy, const float *dw, int n, int base) {
+        const int l = lane_id();
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            const int i = base + k * 64 + l;
+            const bool v = i < n;
+            x[k] = v ? dx[i] : 0.0f;
+            y[k] = v ? dy[i] : 0.0f;
+            w[k] = v ? dw[i] : 0.0f;
+        }
+    }
+};
+
+// Gaussian log-likelihood of the wave's program: lconst - sum_i (f_i - y_i)^2 w_i.
+// Per-lane Kahan sums, wave tree reduction; -inf if any term is not finite.
+template <int KC, bool HI>
+__device__ double wave_loglik(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
+                              const float *dw, const Chunk<KC> &c0, int nchunks) {
+    float acc = 0.0f, comp = 0.0f;
+    const int l = lane_id();
+    for (int ch = 0; ch < nchunks; ch++) {
+        Chunk<KC> c;
+        if (nchunks == 1) c = c0; else c.load(dx, dy, dw, d.n, ch * 64 * KC);
+        float T[KC];
+        wave_eval<KC, HI>(p, c.x, T);
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            const float r = T[k] - c.y[k];
+            const float term = (ch * 64 * KC + k * 64 + l < d.n) ? (r * r) * c.w[k] : 0.0f;
+            const float yv = term - comp;
+            const float tv = acc + yv;
+            comp = (tv - acc) - yv;
+            acc = tv;
+        }
+    }
+    const float sa = wave_sum_f(acc);
+    const float sc = wave_sum_f(comp);
+    if (!__builtin_isfinite(sa) || !__builtin_isfinite(sc)) return -GPR_INF;
+    return d.lconst - ((double)sa - (double)sc);
+}
+
+// Description-length prior plus Cauchy(0,1) on every constant token, summed in token
+// order in FP64 (the same order as the oracle, so the value is bit-reproducible).
+template <bool HI>
+__device__ double wave_prior(const WaveProg &p, double lnsym) {
+    const int l = lane_id();
+    const bool v0 = l < p.len, v1 = HI && (l + 64) < p.len;
+    int dl = p.len + __popcll(ballot(v0 && p.o0 >= GPR_OP_CADD)) + (HI ? __popcll(ballot(v1 && p.o1 >= GPR_OP_CADD)) : 0);
+    uint64_t m0 = ballot(v0 && gpr_has_const(p.o0));
+    const double lc0 = gpr_lcauchy(p.c0);
+    double s = 0.0;
+    while (m0) {
+        const int b = __builtin_ctzll(m0);
+        m0 &= m0 - 1;
+        s += rl_d(lc0, b);
+    }
+    if (HI) {
+        uint64_t m1 = ballot(v1 && gpr_has_const(p.o1));
+        if (m1) {
+            const double lc1 = gpr_lcauchy(p.c1);
+            while (m1) {
+                const int b = __builtin_ctzll(m1);
+                m1 &= m1 - 1;
+                s += rl_d(lc1, b);
+            }
+        }
+    }
+    return gpr_dl_prior(dl, lnsym, s);
+}
+
+// Subtree [s0, j] ending at token j: s0-1 is the last u < j whose stack depth after u is
+// below the depth after j.  Also returns the maximum depth of the tape.
+template <bool HI>
+__device__ __forceinline__ void wave_depths(const WaveProg &p, int &D0, int &D1, int &dmax) {
+    const int l = lane_id();
+    const int d0 = l < p.len ? 1 - gpr_arity(p.o0) : 0;
+    D0 = wave_scan_i(d0);
+    dmax = wave_max_i(l < p.len ? D0 : 0);
+    D1 = 0;
+    if (HI) {
+        const int tot = rl_i(D0, 63);
+        const int d1 = l + 64 < p.len ? 1 - gpr_arity(p.o1) : 0;
+        D1 = wave_scan_i(d1) + tot;
+        dmax = max(dmax, wave_max_i(l + 64 < p.len ? D1 : 0));
+    }
+}
+
+template <bool HI>
+__device__ __forceinline__ int subtree_start(int j, int D0, int D1) {
+    const int l = lane_id();
+    const int Dj = (HI && j >= 64) ? rl_i(D1, j - 64) : rl_i(D0, j);
+    uint64_t m1 = HI ? ballot(l + 64 < j && D1 < Dj) : 0;
+    if (m1) return 64 + (63 - __builtin_clzll(m1)) + 1;
+    uint64_t m0 = ballot(l < j && D0 < Dj);
+    if (m0) return (63 - __builtin_clzll(m0)) + 1;
+    return 0;
+}
+
+// value of a 128-token lane array at token index src (src may differ per lane)
+__device__ __forceinline__ int gather_i(int v0, int v1, int src, bool hi) {
+    const int a = __shfl(v0, src & 63);
+    if (!hi) return a;
+    const int b = __shfl(v1, src & 63);
+    return src >= 64 ? b : a;
+}
+__device__ __forceinline__ float gather_f(float v0, float v1, int src, bool hi) {
+    const float a = __shfl(v0, src & 63);
+    if (!hi) return a;
+    const float b = __shfl(v1, src & 63);
+    return src >= 64 ? b : a;
+}
+
+// config accept rule: invalid current always moves; log u < a otherwise
+__device__ __forceinline__ bool cfg_accept(uint64_t seed, uint32_t chain, uint32_t step, double cur, double prop,
+                                           double a) {
+    if (!__builtin_isfinite(cur)) return true;
+    if (!__builtin_isfinite(prop)) return false;
+    if (a >= 0.0) return true;
+    const float u = gpr_u01(gpr_first(seed, chain, step, GPR_P_ACCEPT, 0));
+    return (double)gpr_logf(u) < a;
+}
+
+// ---------------------------------------------------------------- config MH, wave schedule
+template <int KC, bool HI>
+__global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
+    const int wave = uni((int)(threadIdx.x >> 6));
+    const int l = lane_id();
+    const int N = a.nchains;
+    unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0, n_evals = 0, n_steps = 0;
+    for (;;) {
+        int tile_idx = 0;
+        if (threadIdx.x == 0) tile_idx = atomicAdd(a.tile_counter, 1);
+        tile_idx = __shfl(tile_idx, 0);
+        __shared__ int s_tile;
+        if (threadIdx.x == 0) s_tile = tile_idx;
+        __syncthreads();
+        tile_idx = uni(s_tile);
+        __syncthreads();
+        if (tile_idx >= a.ntiles) break;
+        const GprTile tile = a.tiles[tile_idx];
+        if (wave >= tile.count) continue;
+        const GprDataset d = a.ds[tile.dataset];
+        const uint32_t chain = (uint32_t)uni(tile.chain0 + wave);
+        const int cap = d.cap;
+        const float *dx = a.dx + d.off, *dy = a.dy + d.off, *dw = a.dw + d.off;
+        const int nchunks = (d.n + 64 * KC - 1) / (64 * KC);
+
+        WaveProg cur;
+        cur.len = uni(a.len[chain]);
+        cur.o0 = l < cur.len ? a.ops[(size_t)l * N + chain] : 0;
+        cur.c0 = l < cur.len ? a.cs[(size_t)l * N + chain] : 0.0f;
+        cur.o1 = (HI && l + 64 < cur.len) ? a.ops[(size_t)(l + 64) * N + chain] : 0;
+        cur.c1 = (HI && l + 64 < cur.len) ? a.cs[(size_t)(l + 64) * N + chain] : 0.0f;
+
+        Chunk<KC> c0;
+        c0.load(dx, dy, dw, d.n, 0);
+
+        // iteration -1 evaluates the current state (main.cu:683-684); the prior and
+        // likelihood have a single call site so the kernel body stays small
+        double prior = 0.0, ll = 0.0;
+        for (int i = -1; i < a.steps; i++) {
+            const uint32_t step = a.step0 + (uint32_t)i;
+            const double post = prior + ll;
+            WaveProg prop = cur;
+            double extra = 0.0;
+            const bool structural = i >= 0 && !a.fixed_structure && (step % (uint32_t)a.struct_every) == 0;
+            if (i < 0) {
+            } else if (structural) {
+                n_sprop++;
+                gpr_rstream st;
+                st.init(a.seed, chain, step, GPR_P_STRUCT, 0);
+                const uint32_t r = st.next();
+                const int j = __builtin_isfinite(post) ? (int)(((uint64_t)r * (uint64_t)cur.len) >> 32) : cur.len - 1;
+                int D0, D1, dm;
+                wave_depths<HI>(cur, D0, D1, dm);
+                const int s0 = subtree_start<HI>(j, D0, D1);
+                const int oldsize = j - s0 + 1;
+                const int maxnew = cap - (cur.len - oldsize);
+                // generate in right-child-first prefix order; postfix is the reverse
+                int g0 = 0, g1 = 0;
+                float gc0 = 0.0f, gc1 = 0.0f;
+                int need = 1, m = 0;
+                double lq_new = 0.0;
+                bool fail = false;
+                while (need > 0) {
+                    if (m >= maxnew) { fail = true; break; }
+                    const int s = gpr_gen_symbol(st.next());
+                    lq_new += gpr_gen_logp(s);
+                    float cv = 0.0f;
+                    if (s == GPR_OP_CONST) {
+                        cv = gpr_normal(gpr_u01(st.next()));
+                        lq_new += gpr_lphi(cv);
+                    }
+                    if (!HI || m < 64) { g0 = wl_i(g0, s, m & 63); gc0 = wl_f(gc0, cv, m & 63); }
+                    else { g1 = wl_i(g1, s, m & 63); gc1 = wl_f(gc1, cv, m & 63); }
+                    m++;
+                    need += gpr_arity(s) - 1;
+                }
+                if (fail) continue;
+                const int plen = cur.len - oldsize + m;
+                // splice: [0,s0) from cur, [s0, s0+m) reversed generation, rest from cur
+#pragma unroll
+                for (int h = 0; h < (HI ? 2 : 1); h++) {
+                    const int t = l + 64 * h;
+                    int src;
+                    bool from_gen;
+                    if (t < s0) { src = t; from_gen = false; }
+                    else if (t < s0 + m) { src = m - 1 - (t - s0); from_gen = true; }
+                    else { src = t - m + oldsize; from_gen = false; }
+                    if (src > 127) src = 127;
+                    const int oc = gather_i(cur.o0, cur.o1, src, HI);
+                    const float cc = gather_f(cur.c0, cur.c1, src, HI);
+                    const int og = gather_i(g0, g1, src, HI);
+                    const float cg = gather_f(gc0, gc1, src, HI);
+                    int ov = from_gen ? og : oc;
+                    float cvv = from_gen ? cg : cc;
+                    if (t >= plen) { ov = 0; cvv = 0.0f; }
+                    if (h == 0) { prop.o0 = ov; prop.c0 = cvv; } else { prop.o1 = ov; prop.c1 = cvv; }
+                }
+                prop.len = plen;
+                int P0, P1, pdm;
+                wave_depths<HI>(prop, P0, P1, pdm);
+                if (pdm > a.dmax) continue;
+                double lq_old = 0.0;
+                for (int t = s0; t <= j; t++) {
+                    const int o = (HI && t >= 64) ? rl_i(cur.o1, t & 63) : rl_i(cur.o0, t & 63);
+                    lq_old += gpr_gen_logp(o);
+                    if (o == GPR_OP_CONST) lq_old += gpr_lphi((HI && t >= 64) ? rl_f(cur.c1, t & 63) : rl_f(cur.c0, t & 63));
+                }
+                const double fwd = -(double)gpr_logf((float)cur.len) + lq_new;
+                const double bwd = -(double)gpr_logf((float)plen) + lq_old;
+                extra = bwd - fwd;
+            } else {
+                const bool k0 = l < cur.len && cur.o0 == GPR_OP_CONST;
+                const bool k1 = HI && l + 64 < cur.len && cur.o1 == GPR_OP_CONST;
+                const uint64_t mk0 = ballot(k0), mk1 = HI ? ballot(k1) : 0;
+                if ((mk0 | mk1) == 0) continue;
+                n_cprop++;
+                const gpr_u4 w = gpr_philox(chain, step, ((uint32_t)GPR_P_CPROP << 24) | (uint32_t)l, 0,
+                                            (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+                if (k0) prop.c0 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(w.x)), cur.c0);
+                if (HI && k1) prop.c1 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(w.y)), cur.c1);
+            }
+            const double pprior = wave_prior<HI>(prop, a.lnsym);
+            const double pll = wave_loglik<KC, HI>(prop, d, dx, dy, dw, c0, nchunks);
+            n_evals += (unsigned long long)prop.len * (unsigned long long)d.n;
+            if (i < 0) { prior = pprior; ll = pll; continue; }
+            const double ppost = pprior + pll;
+            if (cfg_accept(a.seed, chain, step, post, ppost, ppost - post + extra)) {
+                cur = prop;
+                prior = pprior;
+                ll = pll;
+                if (structural) n_sacc++; else n_cacc++;
+            }
+        }
+        n_steps += (unsigned long long)a.steps;
+        // every row below the cap: tokens past len are zero
+        if (l < cap) { a.ops[(size_t)l * N + chain] = (uint8_t)cur.o0; a.cs[(size_t)l * N + chain] = cur.c0; }
+        if (HI && l + 64 < cap) {
+            a.ops[(size_t)(l + 64) * N + chain] = (uint8_t)cur.o1;
+            a.cs[(size_t)(l + 64) * N + chain] = cur.c1;
+        }
+        if (l == 0) { a.len[chain] = cur.len; a.prior[chain] = prior; a.ll[chain] = ll; }
+    }
+    if (l == 0 && a.counters) {
+        atomicAdd(&a.counters[GPR_CNT_SPROP], n_sprop);
+        atomicAdd(&a.counters[GPR_CNT_SACC], n_sacc);
+        atomicAdd(&a.counters[GPR_CNT_CPROP], n_cprop);
+        atomicAdd(&a.counters[GPR_CNT_CACC], n_cacc);
+        atomicAdd(&a.counters[GPR_CNT_EVALS], n_evals);
+        atomicAdd(&a.counters[GPR_CNT_STEPS], n_steps);
+    }
+}
+
+hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, int grid, hipStream_t s) {
+#define GPR_L(K, H) hipLaunchKernelGGL((k_mh_config_wave<K, H>), dim3(grid), dim3(GPR_WG), 0, s, a)
+    if (kc == 1) { if (hi) GPR_L(1, true); else GPR_L(1, false); }
+    else if (kc == 2) { if (hi) GPR_L(2, true); else GPR_L(2, false); }
+    else { if (hi) GPR_L(4, true); else GPR_L(4, false); }
+#undef GPR_L
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- standalone evaluator
+template <int KC, bool HI>
+__global__ void __launch_bounds__(GPR_WG) k_eval_wave(GprEvalArgs a) {
+    const int wave = uni((int)(threadIdx.x >> 6));
+    const int l = lane_id();
+    const int chain = uni((int)blockIdx.x * GPR_WAVES_PER_WG + wave);
+    if (chain >= a.nchains) return;
+    const int N = a.nchains;
+    WaveProg p;
+    p.len = uni(a.len[chain]);
+    p.o0 = l < p.len ? a.ops[(size_t)l * N + chain] : 0;
+    p.c0 = l < p.len ? a.cs[(size_t)l * N + chain] : 0.0f;
+    p.o1 = (HI && l + 64 < p.len) ? a.ops[(size_t)(l + 64) * N + chain] : 0;
+    p.c1 = (HI && l + 64 < p.len) ? a.cs[(size_t)(l + 64) * N + chain] : 0.0f;
+    const int nchunks = (a.d.n + 64 * KC - 1) / (64 * KC);
+    Chunk<KC> c0;
+    c0.load(a.dx + a.d.off, a.dy + a.d.off, a.dw + a.d.off, a.d.n, 0);
+    const double ll = wave_loglik<KC, HI>(p, a.d, a.dx + a.d.off, a.dy + a.d.off, a.dw + a.d.off, c0, nchunks);
+    if (l == 0) a.ll[chain] = ll;
+}
+
+hipError_t gpr_launch_eval_wave(const GprEvalArgs &a, hipStream_t s) {
+    const int grid = (a.nchains + GPR_WAVES_PER_WG - 1) / GPR_WAVES_PER_WG;
+    const bool hi = a.cap > 64;
+    const int kc = a.d.kc;
+#define GPR_L(K, H) hipLaunchKernelGGL((k_eval_wave<K, H>), dim3(grid), dim3(GPR_WG), 0, s, a)
+    if (kc == 1) { if (hi) GPR_L(1, true); else GPR_L(1, false); }
+    else if (kc == 2) { if (hi) GPR_L(2, true); else GPR_L(2, false); }
+    else { if (hi) GPR_L(4, true); else GPR_L(4, false); }
+#undef GPR_L
+    return hipGetLastError();
+}
+
+// Lane schedule: one lane per chain; tape staged in LDS [token][lane], value stack in
+// LDS [depth][lane], data points broadcast from LDS in tiles.
+#define GPR_LANE_WG 128
+#define GPR_LANE_DTILE 256
+__global__ void __launch_bounds__(GPR_LANE_WG) k_eval_lane(GprEvalArgs a) {
+    extern __shared__ unsigned char smem[];
+    float *s_stack = (float *)smem;                                   // [GPR_MAX_DEPTH][WG]
+    float *s_c = s_stack + GPR_MAX_DEPTH * GPR_LANE_WG;                // [cap][WG]
+    float *s_x = s_c + a.cap * GPR_LANE_WG;                            // [DTILE]
+    float *s_y = s_x + GPR_LANE_DTILE;
+    float *s_w = s_y + GPR_LANE_DTILE;
+    uint8_t *s_o = (uint8_t *)(s_w + GPR_LANE_DTILE);                  // [cap][WG]
+    const int tid = threadIdx.x;
+    const int chain = blockIdx.x * GPR_LANE_WG + tid;
+    const bool live = chain < a.nchains;
+    const int N = a.nchains;
+    const int len = live ? a.len[chain] : 0;
+    for (int t = 0; t < len; t++) {
+        s_o[t * GPR_LANE_WG + tid] = a.ops[(size_t)t * N + chain];
+        s_c[t * GPR_LANE_WG + tid] = a.cs[(size_t)t * N + chain];
+    }
+    float acc = 0.0f, comp = 0.0f;
+    for (int base = 0; base < a.d.n; base += GPR_LANE_DTILE) {
+        const int cnt = min(GPR_LANE_DTILE, a.d.n - base);
+        __syncthreads();
+        for (int i = tid; i < cnt; i += GPR_LANE_WG) {
+            s_x[i] = a.dx[a.d.off + base + i];
+            s_y[i] = a.dy[a.d.off + base + i];
+            s_w[i] = a.dw[a.d.off + base + i];
+        }
+        __syncthreads();
+        for (int i = 0; i < cnt; i++) {
+            const float x = s_x[i];
+            float T = 0.0f;
+            int sp = 0;
+            for (int t = 0; t < len; t++) {
+                const int op = s_o[t * GPR_LANE_WG + tid];
+                if (op <= GPR_OP_ONE) {
+                    s_stack[min(sp, GPR_MAX_DEPTH - 1) * GPR_LANE_WG + tid] = T;
+                    sp = min(sp + 1, GPR_MAX_DEPTH);
+                    T = op == GPR_OP_X ? x : (op == GPR_OP_CONST ? s_c[t * GPR_LANE_WG + tid] : 1.0f);
+                } else if (op <= GPR_OP_RPOW) {
+                    sp = max(sp - 1, 0);
+                    T = op_bin(op, s_stack[min(sp, GPR_MAX_DEPTH - 1) * GPR_LANE_WG + tid], T);
+                } else {
+                    T = op_un(op, T, s_c[t * GPR_LANE_WG + tid]);
+                }
+            }
+            const float r = T - s_y[i];
+            const float term = (r * r) * s_w[i];
+            const float yv = term - comp;
+            const float tv = acc + yv;
+            comp = (tv - acc) - yv;
+            acc = tv;
+        }
+    }
+    if (live) {
+        double ll = a.d.lconst - ((double)acc - (double)comp);
+        if (!__builtin_isfinite(acc) || !__builtin_isfinite(comp)) ll = -GPR_INF;
+        a.ll[chain] = ll;
+    }
+}
+
+hipError_t gpr_launch_eval_lane(const GprEvalArgs &a, hipStream_t s) {
+    const int grid = (a.nchains + GPR_LANE_WG - 1) / GPR_LANE_WG;
+    const size_t sh = sizeof(float) * (GPR_MAX_DEPTH * GPR_LANE_WG + (size_t)a.cap * GPR_LANE_WG + 3 * GPR_LANE_DTILE) +
+                      (size_t)a.cap * GPR_LANE_WG;
+    if (sh > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_eval_lane, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_eval_lane, dim3(grid), dim3(GPR_LANE_WG), sh, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- config init
+// One thread per chain: draw from the generation distribution until the program fits
+// the cap and depth bound (attempt a uses stream (chain, a, INIT, 0)).
+__global__ void k_init_config(GprMHArgs a) {
+    const int chain = blockIdx.x * blockDim.x + threadIdx.x;
+    if (chain >= a.nchains) return;
+    // dataset of this chain (few datasets: linear scan of the tile table is avoided by
+    // a search over the dataset table)
+    int lo = 0;
+    while (a.ds[lo].chain0 + a.ds[lo].nchains <= chain) lo++;
+    const int cap = a.ds[lo].cap;
+    const int N = a.nchains;
+    for (uint32_t attempt = 0; attempt < 64; attempt++) {
+        gpr_rstream st;
+        st.init(a.seed, (uint32_t)chain, attempt, GPR_P_INIT, 0);
+        int need = 1, m = 0, depth = 0, dmax = 0;
+        bool fail = false;
+        // generation order is right-child-first prefix; write reversed = postfix
+        // into rows [cap-1 .. cap-m] first, then shift down.
+        while (need > 0) {
+            if (m >= cap) { fail = true; break; }
+            const int s = gpr_gen_symbol(st.next());
+            float cv = 0.0f;
+            if (s == GPR_OP_CONST) cv = gpr_normal(gpr_u01(st.next()));
+            a.ops[(size_t)(cap - 1 - m) * N + chain] = (uint8_t)s;
+            a.cs[(size_t)(cap - 1 - m) * N + chain] = cv;
+            m++;
+            need += gpr_arity(s) - 1;
+        }
+        if (fail) continue;
+        // shift rows [cap-m, cap) to [0, m) and compute the depth
+        for (int t = 0; t < m; t++) {
+            const uint8_t o = a.ops[(size_t)(cap - m + t) * N + chain];
+            const float cv = a.cs[(size_t)(cap - m + t) * N + chain];
+            a.ops[(size_t)t * N + chain] = o;
+            a.cs[(size_t)t * N + chain] = cv;
+            depth += 1 - gpr_arity(o);
+            dmax = max(dmax, depth);
+        }
+        for (int t = m; t < cap; t++) { a.ops[(size_t)t * N + chain] = 0; a.cs[(size_t)t * N + chain] = 0.0f; }
+        if (dmax > a.dmax) continue;
+        a.len[chain] = m;
+        return;
+    }
+    a.ops[chain] = GPR_OP_X;
+    a.cs[chain] = 0.0f;
+    for (int t = 1; t < cap; t++) { a.ops[(size_t)t * N + chain] = 0; a.cs[(size_t)t * N + chain] = 0.0f; }
+    a.len[chain] = 1;
+}
+
+hipError_t gpr_launch_init_config(const GprMHArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_config, dim3((a.nchains + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- resampling
+// One workgroup per dataset.  MAP rule: max posterior (NaN ignored), good = post >=
+// MAP - nats, deterministic compaction of good chains in chain order.
+#define GPR_RS_WG 1024
+__device__ __forceinline__ double post_of(const GprResampleArgs &a, int c) { return a.prior[c] + a.ll[c]; }
+
+__global__ void __launch_bounds__(GPR_RS_WG) k_rs_map_scan(GprResampleArgs a) {
+    const GprDataset d = a.ds[blockIdx.x];
+    __shared__ double s_max[GPR_RS_WG / 64];
+    __shared__ int s_arg[GPR_RS_WG / 64];
+    __shared__ int s_cnt[GPR_RS_WG / 64];
+    __shared__ int s_base;
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    double mx = -GPR_INF;
+    int arg = 0x7fffffff;
+    for (int i = tid; i < d.nchains; i += GPR_RS_WG) {
+        const double p = post_of(a, d.chain0 + i);
+        if (p == p && (p > mx || (p == mx && i < arg))) { mx = p; arg = i; }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double m2 = __shfl_xor(mx, o);
+        const int a2 = __shfl_xor(arg, o);
+        if (m2 > mx || (m2 == mx && a2 < arg)) { mx = m2; arg = a2; }
+    }
+    if (l == 0) { s_max[w] = mx; s_arg[w] = arg; }
+    __syncthreads();
+    if (tid == 0) {
+        double m = s_max[0];
+        int ag = s_arg[0];
+        for (int k = 1; k < GPR_RS_WG / 64; k++)
+            if (s_max[k] > m || (s_max[k] == m && s_arg[k] < ag)) { m = s_max[k]; ag = s_arg[k]; }
+        a.map_post[blockIdx.x] = m;
+        a.map_chain[blockIdx.x] = ag == 0x7fffffff ? -1 : d.chain0 + ag;
+        s_max[0] = m;
+        s_base = 0;
+    }
+    __syncthreads();
+    const double thr = s_max[0] - a.nats;
+    for (int base = 0; base < d.nchains; base += GPR_RS_WG) {
+        const int i = base + tid;
+        const bool good = i < d.nchains && post_of(a, d.chain0 + i) >= thr;
+        const uint64_t b = ballot(good);
+        const int before = __popcll(b & ((1ull << l) - 1ull));
+        if (l == 0) s_cnt[w] = __popcll(b);
+        __syncthreads();
+        int off = s_base;
+        for (int k = 0; k < w; k++) off += s_cnt[k];
+        if (good) a.good[d.chain0 + off + before] = d.chain0 + i;
+        __syncthreads();
+        if (tid == 0) { int t = 0; for (int k = 0; k < GPR_RS_WG / 64; k++) t += s_cnt[k]; s_base += t; }
+        __syncthreads();
+    }
+    if (tid == 0) a.ngood[blockIdx.x] = s_base;
+}
+
+__global__ void k_rs_map_gather(GprResampleArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nchains) return;
+    int di = 0;
+    while (di + 1 < a.ndatasets && a.ds[di + 1].chain0 <= c) di++;
+    const GprDataset d = a.ds[di];
+    const double p = post_of(a, c);
+    const int ng = a.ngood[di];
+    if (ng == 0 || p >= a.map_post[di] - a.nats) return;
+    const uint32_t r = gpr_first(a.seed, (uint32_t)c, a.block, GPR_P_RESAMPLE, 0);
+    const int j = a.good[d.chain0 + (int)(((uint64_t)r * (uint64_t)ng) >> 32)];
+    const int N = a.nchains;
+    for (int t = 0; t < a.cap_rows; t++) {
+        a.ops[(size_t)t * N + c] = a.ops[(size_t)t * N + j];
+        a.cs[(size_t)t * N + c] = a.cs[(size_t)t * N + j];
+    }
+    if (a.len) a.len[c] = a.len[j];
+    a.prior[c] = a.prior[j];
+    a.ll[c] = a.ll[j];
+    atomicAdd(a.nresampled, 1);
+}
+
+hipError_t gpr_launch_resample(const GprResampleArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_rs_map_scan, dim3(a.ndatasets), dim3(GPR_RS_WG), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rs_map_gather, dim3((a.nchains + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- small utilities
+__global__ void k_fill_i32(int *p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+hipError_t gpr_launch_fill_i32(int *p, int n, int v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}

@@ -1,0 +1,166 @@
+/*
+ * gpuropolis.h -- C ABI of libgpuropolis.so, the MI355X (gfx950) sampler.
+ *
+ * The reference (piantado/GPUropolis, one CUDA file) has no library boundary: its
+ * host loop in main() calls the device kernel and copies state with cudaMemcpy.
+ * This header replaces those calls one for one; each entry cites what it replaces.
+ *
+ * Conventions: every entry returns 0 on success or a negative GPR_E_* code and sets
+ * a per-thread message readable with gpr_last_error().  Nothing here exits the
+ * process.  Chain state is structure-of-arrays: element t of chain c is at
+ * base[t * nchains + c].  Pointers named d_* are device pointers; all others are host.
+ * Calls on one context are ordered on that context's HIP stream and return after
+ * the work finishes unless the name ends in _async.  One context per host thread.
+ */
+#ifndef GPUROPOLIS_H
+#define GPUROPOLIS_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- op codes */
+/* Postfix tape ops.  Leaves push; binary ops pop rhs then lhs; unary ops rewrite the
+ * top.  Values follow dispatch_eval (main.cu:397-444); CADD..CRPOW use the token's
+ * constant like CPLUS/CTIMES/CPOW/CRPOW there. */
+enum {
+    GPR_OP_X = 0, GPR_OP_CONST, GPR_OP_ONE,
+    GPR_OP_ADD, GPR_OP_SUB, GPR_OP_MUL, GPR_OP_DIV, GPR_OP_POW, GPR_OP_RSUB, GPR_OP_RDIV, GPR_OP_RPOW,
+    GPR_OP_NEG, GPR_OP_EXP, GPR_OP_LOG, GPR_OP_INV, GPR_OP_SQRT, GPR_OP_SIN, GPR_OP_ASIN, GPR_OP_COS,
+    GPR_OP_ACOS, GPR_OP_TAN, GPR_OP_ATAN, GPR_OP_ABS,
+    GPR_OP_CADD, GPR_OP_CMUL, GPR_OP_CPOW, GPR_OP_CRPOW,
+    GPR_NUM_OPS
+};
+
+/* Heap op codes of the reference's default op set, enum OPS at main.cu:53, with the
+ * 27-character PROGRAM_CODE of main.cu:55. */
+enum {
+    GPR_REF_ONE = 0, GPR_REF_CONST, GPR_REF_INV, GPR_REF_A, GPR_REF_B, GPR_REF_PLUS, GPR_REF_MINUS,
+    GPR_REF_RMINUS, GPR_REF_CPLUS, GPR_REF_TIMES, GPR_REF_CTIMES, GPR_REF_DIV, GPR_REF_RDIV,
+    GPR_REF_LOG, GPR_REF_EXP, GPR_REF_POW, GPR_REF_CPOW, GPR_REF_RPOW, GPR_REF_CRPOW, GPR_REF_SQRT,
+    GPR_REF_SIN, GPR_REF_ASIN, GPR_REF_COS, GPR_REF_ACOS, GPR_REF_TAN, GPR_REF_ATAN, GPR_REF_ABS,
+    GPR_REF_NOPS
+};
+#define GPR_REF_SLOTS 31            /* PROGRAM_LENGTH = NCONSTANTS, main.cu:26-27 */
+#define GPR_REF_PROGRAM_CODE "1CIab+-_#*@/|LE^pVPRSsCcTtA"
+
+/* ---------------------------------------------------------------- configuration */
+enum { GPR_SEM_CONFIG = 0, GPR_SEM_REFERENCE = 1 };
+enum { GPR_SCHED_AUTO = 0, GPR_SCHED_LANE = 1, GPR_SCHED_WAVE = 2 };
+enum { GPR_PROP_RW = 0, GPR_PROP_MALA = 1 };
+enum { GPR_RNG_PHILOX = 0, GPR_RNG_LCG = 1 };
+enum { GPR_RS_NONE = 0, GPR_RS_MAP = 1, GPR_RS_QUANTILE = 2 };
+
+enum {
+    GPR_OK = 0, GPR_E_ARG = -1, GPR_E_HIP = -2, GPR_E_STATE = -3, GPR_E_NOMEM = -4, GPR_E_UNSUPPORTED = -5
+};
+
+#define GPR_MAX_CAP 128             /* postfix tokens per program */
+#define GPR_MAX_DEPTH 16            /* evaluation stack depth of any program */
+
+typedef struct {
+    int device;                 /* --gpu, main.cu:1037-1048 */
+    int semantics;              /* GPR_SEM_*: postfix programs or the reference's heap-31 programs */
+    int cap;                    /* max tokens per program (config semantics), <= GPR_MAX_CAP */
+    int dmax;                   /* max evaluation stack depth (config semantics), <= GPR_MAX_DEPTH */
+    int nsym;                   /* description-length alphabet size: prior = -len*log(nsym) */
+    int proposal;               /* GPR_PROP_*: constant proposals */
+    float sigma;                /* random-walk sd on constants */
+    float eps;                  /* MALA step size */
+    int struct_every;           /* structure move when step % struct_every == 0 (main.cu:699) */
+    int fixed_structure;        /* 1: constant moves only */
+    int schedule;               /* GPR_SCHED_*: evaluator schedule of the MH kernel */
+    int rng;                    /* GPR_RNG_*; LCG only with reference semantics + lane schedule */
+    int resample;               /* GPR_RS_* rule applied by gpr_resample */
+    double resample_nats;       /* MAP rule: resample chains below MAP - resample_nats */
+    float resample_quantile;    /* quantile rule: RESAMPLE_QUANTILE, main.cu:36 */
+    uint64_t seed;              /* Philox key */
+} gpr_config;
+
+typedef struct {
+    long long struct_proposed, struct_accepted;
+    long long const_proposed, const_accepted;
+    long long steps;            /* chain-steps executed */
+    long long node_evals;       /* executed token x datapoint evaluations */
+    double last_block_ms;       /* device time of the last MH block (HIP events) */
+    int schedule_used;          /* GPR_SCHED_LANE or GPR_SCHED_WAVE */
+    int map_chain;              /* chain of the MAP after the last resample (dataset 0) */
+    double map_posterior;
+} gpr_stats;
+
+typedef struct gpr_ctx gpr_ctx;
+
+/* Fill cfg with the reference's defaults (main.cu:21-47, 96-109). */
+int gpr_config_default(gpr_config *cfg);
+
+/* Create a context on cfg->device.  Replaces cudaSetDevice/cudaDeviceReset
+ * (main.cu:1037-1051) and the DEVARRAY allocations (main.cu:171-175, 1143-1147). */
+int gpr_create(const gpr_config *cfg, gpr_ctx **out);
+int gpr_destroy(gpr_ctx *ctx);
+const char *gpr_last_error(void);
+const char *gpr_version(void);
+
+/* Datasets and chains.  Dataset d has n[d] points (x, y, sd concatenated over d) and
+ * chains[d] chains; chains of dataset d are numbered consecutively.  caps (may be
+ * NULL) gives a per-dataset token cap <= cfg->cap.  Replaces the D upload of
+ * main.cu:1143 and fixes N (main.cu:111-129 --N).  Allocates all device state. */
+int gpr_set_data(gpr_ctx *ctx, int ndatasets, const int32_t *n, const float *x, const float *y,
+                 const float *sd, const int32_t *chains, const int32_t *caps);
+
+/* Draw every chain from the initial distribution on the device: the generation
+ * distribution (config) or the reference's uniform ops and 10^k N(0,1) constants
+ * (main.cu:1133-1137). */
+int gpr_init_chains(gpr_ctx *ctx);
+
+/* Host copies of the whole state.  Config semantics: ops/cs are [cap][N], len [N].
+ * Reference semantics: ops/cs are the heap arrays P/C [31][N] and len is ignored.
+ * Replaces the cudaMemcpy of P, C, prior, likelihood (main.cu:1163-1166, 1233-1234). */
+int gpr_write_state(gpr_ctx *ctx, const uint8_t *ops, const float *cs, const int32_t *len);
+int gpr_read_state(gpr_ctx *ctx, uint8_t *ops, float *cs, int32_t *len, double *prior, double *ll);
+
+/* One MCMC block of `steps` MH steps on every chain: the launch of MH_simple_kernel
+ * (main.cu:1160, kernel main.cu:679-813).  block_seed plays the role of the rand()
+ * seed argument (used by GPR_RNG_LCG); Philox streams use cfg->seed and the global
+ * step index.  Writes per-chain prior and log-likelihood (FP64). */
+int gpr_run_block(gpr_ctx *ctx, int steps, int32_t block_seed);
+int gpr_run_block_async(gpr_ctx *ctx, int steps, int32_t block_seed);
+
+/* Resampling between blocks (main.cu:1215-1234), on the device.  GPR_RS_MAP: per
+ * dataset, chains below MAP - resample_nats copy a uniformly drawn chain within it.
+ * GPR_RS_QUANTILE: chains below the quantile of the posterior are re-drawn from the
+ * initial distribution (the reference rule). */
+int gpr_resample(gpr_ctx *ctx);
+
+/* Capture `nblocks` x (run_block + resample) as one hipGraph and replay it. */
+int gpr_run_blocks_graph(gpr_ctx *ctx, int nblocks, int steps);
+
+int gpr_get_stats(const gpr_ctx *ctx, gpr_stats *out);
+int gpr_reset_stats(gpr_ctx *ctx);
+int gpr_sync(gpr_ctx *ctx);
+/* hipStream_t of the context (as void*) */
+void *gpr_stream(gpr_ctx *ctx);
+
+/* ---------------------------------------------------------------- standalone kernels */
+/* Gaussian log-likelihood of nchains postfix tapes against dataset `dataset` of ctx:
+ * sum_i -(f(x_i)-y_i)^2/(2 sd_i^2) - log sd_i - log(2 pi)/2, -inf if any f(x_i) or term
+ * is not finite (compute_likelihood, main.cu:454-468).  Node values FP32, compensated
+ * FP32 sums.  d_ops/d_cs are [cap][nchains] device arrays, d_len [nchains]. */
+int gpr_eval_loglik(gpr_ctx *ctx, int dataset, const uint8_t *d_ops, const float *d_cs,
+                    const int32_t *d_len, int nchains, int cap, int schedule, double *d_ll);
+
+/* d log posterior / d cs[t] by a reverse-mode adjoint over each tape (no reference
+ * counterpart: the reference has no gradients).  d_grad is [cap][nchains] FP32 (0 on
+ * tokens without a constant); d_lp receives the log posterior (FP64). */
+int gpr_grad_logpost(gpr_ctx *ctx, int dataset, const uint8_t *d_ops, const float *d_cs,
+                     const int32_t *d_len, int nchains, int cap, float *d_grad, double *d_lp);
+
+/* Heap-31 -> postfix decode of the reachable tree (program_fold, main.cu:371-393):
+ * ops/cs receive up to 63 tokens; returns the token count or a negative error. */
+int gpr_ref_decode(const uint8_t *hop, const float *hc, size_t stride, uint8_t *ops, float *cs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

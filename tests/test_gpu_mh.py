@@ -1,0 +1,98 @@
+"""Config-semantics MH kernel against the oracle on the same Philox streams.
+
+The draws (structure choices, generated subtrees, constant proposals, accept
+uniforms) are bit-identical by construction, so chains must follow the same
+trajectories: identical programs and lengths, constants equal, prior/LL within
+tolerance.  A chain may leave the shared path only through an accept decision whose
+log-ratio lies within rounding of the threshold; at most one such chain is allowed.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import ll_close, say
+
+pytestmark = pytest.mark.gpu
+
+
+def s_data():
+    from gpuropolis_amd import workloads as W
+    return W.dataset_S(0)
+
+
+def test_init_matches_oracle(oracle, gpr):
+    x, y, sd = s_data()
+    for cap in (32, 128):
+        cfg = oracle.make_cfg(cap=cap, seed=7)
+        o_ops, o_cs, o_len = oracle.cfg_init(cfg, 2048)
+        with gpr.Context(cap=cap, seed=7) as ctx:
+            ctx.set_data([(x, y, sd)], [2048])
+            say(f"init parity cap={cap}")
+            ctx.init_chains()
+            ops, cs, ln, _, _ = ctx.read_state()
+        assert np.array_equal(ln, o_len)
+        assert np.array_equal(ops, o_ops)
+        assert np.array_equal(cs.view(np.uint32), o_cs.view(np.uint32))
+
+
+@pytest.mark.parametrize("cap,n,steps", [(32, 32, 300), (64, 250, 150), (128, 70, 150)])
+def test_mh_trajectories_match_oracle(oracle, gpr, cap, n, steps):
+    rng = np.random.default_rng(cap + n)
+    if n == 32:
+        x, y, sd = s_data()
+    else:
+        x = rng.uniform(-5, 5, n).astype(np.float32)
+        y = (1.5 * np.exp(-0.4 * x) + rng.normal(0, 0.05, n)).astype(np.float32)
+        sd = np.full(n, 0.05, dtype=np.float32)
+    nch = 64
+    cfg = oracle.make_cfg(cap=cap, seed=123)
+    ops, cs, ln = oracle.cfg_init(cfg, nch)
+    o_ops, o_cs, o_ln = ops.copy(), cs.copy(), ln.copy()
+    p0, l0, cnt = oracle.cfg_run_block(cfg, o_ops, o_cs, o_ln, x, y, sd, 0, steps)
+    p1, l1, cnt1 = oracle.cfg_run_block(cfg, o_ops, o_cs, o_ln, x, y, sd, steps, steps)
+    with gpr.Context(cap=cap, seed=123) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(ops, cs, ln)
+        say(f"mh trajectories cap={cap} n={n} steps={steps}")
+        ctx.run_block(steps)
+        ctx.run_block(steps)
+        g_ops, g_cs, g_ln, g_pr, g_ll = ctx.read_state()
+        st = ctx.stats()
+    # on the oracle path: identical program, length and constant bits
+    same = (g_ln == o_ln) & np.all(g_ops == o_ops, axis=0) & np.all(g_cs.view(np.uint32) == o_cs.view(np.uint32), axis=0)
+    say("chains on the oracle path:", int(same.sum()), "of", nch)
+    assert same.sum() >= int(0.95 * nch)
+    c = np.flatnonzero(same)
+    assert np.array_equal(g_pr[c], p1[c])          # prior is bit-reproducible
+    kappa = np.array([abs(oracle.tape_loglik(o_ops[:o_ln[i], i], o_cs[:o_ln[i], i], x, y, sd, 0)
+                          - oracle.tape_loglik(o_ops[:o_ln[i], i], o_cs[:o_ln[i], i], x, y, sd, 1)) for i in c])
+    ok, diff, tol = ll_close(g_ll[c], l1[c], n)
+    assert np.all(ok | (diff <= tol + 4 * np.nan_to_num(kappa))), (diff[~ok], tol[~ok], kappa[~ok])
+    tot = cnt + cnt1
+    ndiv = nch - same.sum()
+    assert abs(st.struct_proposed - tot[0]) <= 2 * steps * ndiv // 5 and abs(st.const_proposed - tot[2]) <= 2 * steps * ndiv
+    assert abs(st.struct_accepted - tot[1]) <= 3 + nch - same.sum()
+    assert abs(st.const_accepted - tot[3]) <= 6 + 2 * (nch - same.sum())
+
+
+def test_map_resample_matches_oracle(oracle, gpr):
+    x, y, sd = s_data()
+    nch = 3000
+    cfg = oracle.make_cfg(cap=32, seed=5)
+    ops, cs, ln = oracle.cfg_init(cfg, nch)
+    with gpr.Context(cap=32, seed=5, resample_nats=10.0) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(ops, cs, ln)
+        ctx.run_block(50)
+        g_ops, g_cs, g_ln, g_pr, g_ll = ctx.read_state()
+        say("map resample")
+        ctx.resample()
+        r_ops, r_cs, r_ln, r_pr, r_ll = ctx.read_state()
+    src = np.zeros(nch, dtype=np.int32)
+    o_ops, o_cs, o_ln, o_pr, o_ll = g_ops.copy(), g_cs.copy(), g_ln.copy(), g_pr.copy(), g_ll.copy()
+    import ctypes as C
+    nbad = oracle.lib().gpo_cfg_resample(nch, 0, 1, 5, 32, 10.0, oracle.u8(o_ops), oracle.f32(o_cs),
+                                         oracle.i32(o_ln), oracle.f64(o_pr), oracle.f64(o_ll), oracle.i32(src))
+    assert nbad > 0
+    assert np.array_equal(r_ln, o_ln) and np.array_equal(r_ops, o_ops)
+    assert np.array_equal(r_cs.view(np.uint32), o_cs.view(np.uint32))
+    assert np.array_equal(r_pr + r_ll, o_pr + o_ll)
