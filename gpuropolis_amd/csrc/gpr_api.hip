@@ -434,6 +434,7 @@ extern "C" int gpr_get_stats(const gpr_ctx *c, gpr_stats *o) {
     o->const_accepted = (long long)h[GPR_CNT_CACC];
     o->node_evals = (long long)h[GPR_CNT_EVALS];
     o->steps = (long long)h[GPR_CNT_STEPS];
+    o->alg_flops = (long long)h[GPR_CNT_FLOPS];
     o->last_block_ms = c->last_ms;
     o->schedule_used = c->sched_used;
     o->map_chain = mc;

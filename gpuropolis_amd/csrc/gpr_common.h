@@ -79,7 +79,8 @@ GPR_HD uint32_t gpr_f_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
 
 GPR_HD float gpr_u01(uint32_t r) { return ((float)(r >> 9) + 0.5f) * 0x1p-23f; }
 
-// natural log: m*2^e, m in (sqrt(1/2), sqrt(2)], log m = 2 atanh((m-1)/(m+1))
+// natural log: m*2^e, m in (sqrt(1/2), sqrt(2)], log m = 2 atanh((m-1)/(m+1)); fused
+// multiply-adds are written as fmaf so host and device round identically
 GPR_HD float gpr_logf(float x) {
 #pragma clang fp contract(off)
     if (x != x) return x;
@@ -92,13 +93,16 @@ GPR_HD float gpr_logf(float x) {
     e -= 127;
     float m = gpr_bits_f((b & 0x007fffffu) | 0x3f800000u);
     if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e += 1; }
-    float s = (m - 1.0f) / (m + 1.0f);
-    float s2 = s * s;
-    float p = s2 * (0x1.555556p-2f + s2 * (0x1.99999ap-3f + s2 * (0x1.24924ap-3f
-              + s2 * (0x1.c71c72p-4f + s2 * 0x1.745d18p-4f))));
-    float lm = (s + s * p) * 2.0f;
-    float ef = (float)e;
-    return ef * 0x1.62e4p-1f + (ef * 0x1.7f7d1cp-20f + lm);
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = __builtin_fmaf(s2, 0x1.745d18p-4f, 0x1.c71c72p-4f);
+    p = __builtin_fmaf(s2, p, 0x1.24924ap-3f);
+    p = __builtin_fmaf(s2, p, 0x1.99999ap-3f);
+    p = __builtin_fmaf(s2, p, 0x1.555556p-2f);
+    p = p * s2;
+    const float lm = __builtin_fmaf(s, p, s) * 2.0f;
+    const float ef = (float)e;
+    return __builtin_fmaf(ef, 0x1.62e4p-1f, __builtin_fmaf(ef, 0x1.7f7d1cp-20f, lm));
 }
 
 GPR_HD float gpr_pow2i(int k) { return gpr_bits_f((uint32_t)(k + 127) << 23); }
@@ -109,12 +113,17 @@ GPR_HD float gpr_expf(float x) {
     if (x != x) return x;
     if (x > 0x1.62e42ep+6f) return __builtin_inff();
     if (x < -0x1.9fe368p+6f) return 0.0f;
-    float kf = __builtin_floorf(x * 0x1.715476p+0f + 0.5f);
-    float r = (x - kf * 0x1.62e4p-1f) - kf * 0x1.7f7d1cp-20f;
-    float p = 1.0f + r * (1.0f + r * (0.5f + r * (0x1.555556p-3f + r * (0x1.555556p-5f
-              + r * (0x1.111112p-7f + r * (0x1.6c16c2p-10f + r * 0x1.a01a02p-13f))))));
-    int k = (int)kf;
-    int k1 = k / 2, k2 = k - k1;
+    const float kf = __builtin_floorf(__builtin_fmaf(x, 0x1.715476p+0f, 0.5f));
+    const float r = __builtin_fmaf(-kf, 0x1.7f7d1cp-20f, __builtin_fmaf(-kf, 0x1.62e4p-1f, x));
+    float p = __builtin_fmaf(r, 0x1.a01a02p-13f, 0x1.6c16c2p-10f);
+    p = __builtin_fmaf(r, p, 0x1.111112p-7f);
+    p = __builtin_fmaf(r, p, 0x1.555556p-5f);
+    p = __builtin_fmaf(r, p, 0x1.555556p-3f);
+    p = __builtin_fmaf(r, p, 0.5f);
+    p = __builtin_fmaf(r, p, 1.0f);
+    p = __builtin_fmaf(r, p, 1.0f);
+    const int k = (int)kf;
+    const int k1 = k / 2, k2 = k - k1;
     return (p * gpr_pow2i(k1)) * gpr_pow2i(k2);
 }
 
@@ -122,15 +131,23 @@ GPR_HD float gpr_expf(float x) {
 GPR_HD float gpr_sin2pi(float v) {
 #pragma clang fp contract(off)
     if (!(v - v == 0.0f)) return __builtin_nanf("");
-    float v4 = v * 4.0f;
-    float q = __builtin_floorf(v4 + 0.5f);
-    float r = v4 - q;
-    int qi = ((int)__builtin_fmodf(q, 4.0f) + 4) & 3;
-    float r2 = r * r;
-    float sn = r * (0x1.921fb6p+0f + r2 * (-0x1.4abbcep-1f + r2 * (0x1.466bc6p-4f
-               + r2 * (-0x1.32d2ccp-8f + r2 * (0x1.507834p-13f + r2 * -0x1.e3075p-19f)))));
-    float cs = 1.0f + r2 * (-0x1.3bd3ccp+0f + r2 * (0x1.03c1fp-2f + r2 * (-0x1.55d3c8p-6f
-               + r2 * (0x1.e1f506p-11f + r2 * (-0x1.a6d1f2p-16f + r2 * 0x1.f9d38ap-22f)))));
+    const float v4 = v * 4.0f;
+    const float q = __builtin_floorf(v4 + 0.5f);
+    const float r = v4 - q;
+    const int qi = ((int)__builtin_fmodf(q, 4.0f) + 4) & 3;
+    const float r2 = r * r;
+    float sn = __builtin_fmaf(r2, -0x1.e3075p-19f, 0x1.507834p-13f);
+    sn = __builtin_fmaf(r2, sn, -0x1.32d2ccp-8f);
+    sn = __builtin_fmaf(r2, sn, 0x1.466bc6p-4f);
+    sn = __builtin_fmaf(r2, sn, -0x1.4abbcep-1f);
+    sn = __builtin_fmaf(r2, sn, 0x1.921fb6p+0f);
+    sn = sn * r;
+    float cs = __builtin_fmaf(r2, 0x1.f9d38ap-22f, -0x1.a6d1f2p-16f);
+    cs = __builtin_fmaf(r2, cs, 0x1.e1f506p-11f);
+    cs = __builtin_fmaf(r2, cs, -0x1.55d3c8p-6f);
+    cs = __builtin_fmaf(r2, cs, 0x1.03c1fp-2f);
+    cs = __builtin_fmaf(r2, cs, -0x1.3bd3ccp+0f);
+    cs = __builtin_fmaf(r2, cs, 1.0f);
     return qi == 0 ? sn : (qi == 1 ? cs : (qi == 2 ? -sn : -cs));
 }
 
@@ -138,33 +155,41 @@ GPR_HD float gpr_sin2pi(float v) {
 // Gems, 2011) single-precision polynomials; odd in 2u-1.
 GPR_HD float gpr_normal(float u) {
 #pragma clang fp contract(off)
-    float x = 2.0f * u - 1.0f;
+    const float x = __builtin_fmaf(2.0f, u, -1.0f);
     float w = -gpr_logf((1.0f - x) * (1.0f + x));
     float p;
     if (w < 5.0f) {
         w = w - 2.5f;
         p = 2.81022636e-08f;
-        p = 3.43273939e-07f + p * w;
-        p = -3.5233877e-06f + p * w;
-        p = -4.39150654e-06f + p * w;
-        p = 0.00021858087f + p * w;
-        p = -0.00125372503f + p * w;
-        p = -0.00417768164f + p * w;
-        p = 0.246640727f + p * w;
-        p = 1.50140941f + p * w;
+        p = __builtin_fmaf(p, w, 3.43273939e-07f);
+        p = __builtin_fmaf(p, w, -3.5233877e-06f);
+        p = __builtin_fmaf(p, w, -4.39150654e-06f);
+        p = __builtin_fmaf(p, w, 0.00021858087f);
+        p = __builtin_fmaf(p, w, -0.00125372503f);
+        p = __builtin_fmaf(p, w, -0.00417768164f);
+        p = __builtin_fmaf(p, w, 0.246640727f);
+        p = __builtin_fmaf(p, w, 1.50140941f);
     } else {
         w = __builtin_sqrtf(w) - 3.0f;
         p = -0.000200214257f;
-        p = 0.000100950558f + p * w;
-        p = 0.00134934322f + p * w;
-        p = -0.00367342844f + p * w;
-        p = 0.00573950773f + p * w;
-        p = -0.0076224613f + p * w;
-        p = 0.00943887047f + p * w;
-        p = 1.00167406f + p * w;
-        p = 2.83297682f + p * w;
+        p = __builtin_fmaf(p, w, 0.000100950558f);
+        p = __builtin_fmaf(p, w, 0.00134934322f);
+        p = __builtin_fmaf(p, w, -0.00367342844f);
+        p = __builtin_fmaf(p, w, 0.00573950773f);
+        p = __builtin_fmaf(p, w, -0.0076224613f);
+        p = __builtin_fmaf(p, w, 0.00943887047f);
+        p = __builtin_fmaf(p, w, 1.00167406f);
+        p = __builtin_fmaf(p, w, 2.83297682f);
     }
     return (p * x) * 1.41421356f;
+}
+
+// Words of the batched substreams: a Philox block at counter step>>2 serves four
+// consecutive steps, word step & 3.  Constant proposals of token t use slot t & 63 and
+// block word-row t >> 6; accept draws use slot 0.
+GPR_HD uint32_t gpr_u4_word(const gpr_u4 &w, int i) { return i == 0 ? w.x : (i == 1 ? w.y : (i == 2 ? w.z : w.w)); }
+GPR_HD gpr_u4 gpr_step_block(uint64_t seed, uint32_t chain, uint32_t step, uint32_t purpose, uint32_t slot, uint32_t row) {
+    return gpr_philox(chain, step >> 2, (purpose << 24) | (slot & 0xffffffu), row, (uint32_t)seed, (uint32_t)(seed >> 32));
 }
 
 // ---------------------------------------------------------------- densities

@@ -27,7 +27,9 @@ struct GprTile {
 #define GPR_WG 256
 
 // Counters accumulated by the MH kernels (device, u64).
-enum { GPR_CNT_SPROP = 0, GPR_CNT_SACC, GPR_CNT_CPROP, GPR_CNT_CACC, GPR_CNT_EVALS, GPR_CNT_STEPS, GPR_NCNT };
+// FLOPS: algorithmic FLOPs n*(N_op + 4) + N_const per evaluated program (N_op = non-leaf
+// tokens; the +4 is residual, square, weight, accumulate)
+enum { GPR_CNT_SPROP = 0, GPR_CNT_SACC, GPR_CNT_CPROP, GPR_CNT_CACC, GPR_CNT_EVALS, GPR_CNT_STEPS, GPR_CNT_FLOPS, GPR_NCNT };
 
 struct GprMHArgs {
     uint8_t *ops;           // [cap][N]

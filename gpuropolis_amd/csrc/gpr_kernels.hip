@@ -36,23 +36,50 @@ __device__ __forceinline__ float wl_f(float old, float v, int l) { return lane_i
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// DPP controls (gfx9 encoding): quad_perm, row_shr:n, row_mirror, row_half_mirror,
+// row_bcast:15, row_bcast:31.  DPP moves are VALU operand modifiers: no LDS round trip.
+#define DPP_QP_1032 0xB1
+#define DPP_QP_2301 0x4E
+#define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_ROW_MIRROR 0x140
+#define DPP_ROW_HALF_MIRROR 0x141
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
+}
+
+// Sum over the 64 lanes, returned uniform.  Butterflies inside each 16-lane row, then
+// the four row sums combined in a fixed order.
 __device__ __forceinline__ float wave_sum_f(float v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp_f<DPP_QP_1032>(v);
+    v += dpp_f<DPP_QP_2301>(v);
+    v += dpp_f<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_f<DPP_ROW_MIRROR>(v);
+    return (rl_f(v, 0) + rl_f(v, 16)) + (rl_f(v, 32) + rl_f(v, 48));
 }
 __device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return v;
+    v = max(v, dpp_i<DPP_QP_1032>(v));
+    v = max(v, dpp_i<DPP_QP_2301>(v));
+    v = max(v, dpp_i<DPP_ROW_HALF_MIRROR>(v));
+    v = max(v, dpp_i<DPP_ROW_MIRROR>(v));
+    return max(max(rl_i(v, 0), rl_i(v, 16)), max(rl_i(v, 32), rl_i(v, 48)));
 }
-__device__ __forceinline__ int wave_scan_i(int v) {  // inclusive
-    const int l = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int u = __shfl_up(v, o);
-        if (l >= o) v += u;
-    }
+// Inclusive prefix sum over the 64 lanes: Kogge-Stone inside rows (row_shr with zero
+// fill), then row 15 -> row 1,3 and lane 31 -> rows 2,3 broadcasts.
+__device__ __forceinline__ int wave_scan_i(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(1), 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(2), 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(4), 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(8), 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST15, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST31, 0xc, 0xf, false);
     return v;
 }
 
@@ -298,13 +325,16 @@ __device__ __forceinline__ float gather_f(float v0, float v1, int src, bool hi) 
     return src >= 64 ? b : a;
 }
 
-// config accept rule: invalid current always moves; log u < a otherwise
+// config accept rule: invalid current always moves; log u < a otherwise.  u lies on the
+// grid ((k + 1/2) 2^-23), so log u >= -16.64: any a < -17 rejects without drawing.
 __device__ __forceinline__ bool cfg_accept(uint64_t seed, uint32_t chain, uint32_t step, double cur, double prop,
                                            double a) {
     if (!__builtin_isfinite(cur)) return true;
     if (!__builtin_isfinite(prop)) return false;
     if (a >= 0.0) return true;
-    const float u = gpr_u01(gpr_first(seed, chain, step, GPR_P_ACCEPT, 0));
+    if (a < -17.0) return false;
+    const gpr_u4 w = gpr_step_block(seed, chain, step, GPR_P_ACCEPT, 0, 0);
+    const float u = gpr_u01(gpr_u4_word(w, (int)(step & 3)));
     return (double)gpr_logf(u) < a;
 }
 
@@ -314,15 +344,12 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
     const int N = a.nchains;
-    unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0, n_evals = 0, n_steps = 0;
+    unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0, n_evals = 0, n_steps = 0, n_flops = 0;
     for (;;) {
-        int tile_idx = 0;
-        if (threadIdx.x == 0) tile_idx = atomicAdd(a.tile_counter, 1);
-        tile_idx = __shfl(tile_idx, 0);
         __shared__ int s_tile;
-        if (threadIdx.x == 0) s_tile = tile_idx;
+        if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1);
         __syncthreads();
-        tile_idx = uni(s_tile);
+        const int tile_idx = uni(s_tile);
         __syncthreads();
         if (tile_idx >= a.ntiles) break;
         const GprTile tile = a.tiles[tile_idx];
@@ -346,6 +373,9 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         // iteration -1 evaluates the current state (main.cu:683-684); the prior and
         // likelihood have a single call site so the kernel body stays small
         double prior = 0.0, ll = 0.0;
+        // per-lane constant-proposal block, reused by four consecutive steps
+        gpr_u4 wlo = {0, 0, 0, 0}, whi = {0, 0, 0, 0};
+        uint32_t wblk = 0xffffffffu;
         for (int i = -1; i < a.steps; i++) {
             const uint32_t step = a.step0 + (uint32_t)i;
             const double post = prior + ll;
@@ -424,14 +454,25 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                 const uint64_t mk0 = ballot(k0), mk1 = HI ? ballot(k1) : 0;
                 if ((mk0 | mk1) == 0) continue;
                 n_cprop++;
-                const gpr_u4 w = gpr_philox(chain, step, ((uint32_t)GPR_P_CPROP << 24) | (uint32_t)l, 0,
-                                            (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
-                if (k0) prop.c0 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(w.x)), cur.c0);
-                if (HI && k1) prop.c1 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(w.y)), cur.c1);
+                if ((step >> 2) != wblk) {
+                    wblk = step >> 2;
+                    wlo = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 0);
+                    if (HI) whi = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 1);
+                }
+                const int wi = (int)(step & 3);
+                if (k0) prop.c0 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(wlo, wi))), cur.c0);
+                if (HI && k1) prop.c1 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(whi, wi))), cur.c1);
             }
             const double pprior = wave_prior<HI>(prop, a.lnsym);
             const double pll = wave_loglik<KC, HI>(prop, d, dx, dy, dw, c0, nchunks);
             n_evals += (unsigned long long)prop.len * (unsigned long long)d.n;
+            {
+                const int nleaf = __popcll(ballot(l < prop.len && prop.o0 <= GPR_OP_ONE)) +
+                                  (HI ? __popcll(ballot(l + 64 < prop.len && prop.o1 <= GPR_OP_ONE)) : 0);
+                const int nconst = __popcll(ballot(l < prop.len && gpr_has_const(prop.o0))) +
+                                   (HI ? __popcll(ballot(l + 64 < prop.len && gpr_has_const(prop.o1))) : 0);
+                n_flops += (unsigned long long)(prop.len - nleaf + 4) * (unsigned long long)d.n + (unsigned long long)nconst;
+            }
             if (i < 0) { prior = pprior; ll = pll; continue; }
             const double ppost = pprior + pll;
             if (cfg_accept(a.seed, chain, step, post, ppost, ppost - post + extra)) {
@@ -457,6 +498,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         atomicAdd(&a.counters[GPR_CNT_CACC], n_cacc);
         atomicAdd(&a.counters[GPR_CNT_EVALS], n_evals);
         atomicAdd(&a.counters[GPR_CNT_STEPS], n_steps);
+        atomicAdd(&a.counters[GPR_CNT_FLOPS], n_flops);
     }
 }
 

@@ -44,7 +44,7 @@ class Config(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("struct_proposed", C.c_longlong), ("struct_accepted", C.c_longlong),
                 ("const_proposed", C.c_longlong), ("const_accepted", C.c_longlong), ("steps", C.c_longlong),
-                ("node_evals", C.c_longlong), ("last_block_ms", C.c_double), ("schedule_used", C.c_int),
+                ("node_evals", C.c_longlong), ("alg_flops", C.c_longlong), ("last_block_ms", C.c_double), ("schedule_used", C.c_int),
                 ("map_chain", C.c_int), ("map_posterior", C.c_double)]
 
 

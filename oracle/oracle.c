@@ -82,7 +82,8 @@ uint32_t gpo_lcg_next(uint32_t *rx) {
 static float bits_f(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
 static uint32_t f_bits(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
 
-/* natural log: m*2^e with m in (sqrt(1/2), sqrt(2)], log m = 2 atanh((m-1)/(m+1)) */
+/* natural log: m*2^e with m in (sqrt(1/2), sqrt(2)], log m = 2 atanh((m-1)/(m+1));
+ * fused multiply-adds are explicit fmaf, one rounding each, as on the GPU */
 float gpo_logf(float x) {
     if (x != x) return x;
     if (x < 0.0f) return NAN;
@@ -96,11 +97,14 @@ float gpo_logf(float x) {
     if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e += 1; }
     float s = (m - 1.0f) / (m + 1.0f);
     float s2 = s * s;
-    float p = s2 * (0x1.555556p-2f + s2 * (0x1.99999ap-3f + s2 * (0x1.24924ap-3f
-              + s2 * (0x1.c71c72p-4f + s2 * 0x1.745d18p-4f))));
-    float lm = (s + s * p) * 2.0f;
+    float p = fmaf(s2, 0x1.745d18p-4f, 0x1.c71c72p-4f);
+    p = fmaf(s2, p, 0x1.24924ap-3f);
+    p = fmaf(s2, p, 0x1.99999ap-3f);
+    p = fmaf(s2, p, 0x1.555556p-2f);
+    p = p * s2;
+    float lm = fmaf(s, p, s) * 2.0f;
     float ef = (float)e;
-    return ef * 0x1.62e4p-1f + (ef * 0x1.7f7d1cp-20f + lm);
+    return fmaf(ef, 0x1.62e4p-1f, fmaf(ef, 0x1.7f7d1cp-20f, lm));
 }
 
 /* exp: 2^k * e^r, |r| <= ln2/2, Cody-Waite split of ln2 (k*LN2_HI exact) */
@@ -109,10 +113,15 @@ float gpo_expf(float x) {
     if (x != x) return x;
     if (x > 0x1.62e42ep+6f) return INFINITY;
     if (x < -0x1.9fe368p+6f) return 0.0f;
-    float kf = floorf(x * 0x1.715476p+0f + 0.5f);
-    float r = (x - kf * 0x1.62e4p-1f) - kf * 0x1.7f7d1cp-20f;
-    float p = 1.0f + r * (1.0f + r * (0.5f + r * (0x1.555556p-3f + r * (0x1.555556p-5f
-              + r * (0x1.111112p-7f + r * (0x1.6c16c2p-10f + r * 0x1.a01a02p-13f))))));
+    float kf = floorf(fmaf(x, 0x1.715476p+0f, 0.5f));
+    float r = fmaf(-kf, 0x1.7f7d1cp-20f, fmaf(-kf, 0x1.62e4p-1f, x));
+    float p = fmaf(r, 0x1.a01a02p-13f, 0x1.6c16c2p-10f);
+    p = fmaf(r, p, 0x1.111112p-7f);
+    p = fmaf(r, p, 0x1.555556p-5f);
+    p = fmaf(r, p, 0x1.555556p-3f);
+    p = fmaf(r, p, 0.5f);
+    p = fmaf(r, p, 1.0f);
+    p = fmaf(r, p, 1.0f);
     int k = (int)kf;
     int k1 = k / 2, k2 = k - k1;
     return (p * pow2i(k1)) * pow2i(k2);
@@ -126,10 +135,18 @@ float gpo_sin2pi(float v) {
     float r = v4 - q;
     int qi = ((int)fmodf(q, 4.0f) + 4) & 3;
     float r2 = r * r;
-    float sn = r * (0x1.921fb6p+0f + r2 * (-0x1.4abbcep-1f + r2 * (0x1.466bc6p-4f
-               + r2 * (-0x1.32d2ccp-8f + r2 * (0x1.507834p-13f + r2 * -0x1.e3075p-19f)))));
-    float cs = 1.0f + r2 * (-0x1.3bd3ccp+0f + r2 * (0x1.03c1fp-2f + r2 * (-0x1.55d3c8p-6f
-               + r2 * (0x1.e1f506p-11f + r2 * (-0x1.a6d1f2p-16f + r2 * 0x1.f9d38ap-22f)))));
+    float sn = fmaf(r2, -0x1.e3075p-19f, 0x1.507834p-13f);
+    sn = fmaf(r2, sn, -0x1.32d2ccp-8f);
+    sn = fmaf(r2, sn, 0x1.466bc6p-4f);
+    sn = fmaf(r2, sn, -0x1.4abbcep-1f);
+    sn = fmaf(r2, sn, 0x1.921fb6p+0f);
+    sn = sn * r;
+    float cs = fmaf(r2, 0x1.f9d38ap-22f, -0x1.a6d1f2p-16f);
+    cs = fmaf(r2, cs, 0x1.e1f506p-11f);
+    cs = fmaf(r2, cs, -0x1.55d3c8p-6f);
+    cs = fmaf(r2, cs, 0x1.03c1fp-2f);
+    cs = fmaf(r2, cs, -0x1.3bd3ccp+0f);
+    cs = fmaf(r2, cs, 1.0f);
     switch (qi) { case 0: return sn; case 1: return cs; case 2: return -sn; default: return -cs; }
 }
 
@@ -137,31 +154,31 @@ float gpo_sin2pi(float v) {
  * "Approximating the erfinv function" (GPU Computing Gems, 2011), single-precision
  * polynomials.  Odd-symmetric in 2u-1, so u and 1-u give exactly opposite draws. */
 float gpo_normal_icdf(float u) {
-    float x = 2.0f * u - 1.0f;
+    float x = fmaf(2.0f, u, -1.0f);
     float w = -gpo_logf((1.0f - x) * (1.0f + x));
     float p;
     if (w < 5.0f) {
         w = w - 2.5f;
         p = 2.81022636e-08f;
-        p = 3.43273939e-07f + p * w;
-        p = -3.5233877e-06f + p * w;
-        p = -4.39150654e-06f + p * w;
-        p = 0.00021858087f + p * w;
-        p = -0.00125372503f + p * w;
-        p = -0.00417768164f + p * w;
-        p = 0.246640727f + p * w;
-        p = 1.50140941f + p * w;
+        p = fmaf(p, w, 3.43273939e-07f);
+        p = fmaf(p, w, -3.5233877e-06f);
+        p = fmaf(p, w, -4.39150654e-06f);
+        p = fmaf(p, w, 0.00021858087f);
+        p = fmaf(p, w, -0.00125372503f);
+        p = fmaf(p, w, -0.00417768164f);
+        p = fmaf(p, w, 0.246640727f);
+        p = fmaf(p, w, 1.50140941f);
     } else {
         w = sqrtf(w) - 3.0f;
         p = -0.000200214257f;
-        p = 0.000100950558f + p * w;
-        p = 0.00134934322f + p * w;
-        p = -0.00367342844f + p * w;
-        p = 0.00573950773f + p * w;
-        p = -0.0076224613f + p * w;
-        p = 0.00943887047f + p * w;
-        p = 1.00167406f + p * w;
-        p = 2.83297682f + p * w;
+        p = fmaf(p, w, 0.000100950558f);
+        p = fmaf(p, w, 0.00134934322f);
+        p = fmaf(p, w, -0.00367342844f);
+        p = fmaf(p, w, 0.00573950773f);
+        p = fmaf(p, w, -0.0076224613f);
+        p = fmaf(p, w, 0.00943887047f);
+        p = fmaf(p, w, 1.00167406f);
+        p = fmaf(p, w, 2.83297682f);
     }
     return (p * x) * 1.41421356f;
 }
@@ -650,13 +667,18 @@ int gpo_cfg_init_chain(const gpo_cfg *cfg, uint32_t chain, uint8_t *ops, float *
     return 1;
 }
 
-/* constant-proposal word of token t: Philox block (chain, step, CPROP, t & 63),
- * word x for t < 64 and word y for t >= 64 */
+/* Batched substreams: the Philox block with counter {chain, step>>2, purpose<<24|slot,
+ * row} serves four consecutive steps, word step & 3.  Constant proposals of token t
+ * use slot t & 63, row t >> 6; accept draws use slot 0, row 0. */
+static uint32_t step_word(uint64_t seed, uint32_t chain, uint32_t step, uint32_t purpose, uint32_t slot, uint32_t row) {
+    uint32_t ctr[4] = { chain, step >> 2, (purpose << 24) | (slot & 0xffffffu), row };
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    uint32_t out[4];
+    gpo_philox4x32_10(ctr, key, out);
+    return out[step & 3];
+}
 static uint32_t cprop_word(uint64_t seed, uint32_t chain, uint32_t step, int t) {
-    gpo_stream st;
-    gpo_stream_init(&st, seed, chain, step, GPO_P_CPROP, (uint32_t)(t & 63));
-    uint32_t a = gpo_stream_next(&st), b = gpo_stream_next(&st);
-    return t < 64 ? a : b;
+    return step_word(seed, chain, step, GPO_P_CPROP, (uint32_t)(t & 63), (uint32_t)(t >> 6));
 }
 
 /* MH accept rule shared by all config moves: invalid current state always moves
@@ -665,9 +687,7 @@ static int cfg_accept(const gpo_cfg *cfg, uint32_t chain, uint32_t step, double 
     if (!isfinite(cur)) return 1;
     if (!isfinite(prop)) return 0;
     if (a >= 0.0) return 1;
-    gpo_stream st;
-    gpo_stream_init(&st, cfg->seed, chain, step, GPO_P_ACCEPT, 0);
-    float u = gpo_u01(gpo_stream_next(&st));
+    float u = gpo_u01(step_word(cfg->seed, chain, step, GPO_P_ACCEPT, 0, 0));
     return (double)gpo_logf(u) < a;
 }
 
