@@ -901,6 +901,27 @@ static double ref_logplusexp(double a, double b) {      /* main.cu:190 */
 
 static int ref_valid(double x) { return isfinite(x); }
 
+/* n sequential draws of one reference transform from one source (tests):
+ * kind 0 random_int(arg), 1 random_float, 2 random_normal, 3 random_geometric(0.1),
+ * 4 host random_int(arg), 5 host random_float, 6 host random_normal */
+void gpo_ref_draws(int kind, int rng, uint64_t seed, uint32_t lcg_seed, int n, int arg, double *out) {
+    gpo_refcfg cfg = { rng, seed };
+    uint32_t rx = lcg_seed;
+    src_t s;
+    src_sub(&s, &cfg, &rx, 0, 0, 0, 0);
+    for (int i = 0; i < n; i++) {
+        switch (kind) {
+        case 0: out[i] = dev_random_int(arg, &s); break;
+        case 1: out[i] = dev_random_float(&s); break;
+        case 2: out[i] = dev_random_normal(&s); break;
+        case 3: out[i] = dev_random_geometric(&s); break;
+        case 4: out[i] = host_random_int(arg, &s); break;
+        case 5: out[i] = host_random_float(&s); break;
+        default: out[i] = host_random_normal(&s); break;
+        }
+    }
+}
+
 /* MH_simple_kernel, main.cu:679-813, one chain (AoS heap arrays) */
 static void ref_run_chain(const gpo_refcfg *cfg, uint32_t chain, uint64_t step0, int steps,
                           int32_t random_seed, uint8_t *P, float *C,

@@ -147,6 +147,8 @@ void gpo_ref_run_block(const gpo_refcfg *cfg, int nchains, uint32_t chain0, uint
                        const float *x, const float *y, const float *sd, int n,
                        double *prior, double *ll, int nthreads);
 void gpo_ref_init(uint64_t seed, int nchains, uint32_t chain0, uint8_t *hop, float *hc);
+/* n draws of a reference transform (random_int/float/normal/geometric, host variants) */
+void gpo_ref_draws(int kind, int rng, uint64_t seed, uint32_t lcg_seed, int n, int arg, double *out);
 /* median rule of main.cu:1215-1231; returns the number of re-drawn chains */
 int gpo_ref_resample(uint64_t seed, uint32_t block, int nchains, uint32_t chain0,
                      uint8_t *hop, float *hc, const double *prior, const double *ll,

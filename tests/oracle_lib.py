@@ -104,6 +104,7 @@ def lib():
                                        _u8p, _f32p, _i32p, _f64p, _f64p, _i32p]),
         "gpo_ref_run_block": (None, [C.POINTER(GpoRefCfg), C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_int32,
                                      _u8p, _f32p, _f32p, _f32p, _f32p, C.c_int, _f64p, _f64p, C.c_int]),
+        "gpo_ref_draws": (None, [C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_int, C.c_int, _f64p]),
         "gpo_ref_init": (None, [C.c_uint64, C.c_int, C.c_uint32, _u8p, _f32p]),
         "gpo_ref_resample": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, _u8p, _f32p, _f64p, _f64p,
                                        _f64p]),
@@ -247,3 +248,9 @@ def ref_init(seed, nchains, chain0=0):
     hc = np.zeros((31, nchains), dtype=np.float32)
     lib().gpo_ref_init(seed, nchains, chain0, u8(hop), f32(hc))
     return hop, hc
+
+
+def ref_draws(kind, n, rng=0, seed=0, lcg_seed=0, arg=0):
+    out = np.zeros(n, dtype=np.float64)
+    lib().gpo_ref_draws(kind, rng, seed, lcg_seed, n, arg, f64(out))
+    return out

@@ -1,0 +1,70 @@
+"""The C-ABI library on the CPU: it loads, exports every function include/gpuropolis.h
+declares, its host-only entries agree with the oracle, and calls that need a GPU fail
+with an error code and message instead of crashing (no compute calls here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "gpuropolis.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gpr_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(gpr):
+    L = gpr.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(gpr.EXPORTS) <= set(names)
+
+
+def test_op_tables_agree_with_header():
+    txt = open(os.path.join(ROOT, "include", "gpuropolis.h")).read()
+    ops = re.search(r"GPR_OP_X = 0,(.*?)GPR_NUM_OPS", txt, re.S).group(1)
+    names = ["X"] + re.findall(r"GPR_OP_([A-Z]+)", ops)
+    from gpuropolis_amd.lib import OPS
+    assert names == OPS == O.OPS
+    code = re.search(r'GPR_REF_PROGRAM_CODE "([^"]+)"', txt).group(1)
+    assert len(code) == 27                     # strlen(PROGRAM_CODE) == NOPS, main.cu:1026
+
+
+def test_config_defaults(gpr):
+    cfg = gpr.default_config()
+    assert cfg.struct_every == 5 and abs(cfg.sigma - 0.1) < 1e-7 and cfg.resample_nats == 100.0
+    assert abs(cfg.resample_quantile - 0.5) < 1e-7 and cfg.dmax == 16 and cfg.nsym == 10
+
+
+def test_ref_decode_matches_oracle(gpr):
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        hop = rng.integers(0, 27, 31).astype(np.uint8)
+        hc = rng.normal(0, 3, 31).astype(np.float32)
+        a_ops, a_cs = gpr.ref_decode(hop, hc)
+        b_ops, b_cs, _ = O.ref_decode(hop, hc)
+        assert np.array_equal(a_ops, b_ops) and np.array_equal(a_cs.view(np.uint32), b_cs.view(np.uint32))
+
+
+def test_errors_are_reported_not_fatal(gpr):
+    L = gpr.load()
+    cfg = gpr.default_config(cap=500)
+    h = C.c_void_p()
+    rc = L.gpr_create(C.byref(cfg), C.byref(h))
+    assert rc == -1 and b"cap" in L.gpr_last_error()
+    cfg = gpr.default_config()
+    rc = L.gpr_create(C.byref(cfg), C.byref(h))
+    if rc != 0:                                # no GPU in this container
+        assert L.gpr_last_error()
+    else:
+        L.gpr_destroy(h)
+    bad = np.full(31, 200, dtype=np.uint8)
+    assert L.gpr_ref_decode(bad.ctypes.data, np.zeros(31, np.float32).ctypes.data, 1,
+                            np.zeros(64, np.uint8).ctypes.data, np.zeros(64, np.float32).ctypes.data) == -1

@@ -1,0 +1,197 @@
+"""Pins of the oracle's samplers (CPU only), from what the mathematics fixes.
+
+* Subtree-regeneration MH with the description-length prior and no data samples the
+  prior: every program structure S of at most `cap` tokens has probability
+  proportional to nsym^-len(S) (each constant integrates its Cauchy density to 1).
+  A wrong Hastings term (missing 1/len, missing generation probability of the old
+  subtree, misplaced constant density) changes these frequencies.
+* Fixed structure f(x) = c with Gaussian data: the posterior of c (Cauchy prior) is a
+  one-dimensional density whose mean quadrature gives; random-walk and MALA chains
+  must reproduce it.
+* The FP64 adjoint matches central differences of the FP64 log posterior.
+* Resampling rules: MAP-relative gather and the reference's median re-draw.
+"""
+import ctypes as C
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+BIN = ["ADD", "SUB", "MUL", "DIV", "POW"]
+UN = ["EXP", "LOG", "NEG"]
+LEAF = ["X", "CONST"]
+
+
+def structures(cap):
+    """All config-grammar postfix structures of at most cap tokens."""
+    out = set()
+
+    def gen(n):  # trees of exactly n tokens, as tuples of op names (postfix)
+        if n == 1:
+            return [(l,) for l in LEAF]
+        res = [t + (u,) for u in UN for t in gen(n - 1)]
+        for k in range(1, n - 1):
+            res += [a + b + (o,) for o in BIN for a in gen(k) for b in gen(n - 1 - k)]
+        return res
+
+    for n in range(1, cap + 1):
+        out.update(gen(n))
+    return out
+
+
+def test_prior_only_structure_distribution():
+    cap, nch, steps = 3, 6000, 300
+    cfg = O.make_cfg(cap=cap, seed=11, struct_every=1)
+    ops, cs, ln = O.cfg_init(cfg, nch)
+    empty = np.zeros(0, dtype=np.float32)
+    O.cfg_run_block(cfg, ops, cs, ln, empty, empty, empty, 0, steps)
+    S = sorted(structures(cap))
+    assert len(S) == 2 + 6 + 38
+    w = np.array([10.0 ** -len(s) for s in S])
+    p = w / w.sum()
+    index = {s: i for i, s in enumerate(S)}
+    counts = np.zeros(len(S))
+    for c in range(nch):
+        counts[index[tuple(O.OPS[o] for o in ops[:ln[c], c])]] += 1
+    # by length class
+    for L, mass in ((1, 0.2), (2, 0.06), (3, 0.038)):
+        q = mass / 0.298
+        got = np.mean(ln == L)
+        assert abs(got - q) < 4 * np.sqrt(q * (1 - q) / nch), (L, got, q)
+    chi2 = ((counts - nch * p) ** 2 / (nch * p)).sum()
+    assert chi2 < 90, chi2          # 45 dof, p ~ 1e-4
+
+
+def const_posterior(y, s):
+    grid = np.linspace(y.mean() - 2, y.mean() + 2, 400001)
+    lp = -np.log1p(grid ** 2) - ((y[None, :] - grid[:, None]) ** 2).sum(1) / (2 * s * s) if len(y) < 64 else None
+    lp = lp - lp.max()
+    w = np.exp(lp)
+    m = (grid * w).sum() / w.sum()
+    v = ((grid - m) ** 2 * w).sum() / w.sum()
+    return m, np.sqrt(v)
+
+
+@pytest.mark.parametrize("proposal,scale", [(0, 0.8), (1, 1.2)])
+def test_constant_posterior_rw_and_mala(proposal, scale):
+    rng = np.random.default_rng(3)
+    n, s = 12, 0.5
+    y = (1.3 + rng.normal(0, s, n)).astype(np.float32)
+    x = np.zeros(n, dtype=np.float32)
+    sd = np.full(n, s, dtype=np.float32)
+    m_true, sd_true = const_posterior(y.astype(np.float64), s)
+    step = float(scale * sd_true)
+    nch = 4000
+    cfg = O.make_cfg(cap=4, seed=5, proposal=proposal, sigma=step, eps=step, fixed_structure=1)
+    ops = np.zeros((4, nch), dtype=np.uint8)
+    ops[0, :] = O.OP["CONST"]
+    cs = np.zeros((4, nch), dtype=np.float32)
+    cs[0, :] = rng.normal(1.0, 1.0, nch)
+    ln = np.ones(nch, dtype=np.int32)
+    _, _, cnt = O.cfg_run_block(cfg, ops, cs, ln, x, y, sd, 0, 400)
+    c = cs[0].astype(np.float64)
+    se = sd_true / np.sqrt(nch)
+    assert abs(c.mean() - m_true) < 4 * se, (c.mean(), m_true, se)
+    assert abs(c.std() - sd_true) < 0.05 * sd_true
+    acc = cnt[3] / cnt[2]
+    assert 0.2 < acc < 0.95
+
+
+def test_adjoint_matches_central_differences():
+    from gpuropolis_amd.workloads import dataset_D, rational_tape
+    (x, y, sd), b = dataset_D(0)
+    ops, cs = rational_tape(b.astype(np.float64) * 1.05)
+    rc, g, lp = O.tape_grad(ops, cs, x, y, sd)
+    assert rc == 0
+
+    def logpost(c):
+        ll = O.tape_loglik(ops, c, x, y, sd, mode=1)
+        pr = sum(-np.log(np.pi) - np.log1p(float(v) ** 2) for o, v in zip(ops, c) if o == O.OP["CONST"])
+        return ll + pr
+
+    assert abs(logpost(cs) - lp) <= 1e-9 * abs(lp)
+    for t in np.flatnonzero(ops == O.OP["CONST"]):
+        h = 1e-3 * max(1.0, abs(float(cs[t])))
+        cp, cm = cs.copy(), cs.copy()
+        cp[t] += np.float32(h)
+        cm[t] -= np.float32(h)
+        hp, hm = float(cp[t]) - float(cs[t]), float(cs[t]) - float(cm[t])
+        fd = (logpost(cp) - logpost(cm)) / (hp + hm)
+        assert abs(fd - g[t]) <= 1e-4 * max(1.0, abs(g[t])), (t, fd, g[t])
+
+
+def test_map_resample_rule():
+    nch, cap = 500, 8
+    rng = np.random.default_rng(4)
+    ops = rng.integers(0, 3, (cap, nch)).astype(np.uint8)
+    cs = rng.normal(0, 1, (cap, nch)).astype(np.float32)
+    ln = rng.integers(1, cap, nch).astype(np.int32)
+    prior = np.zeros(nch)
+    ll = -rng.exponential(80, nch)
+    ll[7] = np.nan
+    ll[9] = -np.inf
+    post0 = prior + ll
+    src = np.zeros(nch, dtype=np.int32)
+    o2, c2, l2, p2, ll2 = ops.copy(), cs.copy(), ln.copy(), prior.copy(), ll.copy()
+    nbad = O.lib().gpo_cfg_resample(nch, 0, 3, 17, cap, 100.0, O.u8(o2), O.f32(c2), O.i32(l2), O.f64(p2),
+                                    O.f64(ll2), O.i32(src))
+    mp = np.nanmax(post0)
+    good = post0 >= mp - 100
+    assert nbad == int((~good).sum())
+    assert np.all(src[good] == np.flatnonzero(good))
+    assert np.all(good[src])                       # every source is a good chain
+    for c in range(nch):
+        assert l2[c] == ln[src[c]] and np.array_equal(o2[:, c], ops[:, src[c]])
+    assert np.all(p2 + ll2 >= mp - 100)
+
+
+def test_reference_median_resample_rule():
+    nch = 301
+    rng = np.random.default_rng(5)
+    hop, hc = O.ref_init(9, nch)
+    prior = rng.normal(-50, 5, nch)
+    ll = rng.normal(-100, 30, nch)
+    ll[[3, 5, 8]] = np.nan
+    cutoff = np.zeros(1)
+    h2, c2 = hop.copy(), hc.copy()
+    nre = O.lib().gpo_ref_resample(9, 2, nch, 0, O.u8(h2), O.f32(c2), O.f64(prior), O.f64(ll), O.f64(cutoff))
+    post = prior + ll
+    srt = np.concatenate([np.full(np.isnan(post).sum(), -np.inf), np.sort(post[~np.isnan(post)])])
+    assert cutoff[0] == srt[int(nch * np.float32(0.5))]  # qsort with NaN first, main.cu:987-995,1219
+    below = post < cutoff[0]
+    assert nre == int(below.sum())
+    changed = np.any(h2 != hop, axis=0) | np.any(c2 != hc, axis=0)
+    assert not changed[~below].any()               # NaN chains and chains above the cutoff untouched
+    assert changed[below].mean() > 0.99
+    assert h2.max() <= 25                          # re-draws use U{0..NOPS-2}, main.cu:1225
+
+
+def test_reference_init_distribution():
+    hop, hc = O.ref_init(1, 2000)
+    assert hop.min() == 0 and hop.max() == 25      # ABS (26) is never drawn, main.cu:1133
+    mags = np.log10(np.abs(hc[np.abs(hc) > 0]))
+    assert -6 < np.median(mags) < 1
+    assert np.all(np.isfinite(hc[np.abs(hc) < 1e30]))
+
+
+@pytest.mark.parametrize("rng_kind", [0, 1])
+def test_reference_sampler_runs_and_is_deterministic(rng_kind):
+    from gpuropolis_amd.workloads import dataset_S
+    x, y, sd = dataset_S(0)
+    hop, hc = O.ref_init(2, 64)
+    cfg = O.GpoRefCfg(rng_kind, 77)
+    a = [v.copy() for v in (hop, hc)]
+    b = [v.copy() for v in (hop, hc)]
+    pa, la = O.ref_run_block(cfg, a[0], a[1], x, y, sd, 0, 200, random_seed=1234)
+    pb, lb = O.ref_run_block(cfg, b[0], b[1], x, y, sd, 0, 200, random_seed=1234)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    assert np.array_equal(pa, pb) and np.array_equal(la, lb)
+    assert a[0].max() < 27
+    post = pa + la
+    # the chains improve on their random starts
+    p0 = np.array([O.lib().gpo_ref_prior(O.u8(hop[:, c].copy()), O.f32(hc[:, c].copy()), 1)
+                   + O.lib().gpo_ref_likelihood(O.u8(hop[:, c].copy()), O.f32(hc[:, c].copy()), 1, O.f32(x), O.f32(y),
+                                                O.f32(sd), len(x)) for c in range(64)])
+    assert np.nanmedian(post) > np.nanmedian(np.where(np.isfinite(p0), p0, -1e300))
