@@ -21,7 +21,7 @@ CLI = os.path.join(PKG, "bin", "gpuropolis")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GPR_ARCH", "gfx950")
 
-HIP_SOURCES = ["gpr_kernels.hip", "gpr_api.hip"]
+HIP_SOURCES = ["gpr_kernels.hip", "gpr_ref.hip", "gpr_api.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-munsafe-fp-atomics", "-Wall", "-Wno-unused-parameter", "-Wno-unused-function",
              "-I", os.path.join(ROOT, "include")]

@@ -39,7 +39,8 @@ struct gpr_ctx {
     int nd = 0, N = 0, rows = 0;
     std::vector<GprDataset> hds;
     GprDataset *d_ds = nullptr;
-    float *d_x = nullptr, *d_y = nullptr, *d_w = nullptr;
+    float *d_x = nullptr, *d_y = nullptr, *d_w = nullptr, *d_sd = nullptr;
+    double *d_lnc = nullptr;
     uint8_t *d_ops = nullptr;
     float *d_cs = nullptr;
     int32_t *d_len = nullptr;
@@ -85,12 +86,12 @@ extern "C" int gpr_config_default(gpr_config *c) {
 }
 
 static void free_state(gpr_ctx *c) {
-    void *ps[] = {c->d_ds, c->d_x, c->d_y, c->d_w, c->d_ops, c->d_cs, c->d_len, c->d_prior, c->d_ll, c->d_grad,
+    void *ps[] = {c->d_ds, c->d_x, c->d_y, c->d_w, c->d_sd, c->d_lnc, c->d_ops, c->d_cs, c->d_len, c->d_prior, c->d_ll, c->d_grad,
                   c->d_tiles, c->d_tile_counter, c->d_counters, c->d_good, c->d_ngood, c->d_map_chain,
                   c->d_nres, c->d_map_post, c->d_cutoff};
     for (void *p : ps)
         if (p) (void)hipFree(p);
-    c->d_ds = nullptr; c->d_x = c->d_y = c->d_w = nullptr; c->d_ops = nullptr; c->d_cs = nullptr;
+    c->d_ds = nullptr; c->d_x = c->d_y = c->d_w = c->d_sd = nullptr; c->d_lnc = nullptr; c->d_ops = nullptr; c->d_cs = nullptr;
     c->d_len = nullptr; c->d_prior = c->d_ll = nullptr; c->d_grad = nullptr; c->d_tiles = nullptr;
     c->d_tile_counter = nullptr; c->d_counters = nullptr; c->d_good = c->d_ngood = c->d_map_chain = nullptr;
     c->d_nres = nullptr; c->d_map_post = c->d_cutoff = nullptr;
@@ -169,6 +170,7 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     }
     if (total_chains > (1ll << 30)) return fail(GPR_E_ARG, "too many chains");
     std::vector<float> w((size_t)total_pts);
+    std::vector<double> lnc((size_t)total_pts);
     c->hds.assign(nd, GprDataset{});
     long long off = 0, ch = 0;
     for (int d = 0; d < nd; d++) {
@@ -180,13 +182,22 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
         g.cap = ref ? 63 : (caps ? caps[d] : c->cfg.cap);
         g.kc = kc_for(g.n);
         double lc = 0.0;
+        bool same_sd = true;
         for (int i = 0; i < n[d]; i++) {
             const double s = sd[off + i];
             if (!(s > 0.0) || !std::isfinite(s) || !std::isfinite(x[off + i]) || !std::isfinite(y[off + i]))
                 return fail(GPR_E_ARG, "dataset " + std::to_string(d) + " point " + std::to_string(i) +
                                            ": need finite x, y and sd > 0 (main.cu:332)");
-            w[off + i] = (float)(1.0 / (2.0 * s * s));
-            lc += -log(s) - GPR_HALF_LOG2PI_D;
+            same_sd = same_sd && sd[off + i] == sd[off];
+            // lnormalpdf's normalizer with float logs (main.cu:300)
+            lnc[off + i] = 0.5 * ((double)0x1.d67f1ep+0f + 2.0 * (double)logf(sd[off + i]));
+            lc -= lnc[off + i];
+        }
+        const double s0 = sd[off];
+        g.wconst = same_sd ? 1.0 / (2.0 * s0 * s0) : 0.0;
+        for (int i = 0; i < n[d]; i++) {
+            const double s = sd[off + i];
+            w[off + i] = same_sd ? 1.0f : (float)(1.0 / (2.0 * s * s));
         }
         g.lconst = lc;
         off += n[d];
@@ -217,6 +228,8 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     ALLOC(c->d_x, (size_t)total_pts);
     ALLOC(c->d_y, (size_t)total_pts);
     ALLOC(c->d_w, (size_t)total_pts);
+    ALLOC(c->d_sd, (size_t)total_pts);
+    ALLOC(c->d_lnc, (size_t)total_pts);
     ALLOC(c->d_ops, (size_t)c->rows * N);
     ALLOC(c->d_cs, (size_t)c->rows * N);
     ALLOC(c->d_len, N);
@@ -236,6 +249,8 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     HIP_TRY(hipMemcpy(c->d_x, x, sizeof(float) * total_pts, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_y, y, sizeof(float) * total_pts, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_w, w.data(), sizeof(float) * total_pts, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_sd, sd, sizeof(float) * total_pts, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_lnc, lnc.data(), sizeof(double) * total_pts, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_tiles, c->htiles.data(), sizeof(GprTile) * c->htiles.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(c->d_counters, 0, sizeof(unsigned long long) * GPR_NCNT));
     HIP_TRY(hipMemset(c->d_ops, 0, (size_t)c->rows * N));
@@ -254,6 +269,8 @@ static GprMHArgs mh_args(gpr_ctx *c, int steps, int32_t block_seed) {
     a.ops = c->d_ops; a.cs = c->d_cs; a.len = c->d_len; a.prior = c->d_prior; a.ll = c->d_ll; a.grad = c->d_grad;
     a.nchains = c->N;
     a.dx = c->d_x; a.dy = c->d_y; a.dw = c->d_w;
+    a.dsd = c->d_sd;
+    a.lnc = c->d_lnc;
     a.ds = c->d_ds;
     a.tiles = c->d_tiles;
     a.ntiles = (int)c->htiles.size();
@@ -280,7 +297,7 @@ extern "C" int gpr_init_chains(gpr_ctx *c) {
     if (c->cfg.semantics == GPR_SEM_CONFIG) {
         HIP_TRY(gpr_launch_init_config(a, c->stream));
     } else {
-        return fail(GPR_E_UNSUPPORTED, "reference-semantics init not built yet");
+        HIP_TRY(gpr_launch_init_ref(a, c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->have_state = true;
@@ -340,18 +357,25 @@ extern "C" int gpr_read_state(gpr_ctx *c, uint8_t *ops, float *cs, int32_t *len,
 static int launch_block(gpr_ctx *c, int steps, int32_t block_seed) {
     GprMHArgs a = mh_args(c, steps, block_seed);
     const bool ref = c->cfg.semantics == GPR_SEM_REFERENCE;
-    int sched = c->cfg.schedule == GPR_SCHED_AUTO ? GPR_SCHED_WAVE : c->cfg.schedule;
-    if (ref) return fail(GPR_E_UNSUPPORTED, "reference-semantics MH kernel not built yet");
-    if (sched == GPR_SCHED_LANE) return fail(GPR_E_UNSUPPORTED, "lane schedule MH kernel not built yet");
-    HIP_TRY(gpr_launch_fill_i32(c->d_tile_counter, (int)c->buckets.size(), 0, c->stream));
-    for (size_t b = 0; b < c->buckets.size(); b++) {
-        const KernelBucket &kb = c->buckets[b];
-        GprMHArgs ab = a;
-        ab.tiles = c->d_tiles + kb.tile0;
-        ab.ntiles = kb.ntiles;
-        ab.tile_counter = c->d_tile_counter + b;
-        const int grid = std::min(kb.ntiles, c->cus * 8);
-        HIP_TRY(gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, grid, c->stream));
+    int sched = c->cfg.schedule;
+    if (sched == GPR_SCHED_AUTO) sched = (ref && c->cfg.rng == GPR_RNG_LCG) ? GPR_SCHED_LANE : GPR_SCHED_WAVE;
+    if (c->cfg.rng == GPR_RNG_LCG && sched != GPR_SCHED_LANE)
+        return fail(GPR_E_ARG, "the LCG stream is sequential per chain: it needs the lane schedule");
+    if (sched == GPR_SCHED_LANE) {
+        if (ref) HIP_TRY(gpr_launch_mh_ref_lane(a, c->stream));
+        else return fail(GPR_E_UNSUPPORTED, "lane schedule MH kernel not built yet");
+    } else {
+        HIP_TRY(gpr_launch_fill_i32(c->d_tile_counter, (int)c->buckets.size(), 0, c->stream));
+        for (size_t b = 0; b < c->buckets.size(); b++) {
+            const KernelBucket &kb = c->buckets[b];
+            GprMHArgs ab = a;
+            ab.tiles = c->d_tiles + kb.tile0;
+            ab.ntiles = kb.ntiles;
+            ab.tile_counter = c->d_tile_counter + b;
+            const int grid = std::min(kb.ntiles, c->cus * 8);
+            if (ref) HIP_TRY(gpr_launch_mh_ref_wave(ab, kb.kc, grid, c->stream));
+            else HIP_TRY(gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, grid, c->stream));
+        }
     }
     c->sched_used = sched;
     c->step += (uint32_t)steps;
@@ -405,9 +429,14 @@ extern "C" int gpr_resample(gpr_ctx *c) {
     if (!c || !c->have_state) return fail(GPR_E_STATE, "no chain state");
     if (c->cfg.resample == GPR_RS_NONE) return GPR_OK;
     HIP_TRY(hipSetDevice(c->cfg.device));
-    if (c->cfg.resample != GPR_RS_MAP) return fail(GPR_E_UNSUPPORTED, "quantile resampling not built yet");
     HIP_TRY(gpr_launch_fill_i32(c->d_nres, 1, 0, c->stream));
-    HIP_TRY(gpr_launch_resample(rs_args(c), c->stream));
+    if (c->cfg.resample == GPR_RS_QUANTILE) {
+        if (c->cfg.semantics != GPR_SEM_REFERENCE)
+            return fail(GPR_E_ARG, "the quantile re-draw rule draws heap programs: reference semantics only");
+        HIP_TRY(gpr_launch_resample_quantile(rs_args(c), c->stream));
+    } else {
+        HIP_TRY(gpr_launch_resample(rs_args(c), c->stream));
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     return GPR_OK;
 }
@@ -510,4 +539,22 @@ extern "C" int gpr_ref_decode(const uint8_t *hop, const float *hc, size_t stride
     const int n = decode_heap(hop, hc, stride, 1, 0, ops, cs);
     if (n < 0) return fail(GPR_E_ARG, "bad heap op code");
     return n;
+}
+
+// ---------------------------------------------------------------- test hooks
+extern "C" int gpr_test_ref_decode(gpr_ctx *c, const uint8_t *d_hop, const float *d_hc, int nheaps, uint8_t *d_ops,
+                                   float *d_cs, int32_t *d_meta) {
+    if (!c || !d_hop || !d_hc || !d_ops || !d_cs || !d_meta || nheaps < 1) return fail(GPR_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    HIP_TRY(gpr_launch_test_ref_decode(d_hop, d_hc, nheaps, d_ops, d_cs, d_meta, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
+}
+
+extern "C" int gpr_test_draws(gpr_ctx *c, const uint32_t *d_words, int n, uint64_t seed, float *d_out) {
+    if (!c || !d_words || !d_out || n < 1) return fail(GPR_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    HIP_TRY(gpr_launch_test_draws(d_words, n, seed, d_out, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
 }

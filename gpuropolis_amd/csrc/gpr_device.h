@@ -1,0 +1,356 @@
+// Device building blocks shared by the config and reference kernels: wave-level
+// primitives (DPP reductions and scans, readlane/writelane), FP32 op semantics, the
+// wave-uniform tape interpreter with its VGPR value stack, the Gaussian
+// log-likelihood and description-length prior of a wave-resident program.
+#pragma once
+#include "gpr_internal.h"
+
+#define GPR_INF (__builtin_inf())
+
+// ---------------------------------------------------------------- wave primitives
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rl_f(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double rl_d(double v, int l) {
+    uint64_t b = __builtin_bit_cast(uint64_t, v);
+    int lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    int hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ int wl_i(int old, int v, int l) { return lane_id() == l ? v : old; }
+__device__ __forceinline__ float wl_f(float old, float v, int l) { return lane_id() == l ? v : old; }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Lane gathers that stay in wave-uniform control flow.  ds_bpermute only sees the
+// values of lanes active when it executes; the empty asm pins the result at this point
+// so the compiler cannot sink the gather into a branch that deactivates source lanes.
+__device__ __forceinline__ int shfl_all(int v, int src) {
+    int r = __shfl(v, src);
+    __asm__ volatile("" : "+v"(r));
+    return r;
+}
+__device__ __forceinline__ float shfl_all(float v, int src) {
+    float r = __shfl(v, src);
+    __asm__ volatile("" : "+v"(r));
+    return r;
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// DPP controls (gfx9 encoding): quad_perm, row_shr:n, row_mirror, row_half_mirror,
+// row_bcast:15, row_bcast:31.  DPP moves are VALU operand modifiers: no LDS round trip.
+#define DPP_QP_1032 0xB1
+#define DPP_QP_2301 0x4E
+#define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_ROW_MIRROR 0x140
+#define DPP_ROW_HALF_MIRROR 0x141
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, false);
+}
+
+// Sum over the 64 lanes, returned uniform.  Butterflies inside each 16-lane row, then
+// the four row sums combined in a fixed order.
+__device__ __forceinline__ float wave_sum_f(float v) {
+    v += dpp_f<DPP_QP_1032>(v);
+    v += dpp_f<DPP_QP_2301>(v);
+    v += dpp_f<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_f<DPP_ROW_MIRROR>(v);
+    return (rl_f(v, 0) + rl_f(v, 16)) + (rl_f(v, 32) + rl_f(v, 48));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+    v += dpp_d<DPP_QP_1032>(v);
+    v += dpp_d<DPP_QP_2301>(v);
+    v += dpp_d<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_d<DPP_ROW_MIRROR>(v);
+    return (rl_d(v, 0) + rl_d(v, 16)) + (rl_d(v, 32) + rl_d(v, 48));
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+    v = max(v, dpp_i<DPP_QP_1032>(v));
+    v = max(v, dpp_i<DPP_QP_2301>(v));
+    v = max(v, dpp_i<DPP_ROW_HALF_MIRROR>(v));
+    v = max(v, dpp_i<DPP_ROW_MIRROR>(v));
+    return max(max(rl_i(v, 0), rl_i(v, 16)), max(rl_i(v, 32), rl_i(v, 48)));
+}
+// Inclusive prefix sum over the 64 lanes: Kogge-Stone inside rows (row_shr with zero
+// fill), then row 15 -> row 1,3 and lane 31 -> rows 2,3 broadcasts.
+__device__ __forceinline__ int wave_scan_i(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(1), 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(2), 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(4), 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(8), 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST15, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST31, 0xc, 0xf, false);
+    return v;
+}
+
+// ---------------------------------------------------------------- op semantics (FP32)
+// Values of dispatch_eval (main.cu:397-444) with IEEE division and accurate libm.
+__device__ __forceinline__ float op_bin(int op, float a, float b) {
+    switch (op) {
+    case GPR_OP_ADD: return a + b;
+    case GPR_OP_SUB: return a - b;
+    case GPR_OP_MUL: return a * b;
+    case GPR_OP_DIV: return a / b;
+    case GPR_OP_POW: return powf(a, b);
+    case GPR_OP_RSUB: return b - a;
+    case GPR_OP_RDIV: return b / a;
+    default: return powf(b, a);  // RPOW
+    }
+}
+__device__ __forceinline__ float op_un(int op, float a, float c) {
+    switch (op) {
+    case GPR_OP_NEG: return -a;
+    case GPR_OP_EXP: return expf(a);
+    case GPR_OP_LOG: return logf(a);
+    case GPR_OP_INV: return 1.0f / a;
+    case GPR_OP_SQRT: return sqrtf(a);
+    case GPR_OP_SIN: return sinf(a);
+    case GPR_OP_ASIN: return asinf(a);
+    case GPR_OP_COS: return cosf(a);
+    case GPR_OP_ACOS: return acosf(a);
+    case GPR_OP_TAN: return tanf(a);
+    case GPR_OP_ATAN: return atanf(a);
+    case GPR_OP_ABS: return fabsf(a);
+    case GPR_OP_CADD: return a + c;
+    case GPR_OP_CMUL: return c * a;
+    case GPR_OP_CPOW: return powf(a, c);
+    case GPR_OP_CRPOW: return powf(c, a);
+    default: return __builtin_nanf("");
+    }
+}
+
+// ---------------------------------------------------------------- wave program state
+// Lane l holds token l in (o0, c0) and token l+64 in (o1, c1); tokens >= len are 0.
+struct WaveProg {
+    int o0, o1;
+    float c0, c1;
+    int len;
+};
+
+// Per-lane value stack: KC vector values of GPR_MAX_DEPTH floats.  Element access
+// with the uniform stack pointer lowers to s_set_gpr_idx moves; nothing goes through
+// an alloca, so nothing is spilled to scratch.
+typedef float gpr_f16v __attribute__((ext_vector_type(GPR_MAX_DEPTH)));
+template <int KC>
+struct VStack {
+    gpr_f16v s0, s1, s2, s3;
+    __device__ __forceinline__ float get(int k, int i) const {
+        return k == 0 ? s0[i] : (k == 1 ? s1[i] : (k == 2 ? s2[i] : s3[i]));
+    }
+    __device__ __forceinline__ void put(int k, int i, float v) {
+        if (k == 0) s0[i] = v; else if (k == 1) s1[i] = v; else if (k == 2) s2[i] = v; else s3[i] = v;
+    }
+};
+
+#define GPR_KLOOP(expr)                              \
+    do {                                             \
+        _Pragma("unroll") for (int k = 0; k < KC; k++) T[k] = (expr); \
+    } while (0)
+
+// Interpret the tape at KC points per lane.  op is wave-uniform: every branch below is
+// a scalar branch and only the selected operation executes.
+template <int KC, bool HI>
+__device__ __forceinline__ void wave_eval(const WaveProg &p, const float (&xs)[KC], float (&T)[KC]) {
+    VStack<KC> S;
+    int sp = 0;
+#pragma unroll
+    for (int k = 0; k < KC; k++) T[k] = 0.0f;
+    for (int t = 0; t < p.len; t++) {
+        const int l = t & 63;
+        const int op = (HI && t >= 64) ? rl_i(p.o1, l) : rl_i(p.o0, l);
+        if (op <= GPR_OP_ONE) {
+            // the write index is bounded for malformed tapes; valid ones (depth <= 16) never hit it
+            const int wi = min(sp, GPR_MAX_DEPTH - 1);
+#pragma unroll
+            for (int k = 0; k < KC; k++) S.put(k, wi, T[k]);
+            sp = min(sp + 1, GPR_MAX_DEPTH);
+            if (op == GPR_OP_X) {
+                GPR_KLOOP(xs[k]);
+            } else {
+                const float c = op == GPR_OP_CONST ? ((HI && t >= 64) ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 1.0f;
+                GPR_KLOOP(c);
+            }
+            continue;
+        }
+        if (op <= GPR_OP_RPOW) {
+            sp = max(sp - 1, 0);
+            switch (op) {
+            case GPR_OP_ADD: GPR_KLOOP(S.get(k, sp) + T[k]); break;
+            case GPR_OP_SUB: GPR_KLOOP(S.get(k, sp) - T[k]); break;
+            case GPR_OP_MUL: GPR_KLOOP(S.get(k, sp) * T[k]); break;
+            case GPR_OP_DIV: GPR_KLOOP(S.get(k, sp) / T[k]); break;
+            case GPR_OP_RSUB: GPR_KLOOP(T[k] - S.get(k, sp)); break;
+            case GPR_OP_RDIV: GPR_KLOOP(T[k] / S.get(k, sp)); break;
+            case GPR_OP_POW: GPR_KLOOP(powf(S.get(k, sp), T[k])); break;
+            default: GPR_KLOOP(powf(T[k], S.get(k, sp))); break;
+            }
+            continue;
+        }
+        const float c = op >= GPR_OP_CADD ? ((HI && t >= 64) ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 0.0f;
+        switch (op) {
+        case GPR_OP_NEG: GPR_KLOOP(-T[k]); break;
+        case GPR_OP_EXP: GPR_KLOOP(expf(T[k])); break;
+        case GPR_OP_LOG: GPR_KLOOP(logf(T[k])); break;
+        case GPR_OP_INV: GPR_KLOOP(1.0f / T[k]); break;
+        case GPR_OP_SQRT: GPR_KLOOP(sqrtf(T[k])); break;
+        case GPR_OP_SIN: GPR_KLOOP(sinf(T[k])); break;
+        case GPR_OP_ASIN: GPR_KLOOP(asinf(T[k])); break;
+        case GPR_OP_COS: GPR_KLOOP(cosf(T[k])); break;
+        case GPR_OP_ACOS: GPR_KLOOP(acosf(T[k])); break;
+        case GPR_OP_TAN: GPR_KLOOP(tanf(T[k])); break;
+        case GPR_OP_ATAN: GPR_KLOOP(atanf(T[k])); break;
+        case GPR_OP_ABS: GPR_KLOOP(fabsf(T[k])); break;
+        case GPR_OP_CADD: GPR_KLOOP(T[k] + c); break;
+        case GPR_OP_CMUL: GPR_KLOOP(c * T[k]); break;
+        case GPR_OP_CPOW: GPR_KLOOP(powf(T[k], c)); break;
+        case GPR_OP_CRPOW: GPR_KLOOP(powf(c, T[k])); break;
+        default: GPR_KLOOP(__builtin_nanf("")); break;
+        }
+    }
+}
+
+// Dataset points of one chunk: point index base + k*64 + lane.
+template <int KC>
+struct Chunk {
+    float x[KC], y[KC], w[KC];
+    __device__ __forceinline__ void load(const float *dx, const float *dy, const float *dw, int n, int base) {
+        const int l = lane_id();
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            const int i = base + k * 64 + l;
+            const bool v = i < n;
+            x[k] = v ? dx[i] : 0.0f;
+            y[k] = v ? dy[i] : 0.0f;
+            w[k] = v ? dw[i] : 0.0f;
+        }
+    }
+};
+
+// Gaussian log-likelihood of the wave's program: lconst - sum_i (f_i - y_i)^2 w_i.
+// Per-lane Kahan sums, wave tree reduction; -inf if any term is not finite.
+template <int KC, bool HI>
+__device__ double wave_loglik(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
+                              const float *dw, const Chunk<KC> &c0, int nchunks) {
+    float acc = 0.0f, comp = 0.0f;
+    const int l = lane_id();
+    for (int ch = 0; ch < nchunks; ch++) {
+        Chunk<KC> c;
+        if (nchunks == 1) c = c0; else c.load(dx, dy, dw, d.n, ch * 64 * KC);
+        float T[KC];
+        wave_eval<KC, HI>(p, c.x, T);
+#pragma unroll
+        for (int k = 0; k < KC; k++) {
+            const float r = T[k] - c.y[k];
+            const float term = (ch * 64 * KC + k * 64 + l < d.n) ? (r * r) * c.w[k] : 0.0f;
+            const float yv = term - comp;
+            const float tv = acc + yv;
+            comp = (tv - acc) - yv;
+            acc = tv;
+        }
+    }
+    // each lane's compensated partial, combined across the wave in FP64
+    const double s = wave_sum_d((double)acc - (double)comp);
+    if (!__builtin_isfinite(s)) return -GPR_INF;
+    return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
+}
+
+// Description-length prior plus Cauchy(0,1) on every constant token, summed in token
+// order in FP64 (the same order as the oracle, so the value is bit-reproducible).
+template <bool HI>
+__device__ double wave_prior(const WaveProg &p, double lnsym) {
+    const int l = lane_id();
+    const bool v0 = l < p.len, v1 = HI && (l + 64) < p.len;
+    int dl = p.len + __popcll(ballot(v0 && p.o0 >= GPR_OP_CADD)) + (HI ? __popcll(ballot(v1 && p.o1 >= GPR_OP_CADD)) : 0);
+    uint64_t m0 = ballot(v0 && gpr_has_const(p.o0));
+    const double lc0 = gpr_lcauchy(p.c0);
+    double s = 0.0;
+    while (m0) {
+        const int b = __builtin_ctzll(m0);
+        m0 &= m0 - 1;
+        s += rl_d(lc0, b);
+    }
+    if (HI) {
+        uint64_t m1 = ballot(v1 && gpr_has_const(p.o1));
+        if (m1) {
+            const double lc1 = gpr_lcauchy(p.c1);
+            while (m1) {
+                const int b = __builtin_ctzll(m1);
+                m1 &= m1 - 1;
+                s += rl_d(lc1, b);
+            }
+        }
+    }
+    return gpr_dl_prior(dl, lnsym, s);
+}
+
+// Subtree [s0, j] ending at token j: s0-1 is the last u < j whose stack depth after u is
+// below the depth after j.  Also returns the maximum depth of the tape.
+template <bool HI>
+__device__ __forceinline__ void wave_depths(const WaveProg &p, int &D0, int &D1, int &dmax) {
+    const int l = lane_id();
+    const int d0 = l < p.len ? 1 - gpr_arity(p.o0) : 0;
+    D0 = wave_scan_i(d0);
+    dmax = wave_max_i(l < p.len ? D0 : 0);
+    D1 = 0;
+    if (HI) {
+        const int tot = rl_i(D0, 63);
+        const int d1 = l + 64 < p.len ? 1 - gpr_arity(p.o1) : 0;
+        D1 = wave_scan_i(d1) + tot;
+        dmax = max(dmax, wave_max_i(l + 64 < p.len ? D1 : 0));
+    }
+}
+
+template <bool HI>
+__device__ __forceinline__ int subtree_start(int j, int D0, int D1) {
+    const int l = lane_id();
+    const int Dj = (HI && j >= 64) ? rl_i(D1, j - 64) : rl_i(D0, j);
+    uint64_t m1 = HI ? ballot(l + 64 < j && D1 < Dj) : 0;
+    if (m1) return 64 + (63 - __builtin_clzll(m1)) + 1;
+    uint64_t m0 = ballot(l < j && D0 < Dj);
+    if (m0) return (63 - __builtin_clzll(m0)) + 1;
+    return 0;
+}
+
+// value of a 128-token lane array at token index src (src may differ per lane)
+__device__ __forceinline__ int gather_i(int v0, int v1, int src, bool hi) {
+    const int a = shfl_all(v0, src & 63);
+    if (!hi) return a;
+    const int b = shfl_all(v1, src & 63);
+    return src >= 64 ? b : a;
+}
+__device__ __forceinline__ float gather_f(float v0, float v1, int src, bool hi) {
+    const float a = shfl_all(v0, src & 63);
+    if (!hi) return a;
+    const float b = shfl_all(v1, src & 63);
+    return src >= 64 ? b : a;
+}
+
+// config accept rule: invalid current always moves; log u < a otherwise.  u lies on the
+// grid ((k + 1/2) 2^-23), so log u >= -16.64: any a < -17 rejects without drawing.
+__device__ __forceinline__ bool cfg_accept(uint64_t seed, uint32_t chain, uint32_t step, double cur, double prop,
+                                           double a) {
+    if (!__builtin_isfinite(cur)) return true;
+    if (!__builtin_isfinite(prop)) return false;
+    if (a >= 0.0) return true;
+    if (a < -17.0) return false;
+    const gpr_u4 w = gpr_step_block(seed, chain, step, GPR_P_ACCEPT, 0, 0);
+    const float u = gpr_u01(gpr_u4_word(w, (int)(step & 3)));
+    return (double)gpr_logf(u) < a;
+}
+

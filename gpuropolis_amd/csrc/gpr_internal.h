@@ -12,7 +12,8 @@ struct GprDataset {
     int nchains;    // chains of this dataset
     int cap;        // token cap of this dataset's programs
     int kc;         // points per lane per chunk chosen for the wave schedule
-    double lconst;
+    double lconst;  // -sum_i 0.5*(logf(2 pi) + 2 logf(sd_i)), host libm, FP64 sum
+    double wconst;  // 1/(2 sd^2) in FP64 when sd is constant (then w[] holds 1), else 0
 };
 
 // A unit of work for the wave kernels: up to GPR_WAVES_PER_WG chains of one dataset.
@@ -40,6 +41,8 @@ struct GprMHArgs {
     float *grad;            // [cap][N] MALA gradient cache (may be null)
     int nchains;            // N, the stride
     const float *dx, *dy, *dw;
+    const float *dsd;       // sd per point (reference lane kernel)
+    const double *lnc;      // 0.5*(logf(2 pi) + 2 logf(sd)) per point, host libm (main.cu:300)
     const GprDataset *ds;
     const GprTile *tiles;
     int ntiles;
@@ -106,4 +109,8 @@ hipError_t gpr_launch_grad_wave(const GprEvalArgs &a, hipStream_t s);
 hipError_t gpr_launch_init_config(const GprMHArgs &a, hipStream_t s);
 hipError_t gpr_launch_init_ref(const GprMHArgs &a, hipStream_t s);
 hipError_t gpr_launch_resample(const GprResampleArgs &a, hipStream_t s);
+hipError_t gpr_launch_resample_quantile(const GprResampleArgs &a, hipStream_t s);
 hipError_t gpr_launch_fill_i32(int *p, int n, int v, hipStream_t s);
+hipError_t gpr_launch_test_ref_decode(const uint8_t *hop, const float *hc, int nheaps, uint8_t *ops, float *cs,
+                                      int32_t *meta, hipStream_t s);
+hipError_t gpr_launch_test_draws(const uint32_t *words, int n, uint64_t seed, float *out, hipStream_t s);

@@ -30,7 +30,7 @@ MAX_CAP, MAX_DEPTH = 128, 16
 EXPORTS = ["gpr_config_default", "gpr_create", "gpr_destroy", "gpr_last_error", "gpr_version", "gpr_set_data",
            "gpr_init_chains", "gpr_write_state", "gpr_read_state", "gpr_run_block", "gpr_run_block_async",
            "gpr_resample", "gpr_run_blocks_graph", "gpr_get_stats", "gpr_reset_stats", "gpr_sync", "gpr_stream",
-           "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode"]
+           "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws"]
 
 
 class Config(C.Structure):
@@ -85,6 +85,8 @@ def load():
         "gpr_eval_loglik": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp]),
         "gpr_grad_logpost": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, vp, vp]),
         "gpr_ref_decode": (C.c_int, [vp, vp, C.c_size_t, vp, vp]),
+        "gpr_test_ref_decode": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, vp]),
+        "gpr_test_draws": (C.c_int, [vp, vp, C.c_int, C.c_uint64, vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -206,6 +208,12 @@ class Context:
     def eval_loglik_dev(self, dataset, d_ops, d_cs, d_len, nchains, cap, schedule, d_ll):
         """All pointers are device addresses (ints)."""
         check(load().gpr_eval_loglik(self.h, dataset, d_ops, d_cs, d_len, nchains, cap, schedule, d_ll))
+
+    def test_ref_decode_dev(self, d_hop, d_hc, nheaps, d_ops, d_cs, d_meta):
+        check(load().gpr_test_ref_decode(self.h, d_hop, d_hc, nheaps, d_ops, d_cs, d_meta))
+
+    def test_draws_dev(self, d_words, n, seed, d_out):
+        check(load().gpr_test_draws(self.h, d_words, n, seed, d_out))
 
     def grad_logpost_dev(self, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp):
         check(load().gpr_grad_logpost(self.h, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp))

@@ -161,6 +161,18 @@ int gpr_grad_logpost(gpr_ctx *ctx, int dataset, const uint8_t *d_ops, const floa
  * ops/cs receive up to 63 tokens; returns the token count or a negative error. */
 int gpr_ref_decode(const uint8_t *hop, const float *hc, size_t stride, uint8_t *ops, float *cs);
 
+/* ---------------------------------------------------------------- test hooks */
+/* No reference counterpart: device-side pieces exposed so tests can compare them with
+ * the host bit for bit.  gpr_test_ref_decode runs the wave decode on heaps [nheaps][31]
+ * and writes tapes [nheaps][64] and meta [nheaps][3] = {length, count_nodes, description
+ * length}.  gpr_test_draws writes 10 floats per input word: u01, normal quantile,
+ * logf(bits & 0x7fffffff), expf(int(word) * 2^-24), sin2pi(u01), and the reference's
+ * random_float, random_normal, random_geometric, host random_normal and random_int(27)
+ * drawn in sequence from the substream (seed, chain = index, 0, 0, 0). */
+int gpr_test_ref_decode(gpr_ctx *ctx, const uint8_t *d_hop, const float *d_hc, int nheaps, uint8_t *d_ops,
+                        float *d_cs, int32_t *d_meta);
+int gpr_test_draws(gpr_ctx *ctx, const uint32_t *d_words, int n, uint64_t seed, float *d_out);
+
 #ifdef __cplusplus
 }
 #endif

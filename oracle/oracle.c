@@ -502,6 +502,18 @@ double gpo_eval_tape_f64(const uint8_t *ops, const float *cs, int len, double x)
 double gpo_tape_loglik(const uint8_t *ops, const float *cs, int len,
                        const float *x, const float *y, const float *sd, int n, int mode) {
     double ll = 0.0;
+    if (mode == 2) {
+        /* the sampler's definition: lnormalpdf of main.cu:299-301 on FP32 node values,
+         * FP32 residual and square, FP64 division and sum */
+        for (int i = 0; i < n; i++) {
+            float f = gpo_eval_tape_f32(ops, cs, len, x[i]);
+            if (!isfinite(f)) return -INFINITY;
+            double l = gpo_ref_lnormalpdf(f - y[i], sd[i]);
+            if (!isfinite(l)) return -INFINITY;
+            ll += l;
+        }
+        return ll;
+    }
     for (int i = 0; i < n; i++) {
         double f = mode == 0 ? (double)gpo_eval_tape_f32(ops, cs, len, x[i])
                              : gpo_eval_tape_f64(ops, cs, len, (double)x[i]);
@@ -700,7 +712,7 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
     const double lns = log((double)cfg->nsym);
     int len = *lenp;
     double prior = gpo_tape_prior(ops, cs, len, lns, 0);
-    double ll = gpo_tape_loglik(ops, cs, len, x, y, sd, n, 0);
+    double ll = gpo_tape_loglik(ops, cs, len, x, y, sd, n, 2);
     double grad[256];
     int grad_ok = 0;
     uint8_t pops[256]; float pcs[256];
@@ -728,7 +740,7 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
             double lq_old = 0.0;
             for (int t = s0; t <= j; t++) { lq_old += logp[ops[t]]; if (ops[t] == GPO_CONST) lq_old += lphi(cs[t]); }
             double pprior = gpo_tape_prior(pops, pcs, plen, lns, 0);
-            double pll = gpo_tape_loglik(pops, pcs, plen, x, y, sd, n, 0);
+            double pll = gpo_tape_loglik(pops, pcs, plen, x, y, sd, n, 2);
             double fwd = -log((double)len) + lq_new;
             double bwd = -log((double)plen) + lq_old;
             double a = (pprior + pll) - cur + bwd - fwd;
@@ -778,7 +790,7 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
                 }
             }
             double pprior = gpo_tape_prior(ops, pcs, len, lns, 0);
-            double pll = gpo_tape_loglik(ops, pcs, len, x, y, sd, n, 0);
+            double pll = gpo_tape_loglik(ops, pcs, len, x, y, sd, n, 2);
             double a = (pprior + pll) - cur + a_extra;
             if (cfg_accept(cfg, chain, step, cur, pprior + pll, a)) {
                 memcpy(cs, pcs, sizeof(float) * (size_t)len);

@@ -87,7 +87,9 @@ int   gpo_subtree_start(const uint8_t *ops, int j);
 float gpo_eval_tape_f32(const uint8_t *ops, const float *cs, int len, float x);
 double gpo_eval_tape_f64(const uint8_t *ops, const float *cs, int len, double x);
 /* mode 0: FP32 node values, FP64 terms and sum (the north-star oracle)
- * mode 1: FP64 node values and sum */
+ * mode 1: FP64 node values and sum
+ * mode 2: the samplers' definition, lnormalpdf of main.cu:299-301 on FP32 node values
+ *         (FP32 residual and square, FP64 division and sum) */
 double gpo_tape_loglik(const uint8_t *ops, const float *cs, int len,
                        const float *x, const float *y, const float *sd, int n, int mode);
 double gpo_tape_prior(const uint8_t *ops, const float *cs, int len, double log_nsym,

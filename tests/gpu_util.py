@@ -23,3 +23,16 @@ def ll_close(gpu, ref, n, rel=1e-5):
 
 def say(*a):
     print(*a, flush=True)
+
+
+def kappa(oracle, ops, cs, x, y, sd):
+    """Sensitivity of a program's LL to FP32 rounding: the larger of the FP32-node vs
+    FP64-node gap and the LL change under a one-ulp move of every x (FP64 nodes)."""
+    r0 = oracle.tape_loglik(ops, cs, x, y, sd, 0)
+    r1 = oracle.tape_loglik(ops, cs, x, y, sd, 1)
+    xp = np.nextafter(np.asarray(x, np.float32), np.float32(np.inf))
+    r1p = oracle.tape_loglik(ops, cs, xp, y, sd, 1)
+    vals = [abs(r0 - r1), abs(r1p - r1)]
+    if not all(np.isfinite(v) for v in vals):
+        return np.inf
+    return max(vals)

@@ -9,7 +9,7 @@ log-ratio lies within rounding of the threshold; at most one such chain is allow
 import numpy as np
 import pytest
 
-from gpu_util import ll_close, say
+from gpu_util import kappa, ll_close, say
 
 pytestmark = pytest.mark.gpu
 
@@ -63,10 +63,9 @@ def test_mh_trajectories_match_oracle(oracle, gpr, cap, n, steps):
     assert same.sum() >= int(0.95 * nch)
     c = np.flatnonzero(same)
     assert np.array_equal(g_pr[c], p1[c])          # prior is bit-reproducible
-    kappa = np.array([abs(oracle.tape_loglik(o_ops[:o_ln[i], i], o_cs[:o_ln[i], i], x, y, sd, 0)
-                          - oracle.tape_loglik(o_ops[:o_ln[i], i], o_cs[:o_ln[i], i], x, y, sd, 1)) for i in c])
+    kap = np.array([kappa(oracle, o_ops[:o_ln[i], i], o_cs[:o_ln[i], i], x, y, sd) for i in c])
     ok, diff, tol = ll_close(g_ll[c], l1[c], n)
-    assert np.all(ok | (diff <= tol + 4 * np.nan_to_num(kappa))), (diff[~ok], tol[~ok], kappa[~ok])
+    assert np.all(ok | (np.isfinite(g_ll[c]) & (diff <= tol + 4 * kap))), (diff[~ok], tol[~ok], kap[~ok])
     tot = cnt + cnt1
     ndiv = nch - same.sum()
     assert abs(st.struct_proposed - tot[0]) <= 2 * steps * ndiv // 5 and abs(st.const_proposed - tot[2]) <= 2 * steps * ndiv

@@ -1,0 +1,92 @@
+"""Reference-semantics kernels (heap-31 programs, the reference's proposals and median
+resampling) against the oracle's restatement of MH_simple_kernel (main.cu:679-813).
+
+Every draw is bit-identical (Philox substreams, or the reference LCG seeded
+block_seed + chain), so chains follow the oracle's trajectories: identical heaps and
+constants, identical priors; likelihoods within tolerance (device vs host libm in the
+programs' transcendentals).  A chain may leave the shared path only through an
+accept decision within rounding of its threshold.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import kappa, ll_close, say
+
+pytestmark = pytest.mark.gpu
+
+
+def data(name):
+    import os
+    if name == "S":
+        from gpuropolis_amd import workloads as W
+        return W.dataset_S(0)
+    a = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "nist", name, "data.txt"))
+    return a[:, 0].astype(np.float32), a[:, 1].astype(np.float32), a[:, 2].astype(np.float32)
+
+
+def test_ref_init_matches_oracle(oracle, gpr):
+    x, y, sd = data("S")
+    hop, hc = oracle.ref_init(21, 3000)
+    with gpr.Context(semantics=gpr.SEM_REFERENCE, seed=21) as ctx:
+        ctx.set_data([(x, y, sd)], [3000])
+        say("ref init parity")
+        ctx.init_chains()
+        g_hop, g_hc, _, _, _ = ctx.read_state()
+    assert np.array_equal(g_hop, hop)
+    assert np.array_equal(g_hc.view(np.uint32), hc.view(np.uint32))
+
+
+CASES = [("lane", 0, "S", 200), ("lane", 1, "S", 200), ("wave", 0, "S", 200), ("wave", 0, "BoxBOD", 300),
+         ("wave", 0, "Thurber", 150), ("lane", 1, "Misra1a", 100)]
+
+
+@pytest.mark.parametrize("sched,rng,dname,steps", CASES)
+def test_ref_trajectories_match_oracle(oracle, gpr, sched, rng, dname, steps):
+    x, y, sd = data(dname)
+    nch = 64
+    hop, hc = oracle.ref_init(5, nch)
+    o_hop, o_hc = hop.copy(), hc.copy()
+    cfg = oracle.GpoRefCfg(rng, 99)
+    p0, l0 = oracle.ref_run_block(cfg, o_hop, o_hc, x, y, sd, 0, steps, random_seed=4321)
+    p1, l1 = oracle.ref_run_block(cfg, o_hop, o_hc, x, y, sd, steps, steps, random_seed=8765)
+    sch = gpr.SCHED_LANE if sched == "lane" else gpr.SCHED_WAVE
+    with gpr.Context(semantics=gpr.SEM_REFERENCE, seed=99, rng=rng, schedule=sch) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(hop, hc)
+        say(f"ref trajectories sched={sched} rng={rng} data={dname} steps={steps}")
+        ctx.run_block(steps, 4321)
+        ctx.run_block(steps, 8765)
+        g_hop, g_hc, _, g_pr, g_ll = ctx.read_state()
+    same = np.all(g_hop == o_hop, axis=0) & np.all(g_hc.view(np.uint32) == o_hc.view(np.uint32), axis=0)
+    say("chains on the oracle path:", int(same.sum()), "of", nch)
+    assert same.sum() >= int(0.95 * nch)
+    c = np.flatnonzero(same)
+    assert np.array_equal(g_pr[c], p1[c])
+    ok, diff, tol = ll_close(g_ll[c], l1[c], len(x))
+    # programs whose LL is sensitive to one-ulp differences of the device libm get that
+    # sensitivity (kappa) added to the tolerance
+    kap = np.array([kappa(oracle, *oracle.ref_decode(o_hop[:, i].copy(), o_hc[:, i].copy())[:2], x, y, sd)
+                    for i in c])
+    good = ok | (np.isfinite(g_ll[c]) & (diff <= tol + 4 * kap))
+    assert good.all(), (diff[~good], tol[~good], kap[~good])
+
+
+def test_ref_quantile_resample_matches_oracle(oracle, gpr):
+    x, y, sd = data("S")
+    nch = 2001
+    hop, hc = oracle.ref_init(8, nch)
+    with gpr.Context(semantics=gpr.SEM_REFERENCE, seed=8, resample=gpr.RS_QUANTILE) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(hop, hc)
+        ctx.run_block(20, 1)
+        g_hop, g_hc, _, g_pr, g_ll = ctx.read_state()
+        say("ref quantile resample")
+        ctx.resample()
+        r_hop, r_hc, _, _, _ = ctx.read_state()
+    o_hop, o_hc = g_hop.copy(), g_hc.copy()
+    cutoff = np.zeros(1)
+    nre = oracle.lib().gpo_ref_resample(8, 1, nch, 0, oracle.u8(o_hop), oracle.f32(o_hc), oracle.f64(g_pr),
+                                        oracle.f64(g_ll), oracle.f64(cutoff))
+    assert nre > 0
+    assert np.array_equal(r_hop, o_hop)
+    assert np.array_equal(r_hc.view(np.uint32), o_hc.view(np.uint32))
