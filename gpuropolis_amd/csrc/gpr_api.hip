@@ -363,7 +363,13 @@ static int launch_block(gpr_ctx *c, int steps, int32_t block_seed) {
         return fail(GPR_E_ARG, "the LCG stream is sequential per chain: it needs the lane schedule");
     if (sched == GPR_SCHED_LANE) {
         if (ref) HIP_TRY(gpr_launch_mh_ref_lane(a, c->stream));
-        else return fail(GPR_E_UNSUPPORTED, "lane schedule MH kernel not built yet");
+        else {
+            if (c->cfg.proposal == GPR_PROP_MALA)
+                return fail(GPR_E_UNSUPPORTED, "MALA runs on the wave schedule only");
+            int capmax = 1;
+            for (const GprDataset &g : c->hds) capmax = std::max(capmax, g.cap);
+            HIP_TRY(gpr_launch_mh_config_lane(a, capmax, c->stream));
+        }
     } else {
         HIP_TRY(gpr_launch_fill_i32(c->d_tile_counter, (int)c->buckets.size(), 0, c->stream));
         for (size_t b = 0; b < c->buckets.size(); b++) {

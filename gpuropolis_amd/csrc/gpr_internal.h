@@ -59,6 +59,7 @@ struct GprMHArgs {
     unsigned long long *counters;
     int32_t block_seed;
     int rng;
+    int dmax_cap;           // largest per-dataset cap (lane kernel LDS sizing)
 };
 
 struct GprEvalArgs {

@@ -190,6 +190,221 @@ hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, int gr
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- config MH, lane schedule
+// One lane per chain: the per-chain sequential logic of the wave kernel (and the
+// oracle) with every lane running its own program.  The current program lives in the
+// global SoA state (coalesced [token][chain] accesses), the proposal tape and the
+// value stack in LDS as [slot][lane]; dispatch is per lane (divergent).
+#define GPR_CL_WG 64
+struct LaneTape {
+    uint8_t *o;     // LDS ops of this lane: o[t * GPR_CL_WG]
+    float *c;       // LDS constants
+    __device__ __forceinline__ uint8_t &op(int t) { return o[t * GPR_CL_WG]; }
+    __device__ __forceinline__ float &cv(int t) { return c[t * GPR_CL_WG]; }
+};
+
+__device__ __forceinline__ float lane_op(int o, float a, float b, float c) {
+    switch (o) {
+    case GPR_OP_ADD: return a + b;
+    case GPR_OP_SUB: return a - b;
+    case GPR_OP_MUL: return a * b;
+    case GPR_OP_DIV: return a / b;
+    case GPR_OP_POW: return powf(a, b);
+    case GPR_OP_RSUB: return b - a;
+    case GPR_OP_RDIV: return b / a;
+    case GPR_OP_RPOW: return powf(b, a);
+    default: return op_un(o, a, c);
+    }
+}
+
+// Sampler log-likelihood of one lane's tape (reference lnormalpdf numerics, as the
+// wave kernel: FP32 residual and square, compensated sum, FP64 weight).
+__device__ double lane_loglik(LaneTape &tp, int len, float *stk, const GprDataset &d, const float *dx,
+                              const float *dy, const float *dw) {
+    float acc = 0.0f, comp = 0.0f;
+    for (int i = 0; i < d.n; i++) {
+        const float x = dx[i];
+        float T = 0.0f;
+        int sp = 0;
+        for (int t = 0; t < len; t++) {
+            const int o = tp.op(t);
+            if (o <= GPR_OP_ONE) {
+                stk[min(sp, GPR_MAX_DEPTH - 1) * GPR_CL_WG] = T;
+                sp = min(sp + 1, GPR_MAX_DEPTH);
+                T = o == GPR_OP_X ? x : (o == GPR_OP_CONST ? tp.cv(t) : 1.0f);
+            } else if (o <= GPR_OP_RPOW) {
+                sp = max(sp - 1, 0);
+                T = lane_op(o, stk[min(sp, GPR_MAX_DEPTH - 1) * GPR_CL_WG], T, 0.0f);
+            } else {
+                T = lane_op(o, T, 0.0f, tp.cv(t));
+            }
+        }
+        const float r = T - dy[i];
+        const float term = (r * r) * dw[i];
+        const float yv = term - comp;
+        const float tv = acc + yv;
+        comp = (tv - acc) - yv;
+        acc = tv;
+    }
+    const double s = (double)acc - (double)comp;
+    if (!__builtin_isfinite(s)) return -GPR_INF;
+    return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
+}
+
+__device__ double lane_prior(LaneTape &tp, int len, double lnsym) {
+    int dl = 0;
+    double p = 0.0;
+    for (int t = 0; t < len; t++) {
+        const int o = tp.op(t);
+        dl += gpr_dl_cost(o);
+        if (gpr_has_const(o)) p += gpr_lcauchy(tp.cv(t));
+    }
+    return gpr_dl_prior(dl, lnsym, p);
+}
+
+__global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
+    extern __shared__ unsigned char smem[];
+    const int tid = threadIdx.x;
+    const int chain = blockIdx.x * GPR_CL_WG + tid;
+    // per-dataset cap bounds the LDS tape; the launch sizes it for the largest cap
+    const int capmax = a.dmax_cap;
+    float *s_stack = (float *)smem;                                  // [16][WG]
+    float *s_c = s_stack + GPR_MAX_DEPTH * GPR_CL_WG;                // [cap][WG]
+    uint8_t *s_o = (uint8_t *)(s_c + capmax * GPR_CL_WG);            // [cap][WG]
+    if (chain >= a.nchains) return;
+    int di = 0;
+    while (a.ds[di].chain0 + a.ds[di].nchains <= chain) di++;
+    const GprDataset d = a.ds[di];
+    const float *dx = a.dx + d.off, *dy = a.dy + d.off, *dw = a.dw + d.off;
+    const int N = a.nchains, cap = d.cap;
+    LaneTape tp{s_o + tid, s_c + tid};
+    float *stk = s_stack + tid;
+    uint8_t *gops = a.ops + chain;
+    float *gcs = a.cs + chain;
+    int len = a.len[chain];
+    // proposal = current; evaluate the current state
+    for (int t = 0; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
+    double prior = lane_prior(tp, len, a.lnsym);
+    double ll = lane_loglik(tp, len, stk, d, dx, dy, dw);
+    unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0, n_evals = 0, n_flops = 0;
+    for (int i = 0; i < a.steps; i++) {
+        const uint32_t step = a.step0 + (uint32_t)i;
+        const double post = prior + ll;
+        double extra = 0.0;
+        int plen = len;
+        const bool structural = !a.fixed_structure && (step % (uint32_t)a.struct_every) == 0;
+        if (structural) {
+            n_sprop++;
+            gpr_rstream st;
+            st.init(a.seed, (uint32_t)chain, step, GPR_P_STRUCT, 0);
+            const uint32_t r = st.next();
+            const int j = __builtin_isfinite(post) ? (int)(((uint64_t)r * (uint64_t)len) >> 32) : len - 1;
+            int need = 1, s0 = j;
+            while (true) {
+                need += gpr_arity(gops[(size_t)s0 * N]) - 1;
+                if (need == 0) break;
+                s0--;
+            }
+            const int oldsize = j - s0 + 1;
+            const int maxnew = cap - (len - oldsize);
+            // generate in right-child-first prefix order into [cap-m, cap): that run is the postfix
+            int m = 0;
+            need = 1;
+            double lq_new = 0.0;
+            bool fail = false;
+            while (need > 0) {
+                if (m >= maxnew) { fail = true; break; }
+                const int sy = gpr_gen_symbol(st.next());
+                lq_new += gpr_gen_logp(sy);
+                float cvv = 0.0f;
+                if (sy == GPR_OP_CONST) { cvv = gpr_normal(gpr_u01(st.next())); lq_new += gpr_lphi(cvv); }
+                tp.op(cap - 1 - m) = (uint8_t)sy;
+                tp.cv(cap - 1 - m) = cvv;
+                m++;
+                need += gpr_arity(sy) - 1;
+            }
+            if (fail) {
+                for (int t = 0; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
+                continue;
+            }
+            plen = len - oldsize + m;
+            for (int t = 0; t < m; t++) { tp.op(s0 + t) = tp.op(cap - m + t); tp.cv(s0 + t) = tp.cv(cap - m + t); }
+            for (int t = j + 1; t < len; t++) {
+                tp.op(t - oldsize + m) = gops[(size_t)t * N];
+                tp.cv(t - oldsize + m) = gcs[(size_t)t * N];
+            }
+            for (int t = 0; t < s0; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
+            int depth = 0, dmx = 0;
+            for (int t = 0; t < plen; t++) { depth += 1 - gpr_arity(tp.op(t)); dmx = max(dmx, depth); }
+            double lq_old = 0.0;
+            for (int t = s0; t <= j; t++) {
+                const int o = gops[(size_t)t * N];
+                lq_old += gpr_gen_logp(o);
+                if (o == GPR_OP_CONST) lq_old += gpr_lphi(gcs[(size_t)t * N]);
+            }
+            if (dmx > a.dmax) {
+                for (int t = 0; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
+                continue;
+            }
+            extra = (-(double)gpr_logf((float)plen) + lq_old) - (-(double)gpr_logf((float)len) + lq_new);
+        } else {
+            int nconst = 0;
+            for (int t = 0; t < len; t++) nconst += tp.op(t) == GPR_OP_CONST;
+            if (nconst == 0) continue;
+            n_cprop++;
+            for (int t = 0; t < len; t++) {
+                if (tp.op(t) != GPR_OP_CONST) continue;
+                const gpr_u4 w = gpr_step_block(a.seed, (uint32_t)chain, step, GPR_P_CPROP, (uint32_t)(t & 63),
+                                                (uint32_t)(t >> 6));
+                tp.cv(t) = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(w, (int)(step & 3)))), gcs[(size_t)t * N]);
+            }
+        }
+        const double pp = lane_prior(tp, plen, a.lnsym);
+        const double pl = lane_loglik(tp, plen, stk, d, dx, dy, dw);
+        n_evals += (unsigned long long)plen * (unsigned long long)d.n;
+        {
+            int nleaf = 0, nconst = 0;
+            for (int t = 0; t < plen; t++) { nleaf += tp.op(t) <= GPR_OP_ONE; nconst += gpr_has_const(tp.op(t)); }
+            n_flops += (unsigned long long)(plen - nleaf + 4) * (unsigned long long)d.n + (unsigned long long)nconst;
+        }
+        if (cfg_accept(a.seed, (uint32_t)chain, step, post, pp + pl, (pp + pl) - post + extra)) {
+            for (int t = 0; t < plen; t++) { gops[(size_t)t * N] = tp.op(t); gcs[(size_t)t * N] = tp.cv(t); }
+            for (int t = plen; t < len; t++) { gops[(size_t)t * N] = 0; gcs[(size_t)t * N] = 0.0f; }
+            len = plen;
+            prior = pp;
+            ll = pl;
+            if (structural) n_sacc++; else n_cacc++;
+        } else {
+            for (int t = 0; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
+        }
+    }
+    a.len[chain] = len;
+    a.prior[chain] = prior;
+    a.ll[chain] = ll;
+    if (a.counters) {
+        atomicAdd(&a.counters[GPR_CNT_SPROP], n_sprop);
+        atomicAdd(&a.counters[GPR_CNT_SACC], n_sacc);
+        atomicAdd(&a.counters[GPR_CNT_CPROP], n_cprop);
+        atomicAdd(&a.counters[GPR_CNT_CACC], n_cacc);
+        atomicAdd(&a.counters[GPR_CNT_EVALS], n_evals);
+        atomicAdd(&a.counters[GPR_CNT_STEPS], (unsigned long long)a.steps);
+        atomicAdd(&a.counters[GPR_CNT_FLOPS], n_flops);
+    }
+}
+
+hipError_t gpr_launch_mh_config_lane(const GprMHArgs &a, int cap_max, hipStream_t s) {
+    const int grid = (a.nchains + GPR_CL_WG - 1) / GPR_CL_WG;
+    const size_t sh = sizeof(float) * (GPR_MAX_DEPTH + (size_t)cap_max) * GPR_CL_WG + (size_t)cap_max * GPR_CL_WG;
+    if (sh > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_lane, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        if (e != hipSuccess) return e;
+    }
+    GprMHArgs b = a;
+    b.dmax_cap = cap_max;
+    hipLaunchKernelGGL(k_mh_config_lane, dim3(grid), dim3(GPR_CL_WG), sh, s, b);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- standalone evaluator
 template <int KC, bool HI>
 __global__ void __launch_bounds__(GPR_WG) k_eval_wave(GprEvalArgs a) {

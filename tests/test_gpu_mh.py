@@ -34,8 +34,9 @@ def test_init_matches_oracle(oracle, gpr):
         assert np.array_equal(cs.view(np.uint32), o_cs.view(np.uint32))
 
 
-@pytest.mark.parametrize("cap,n,steps", [(32, 32, 300), (64, 250, 150), (128, 70, 150)])
-def test_mh_trajectories_match_oracle(oracle, gpr, cap, n, steps):
+@pytest.mark.parametrize("cap,n,steps,sched", [(32, 32, 300, "wave"), (64, 250, 150, "wave"), (128, 70, 150, "wave"),
+                                               (32, 32, 300, "lane"), (128, 70, 100, "lane")])
+def test_mh_trajectories_match_oracle(oracle, gpr, cap, n, steps, sched):
     rng = np.random.default_rng(cap + n)
     if n == 32:
         x, y, sd = s_data()
@@ -49,10 +50,11 @@ def test_mh_trajectories_match_oracle(oracle, gpr, cap, n, steps):
     o_ops, o_cs, o_ln = ops.copy(), cs.copy(), ln.copy()
     p0, l0, cnt = oracle.cfg_run_block(cfg, o_ops, o_cs, o_ln, x, y, sd, 0, steps)
     p1, l1, cnt1 = oracle.cfg_run_block(cfg, o_ops, o_cs, o_ln, x, y, sd, steps, steps)
-    with gpr.Context(cap=cap, seed=123) as ctx:
+    schedule = gpr.SCHED_WAVE if sched == "wave" else gpr.SCHED_LANE
+    with gpr.Context(cap=cap, seed=123, schedule=schedule) as ctx:
         ctx.set_data([(x, y, sd)], [nch])
         ctx.write_state(ops, cs, ln)
-        say(f"mh trajectories cap={cap} n={n} steps={steps}")
+        say(f"mh trajectories cap={cap} n={n} steps={steps} schedule={sched}")
         ctx.run_block(steps)
         ctx.run_block(steps)
         g_ops, g_cs, g_ln, g_pr, g_ll = ctx.read_state()
