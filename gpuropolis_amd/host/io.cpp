@@ -1,0 +1,309 @@
+// Data loader and program display of the gpuropolis CLI (see io.hpp).
+#include "io.hpp"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "gpuropolis.h"
+
+namespace gpr_host {
+
+bool load_data_file(const char *path, float sdscale, bool first_half, bool even_half, std::vector<Datum> &out,
+                    std::string &err) {
+    FILE *fp = fopen(path, "r");
+    if (!fp) {
+        err = std::string("*** ERROR: Cannot open file:\t") + path;
+        return false;
+    }
+    out.clear();
+    char *line = nullptr;
+    size_t cap = 0;
+    double x, y, sd;
+    bool ok = true;
+    while (getline(&line, &cap, fp) != -1) {
+        if (line[0] == '#') continue;
+        if (sscanf(line, "%lf\t%lf\t%lf\n", &x, &y, &sd) == 3 || sscanf(line, "%le\t%le\t%le\n", &x, &y, &sd) == 3) {
+            sd = sdscale * sd;
+            if (!((float)sd > 0.0f)) {
+                err = std::string("*** ERROR: sd must be positive:\t") + line;
+                ok = false;
+                break;
+            }
+            out.push_back(Datum{(float)x, (float)y, (float)sd});
+        } else if (strspn(line, " \r\n\t") == strlen(line)) {
+            continue;
+        } else {
+            err = std::string("*** ERROR IN PARSING INPUT\t") + line;
+            ok = false;
+            break;
+        }
+    }
+    free(line);
+    fclose(fp);
+    if (!ok) return false;
+    if (first_half) out.resize(out.size() / 2);
+    if (even_half) {
+        std::vector<Datum> kept;
+        for (size_t i = 1; i < out.size(); i += 2) kept.push_back(out[i]);
+        out.swap(kept);
+    }
+    return true;
+}
+
+static std::string fmt(const char *f, float v) {
+    char b[64];
+    snprintf(b, sizeof b, f, v);
+    return b;
+}
+
+// ------------------------------------------------------------------ heap programs
+// string_dispatch (main.cu:821-873)
+static std::string heap_str_op(int o, const std::string &a, const std::string &b, const std::string &C) {
+    switch (o) {
+    case GPR_REF_ONE: return "1";
+    case GPR_REF_CONST: return C;
+    case GPR_REF_INV: return "(1/" + a + ")";
+    case GPR_REF_A: return a;
+    case GPR_REF_B: return b;
+    case GPR_REF_PLUS: return "(" + a + "+" + b + ")";
+    case GPR_REF_CPLUS: return "(" + a + "+" + C + ")";
+    case GPR_REF_MINUS: return "(" + a + "-" + b + ")";
+    case GPR_REF_RMINUS: return "(" + b + "-" + a + ")";
+    case GPR_REF_TIMES: return "(" + b + "*" + a + ")";
+    case GPR_REF_CTIMES: return "(" + a + "*" + C + ")";
+    case GPR_REF_DIV: return "(" + a + "/" + b + ")";
+    case GPR_REF_RDIV: return "(" + b + "/" + a + ")";
+    case GPR_REF_LOG: return "log(" + a + ")";
+    case GPR_REF_SIN: return "sin(" + a + ")";
+    case GPR_REF_ASIN: return "asin(" + a + ")";
+    case GPR_REF_COS: return "cos(" + a + ")";
+    case GPR_REF_ACOS: return "acos(" + a + ")";
+    case GPR_REF_TAN: return "tan(" + a + ")";
+    case GPR_REF_ATAN: return "atan(" + a + ")";
+    case GPR_REF_SQRT: return "sqrt(" + a + ")";
+    case GPR_REF_EXP: return "exp(" + a + ")";
+    case GPR_REF_POW: return "(" + a + "^" + b + ")";
+    case GPR_REF_CPOW: return "(" + a + "^" + C + ")";
+    case GPR_REF_RPOW: return "(" + b + "^" + a + ")";
+    case GPR_REF_CRPOW: return "(" + C + "^" + a + ")";
+    case GPR_REF_ABS: return "abs(" + a + ")";
+    default: return "NAN";
+    }
+}
+// latex_dispatch (main.cu:901-953)
+static std::string heap_tex_op(int o, const std::string &a, const std::string &b, const std::string &C) {
+    switch (o) {
+    case GPR_REF_ONE: return "1";
+    case GPR_REF_CONST: return C;
+    case GPR_REF_INV: return "1/" + a;
+    case GPR_REF_A: return a;
+    case GPR_REF_B: return b;
+    case GPR_REF_PLUS: return a + "+" + b;
+    case GPR_REF_CPLUS: return a + "+" + C;
+    case GPR_REF_MINUS: return a + "-" + b;
+    case GPR_REF_RMINUS: return b + "-" + a;
+    case GPR_REF_TIMES: return b + " " + a;
+    case GPR_REF_CTIMES: return a + " \\cdot " + C;
+    case GPR_REF_DIV: return "\\frac{" + a + "}{" + b + "}";
+    case GPR_REF_RDIV: return "\\frac{" + b + "}{" + a + "}";
+    case GPR_REF_LOG: return "\\log(" + a + ")";
+    case GPR_REF_SIN: return "sin(" + a + ")";
+    case GPR_REF_ASIN: return "asin(" + a + ")";
+    case GPR_REF_COS: return "cos(" + a + ")";
+    case GPR_REF_ACOS: return "acos(" + a + ")";
+    case GPR_REF_TAN: return "tan(" + a + ")";
+    case GPR_REF_ATAN: return "atan(" + a + ")";
+    case GPR_REF_SQRT: return "\\sqrt{" + a + "}";
+    case GPR_REF_EXP: return "e^{" + a + "}";
+    case GPR_REF_POW: return a + "^{" + b + "}";
+    case GPR_REF_CPOW: return a + "^{" + C + "}";
+    case GPR_REF_RPOW: return b + "^{" + a + "}";
+    case GPR_REF_CRPOW: return C + "^{" + a + "}";
+    case GPR_REF_ABS: return "\\mid " + a + "\\mid ";
+    default: return "NAN";
+    }
+}
+
+// bottom-up string fold over the 31 slots, x below slots 16..31 (main.cu:878-898)
+template <class F>
+static std::string heap_fold_str(const uint8_t *P, const float *C, const char *cf, F op) {
+    std::vector<std::string> buf(2 * GPR_REF_SLOTS + 2);
+    for (int i = GPR_REF_SLOTS; i >= 1; i--) {
+        const int l = 2 * i, r = 2 * i + 1;
+        const std::string a = l > GPR_REF_SLOTS ? "x" : buf[l];
+        const std::string b = r > GPR_REF_SLOTS ? "x" : buf[r];
+        buf[i] = op(P[i - 1], a, b, fmt(cf, C[i - 1]));
+    }
+    return buf[1];
+}
+std::string heap_string(const uint8_t *P, const float *C, const char *cf) { return heap_fold_str(P, C, cf, heap_str_op); }
+std::string heap_latex(const uint8_t *P, const float *C, const char *cf) { return heap_fold_str(P, C, cf, heap_tex_op); }
+
+float heap_length(const uint8_t *P) {  // dispatch_length fold, main.cu:477-526
+    float buf[GPR_REF_SLOTS + 2];
+    for (int i = GPR_REF_SLOTS; i >= 1; i--) {
+        const int l = 2 * i;
+        const float a = l > GPR_REF_SLOTS ? 1.0f : buf[l], b = l > GPR_REF_SLOTS ? 1.0f : buf[l + 1];
+        float v;
+        switch (P[i - 1]) {
+        case GPR_REF_ONE: case GPR_REF_CONST: v = 1.0f; break;
+        case GPR_REF_A: v = a; break;
+        case GPR_REF_B: v = b; break;
+        case GPR_REF_CPLUS: case GPR_REF_CTIMES: case GPR_REF_CPOW: case GPR_REF_CRPOW: v = 1.0f + a + 1.0f; break;
+        case GPR_REF_PLUS: case GPR_REF_MINUS: case GPR_REF_TIMES: case GPR_REF_DIV: case GPR_REF_RDIV:
+        case GPR_REF_POW: v = 1.0f + a + b; break;
+        case GPR_REF_RMINUS: case GPR_REF_RPOW: v = 1.0f + b + a; break;
+        case GPR_REF_INV: case GPR_REF_SQRT: case GPR_REF_LOG: case GPR_REF_SIN: case GPR_REF_ASIN: case GPR_REF_COS:
+        case GPR_REF_ACOS: case GPR_REF_TAN: case GPR_REF_ATAN: case GPR_REF_EXP: case GPR_REF_ABS: v = 1.0f + a; break;
+        default: v = NAN; break;
+        }
+        buf[i] = v;
+    }
+    return buf[1];
+}
+
+static float fmaxr(float a, float b) { return a > b ? a : b; }  // max macro, main.cu:184-187
+float heap_degree(const uint8_t *P, const float *C) {  // dispatch_degree fold, main.cu:615-654
+    float buf[GPR_REF_SLOTS + 2];
+    for (int i = GPR_REF_SLOTS; i >= 1; i--) {
+        const int l = 2 * i;
+        const float a = l > GPR_REF_SLOTS ? 1.0f : buf[l], b = l > GPR_REF_SLOTS ? 1.0f : buf[l + 1];
+        const float c = C[i - 1];
+        float v;
+        switch (P[i - 1]) {
+        case GPR_REF_ONE: case GPR_REF_CONST: v = 0.0f; break;
+        case GPR_REF_A: v = a; break;
+        case GPR_REF_B: v = b; break;
+        case GPR_REF_PLUS: case GPR_REF_MINUS: case GPR_REF_RMINUS: v = fmaxr(a, b); break;
+        case GPR_REF_CPLUS: case GPR_REF_CTIMES: v = a; break;
+        case GPR_REF_TIMES: v = a + b; break;
+        case GPR_REF_CPOW: v = a * c; break;
+        case GPR_REF_POW: v = b == 0.0f ? a * c : INFINITY; break;
+        case GPR_REF_DIV: v = b == 0.0f ? a : INFINITY; break;
+        default: v = INFINITY; break;
+        }
+        buf[i] = v;
+    }
+    return buf[1];
+}
+
+std::string heap_code(const uint8_t *P) {
+    static const char *code = GPR_REF_PROGRAM_CODE;
+    std::string s;
+    for (int i = 0; i < GPR_REF_SLOTS; i++) s += P[i] < GPR_REF_NOPS ? code[P[i]] : '?';
+    return s;
+}
+
+// ------------------------------------------------------------------ postfix programs
+static std::string tape_str_op(int o, const std::string &a, const std::string &b, const std::string &C, bool tex) {
+    switch (o) {
+    case GPR_OP_ADD: return tex ? a + "+" + b : "(" + a + "+" + b + ")";
+    case GPR_OP_SUB: return tex ? a + "-" + b : "(" + a + "-" + b + ")";
+    case GPR_OP_MUL: return tex ? a + " " + b : "(" + a + "*" + b + ")";
+    case GPR_OP_DIV: return tex ? "\\frac{" + a + "}{" + b + "}" : "(" + a + "/" + b + ")";
+    case GPR_OP_POW: return tex ? a + "^{" + b + "}" : "(" + a + "^" + b + ")";
+    case GPR_OP_RSUB: return tex ? b + "-" + a : "(" + b + "-" + a + ")";
+    case GPR_OP_RDIV: return tex ? "\\frac{" + b + "}{" + a + "}" : "(" + b + "/" + a + ")";
+    case GPR_OP_RPOW: return tex ? b + "^{" + a + "}" : "(" + b + "^" + a + ")";
+    case GPR_OP_NEG: return tex ? "-" + a : "(-" + a + ")";
+    case GPR_OP_EXP: return tex ? "e^{" + a + "}" : "exp(" + a + ")";
+    case GPR_OP_LOG: return tex ? "\\log(" + a + ")" : "log(" + a + ")";
+    case GPR_OP_INV: return tex ? "1/" + a : "(1/" + a + ")";
+    case GPR_OP_SQRT: return tex ? "\\sqrt{" + a + "}" : "sqrt(" + a + ")";
+    case GPR_OP_SIN: return "sin(" + a + ")";
+    case GPR_OP_ASIN: return "asin(" + a + ")";
+    case GPR_OP_COS: return "cos(" + a + ")";
+    case GPR_OP_ACOS: return "acos(" + a + ")";
+    case GPR_OP_TAN: return "tan(" + a + ")";
+    case GPR_OP_ATAN: return "atan(" + a + ")";
+    case GPR_OP_ABS: return tex ? "\\mid " + a + "\\mid " : "abs(" + a + ")";
+    case GPR_OP_CADD: return tex ? a + "+" + C : "(" + a + "+" + C + ")";
+    case GPR_OP_CMUL: return tex ? a + " \\cdot " + C : "(" + a + "*" + C + ")";
+    case GPR_OP_CPOW: return tex ? a + "^{" + C + "}" : "(" + a + "^" + C + ")";
+    case GPR_OP_CRPOW: return tex ? C + "^{" + a + "}" : "(" + C + "^" + a + ")";
+    default: return "NAN";
+    }
+}
+
+static std::string tape_fold(const uint8_t *ops, const float *cs, int len, const char *cf, bool tex) {
+    std::vector<std::string> st;
+    for (int t = 0; t < len; t++) {
+        const int o = ops[t];
+        if (o == GPR_OP_X) st.push_back("x");
+        else if (o == GPR_OP_CONST) st.push_back(fmt(cf, cs[t]));
+        else if (o == GPR_OP_ONE) st.push_back("1");
+        else if (o >= GPR_OP_ADD && o <= GPR_OP_RPOW) {
+            if (st.size() < 2) return "NAN";
+            std::string b = st.back(); st.pop_back();
+            std::string a = st.back(); st.pop_back();
+            st.push_back(tape_str_op(o, a, b, "", tex));
+        } else {
+            if (st.empty()) return "NAN";
+            std::string a = st.back(); st.pop_back();
+            st.push_back(tape_str_op(o, a, "", fmt(cf, cs[t]), tex));
+        }
+    }
+    return st.size() == 1 ? st[0] : "NAN";
+}
+std::string tape_string(const uint8_t *ops, const float *cs, int len, const char *cf) { return tape_fold(ops, cs, len, cf, false); }
+std::string tape_latex(const uint8_t *ops, const float *cs, int len, const char *cf) { return tape_fold(ops, cs, len, cf, true); }
+
+float tape_length(const uint8_t *ops, int len) {
+    float l = 0.0f;
+    for (int t = 0; t < len; t++) l += ops[t] >= GPR_OP_CADD ? 2.0f : 1.0f;
+    return l;
+}
+
+// polynomial degree of a postfix program: x 1, constants 0, + - max, * sum, / and ^ only by
+// constant divisors / exponents, constant-operand forms as in dispatch_degree
+float tape_degree(const uint8_t *ops, const float *cs, int len) {
+    std::vector<float> st;
+    std::vector<int> isc;
+    std::vector<float> cval;
+    for (int t = 0; t < len; t++) {
+        const int o = ops[t];
+        if (o == GPR_OP_X) { st.push_back(1.0f); isc.push_back(0); cval.push_back(0.0f); continue; }
+        if (o == GPR_OP_CONST || o == GPR_OP_ONE) {
+            st.push_back(0.0f); isc.push_back(1); cval.push_back(o == GPR_OP_ONE ? 1.0f : cs[t]);
+            continue;
+        }
+        if (o >= GPR_OP_ADD && o <= GPR_OP_RPOW) {
+            if (st.size() < 2) return INFINITY;
+            const float b = st.back(), a = st[st.size() - 2];
+            const int cb = isc.back();
+            const float vb = cval.back();
+            st.pop_back(); isc.pop_back(); cval.pop_back();
+            float v;
+            switch (o) {
+            case GPR_OP_ADD: case GPR_OP_SUB: case GPR_OP_RSUB: v = fmaxr(a, b); break;
+            case GPR_OP_MUL: v = a + b; break;
+            case GPR_OP_DIV: v = b == 0.0f ? a : INFINITY; break;
+            case GPR_OP_POW: v = (cb && vb >= 0.0f && vb == floorf(vb)) ? a * vb : INFINITY; break;
+            default: v = INFINITY; break;
+            }
+            st.back() = v; isc.back() = 0; cval.back() = 0.0f;
+            continue;
+        }
+        if (st.empty()) return INFINITY;
+        float &a = st.back();
+        switch (o) {
+        case GPR_OP_NEG: case GPR_OP_CADD: case GPR_OP_CMUL: break;
+        case GPR_OP_CPOW: a = (cs[t] >= 0.0f && cs[t] == floorf(cs[t])) ? a * cs[t] : INFINITY; break;
+        default: a = INFINITY; break;
+        }
+        isc.back() = 0;
+    }
+    return st.size() == 1 ? st[0] : INFINITY;
+}
+
+std::string tape_code(const uint8_t *ops, int len) {
+    // the reference's symbols where an op exists there (PROGRAM_CODE, main.cu:55), x and ~ (neg)
+    static const char code[GPR_NUM_OPS + 1] = "xC1+-*/^_|V~ELIRSsCcTtA#@pP";
+    std::string s;
+    for (int t = 0; t < len; t++) s += ops[t] < GPR_NUM_OPS ? code[ops[t]] : '?';
+    return s;
+}
+
+}  // namespace gpr_host
