@@ -45,7 +45,6 @@ struct gpr_ctx {
     float *d_cs = nullptr;
     int32_t *d_len = nullptr;
     double *d_prior = nullptr, *d_ll = nullptr;
-    float *d_grad = nullptr;
     std::vector<GprTile> htiles;
     std::vector<KernelBucket> buckets;
     GprTile *d_tiles = nullptr;
@@ -86,13 +85,13 @@ extern "C" int gpr_config_default(gpr_config *c) {
 }
 
 static void free_state(gpr_ctx *c) {
-    void *ps[] = {c->d_ds, c->d_x, c->d_y, c->d_w, c->d_sd, c->d_lnc, c->d_ops, c->d_cs, c->d_len, c->d_prior, c->d_ll, c->d_grad,
+    void *ps[] = {c->d_ds, c->d_x, c->d_y, c->d_w, c->d_sd, c->d_lnc, c->d_ops, c->d_cs, c->d_len, c->d_prior, c->d_ll,
                   c->d_tiles, c->d_tile_counter, c->d_counters, c->d_good, c->d_ngood, c->d_map_chain,
                   c->d_nres, c->d_map_post, c->d_cutoff};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     c->d_ds = nullptr; c->d_x = c->d_y = c->d_w = c->d_sd = nullptr; c->d_lnc = nullptr; c->d_ops = nullptr; c->d_cs = nullptr;
-    c->d_len = nullptr; c->d_prior = c->d_ll = nullptr; c->d_grad = nullptr; c->d_tiles = nullptr;
+    c->d_len = nullptr; c->d_prior = c->d_ll = nullptr; c->d_tiles = nullptr;
     c->d_tile_counter = nullptr; c->d_counters = nullptr; c->d_good = c->d_ngood = c->d_map_chain = nullptr;
     c->d_nres = nullptr; c->d_map_post = c->d_cutoff = nullptr;
     c->have_data = c->have_state = false;
@@ -235,7 +234,6 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     ALLOC(c->d_len, N);
     ALLOC(c->d_prior, N);
     ALLOC(c->d_ll, N);
-    if (!ref && c->cfg.proposal == GPR_PROP_MALA) ALLOC(c->d_grad, (size_t)c->rows * N);
     ALLOC(c->d_tiles, c->htiles.size());
     ALLOC(c->d_tile_counter, std::max<size_t>(1, c->buckets.size()));
     ALLOC(c->d_counters, (size_t)GPR_NCNT);
@@ -266,7 +264,7 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
 
 static GprMHArgs mh_args(gpr_ctx *c, int steps, int32_t block_seed) {
     GprMHArgs a{};
-    a.ops = c->d_ops; a.cs = c->d_cs; a.len = c->d_len; a.prior = c->d_prior; a.ll = c->d_ll; a.grad = c->d_grad;
+    a.ops = c->d_ops; a.cs = c->d_cs; a.len = c->d_len; a.prior = c->d_prior; a.ll = c->d_ll;
     a.nchains = c->N;
     a.dx = c->d_x; a.dy = c->d_y; a.dw = c->d_w;
     a.dsd = c->d_sd;
@@ -379,8 +377,9 @@ static int launch_block(gpr_ctx *c, int steps, int32_t block_seed) {
             ab.ntiles = kb.ntiles;
             ab.tile_counter = c->d_tile_counter + b;
             const int grid = std::min(kb.ntiles, c->cus * 8);
+            ab.dmax_cap = c->rows;
             if (ref) HIP_TRY(gpr_launch_mh_ref_wave(ab, kb.kc, grid, c->stream));
-            else HIP_TRY(gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, grid, c->stream));
+            else HIP_TRY(gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, c->cfg.proposal == GPR_PROP_MALA, grid, c->stream));
         }
     }
     c->sched_used = sched;
@@ -415,7 +414,7 @@ static GprResampleArgs rs_args(gpr_ctx *c) {
     GprResampleArgs r{};
     r.ops = c->d_ops; r.cs = c->d_cs;
     r.len = c->cfg.semantics == GPR_SEM_CONFIG ? c->d_len : nullptr;
-    r.prior = c->d_prior; r.ll = c->d_ll; r.grad = c->d_grad;
+    r.prior = c->d_prior; r.ll = c->d_ll;
     r.nchains = c->N;
     r.cap_rows = c->rows;
     r.ds = c->d_ds;
@@ -514,7 +513,21 @@ extern "C" int gpr_eval_loglik(gpr_ctx *c, int dataset, const uint8_t *d_ops, co
 
 extern "C" int gpr_grad_logpost(gpr_ctx *c, int dataset, const uint8_t *d_ops, const float *d_cs,
                                 const int32_t *d_len, int nchains, int cap, float *d_grad, double *d_lp) {
-    return fail(GPR_E_UNSUPPORTED, "adjoint kernel not built yet");
+    if (!c || !c->have_data) return fail(GPR_E_STATE, "gpr_set_data must come first");
+    if (dataset < 0 || dataset >= c->nd) return fail(GPR_E_ARG, "dataset index out of range");
+    if (!d_ops || !d_cs || !d_len || !d_grad || !d_lp || nchains < 1) return fail(GPR_E_ARG, "null or empty arrays");
+    if (cap < 1 || cap > GPR_MAX_CAP) return fail(GPR_E_ARG, "cap must be in [1, 128]");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    GprEvalArgs a{};
+    a.ops = d_ops; a.cs = d_cs; a.len = d_len; a.nchains = nchains; a.cap = cap;
+    a.dx = c->d_x; a.dy = c->d_y; a.dw = c->d_w;
+    a.d = c->hds[dataset];
+    a.ll = d_lp;
+    a.grad = d_grad;
+    a.lnsym = log((double)c->cfg.nsym);
+    HIP_TRY(gpr_launch_grad_wave(a, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
 }
 
 // heap -> postfix of the reachable tree (host), main.cu:371-393 semantics

@@ -341,6 +341,183 @@ __device__ __forceinline__ float gather_f(float v0, float v1, int src, bool hi) 
     return src >= 64 ? b : a;
 }
 
+// ---------------------------------------------------------------- reverse-mode adjoint
+// One point per lane (KC = 1).  The forward sweep is the interpreter above and also
+// stores every token's value in this wave's LDS slab lv[t * 64 + lane]; the reverse
+// sweep walks the tape backwards with an adjoint stack: reverse postfix order visits a
+// node, then its right subtree, then its left subtree, so a binary node pushes its
+// left child's adjoint and carries the right child's.  A binary token t has its right
+// operand at t-1 and its left operand at the last u < t whose stack depth equals t's.
+template <bool HI>
+__device__ __forceinline__ float wave_eval_store(const WaveProg &p, float x, float *lv) {
+    const int l = lane_id();
+    VStack<1> S;
+    int sp = 0;
+    float T = 0.0f;
+    for (int t = 0; t < p.len; t++) {
+        const int li = t & 63;
+        const bool h = HI && t >= 64;
+        const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
+        if (op <= GPR_OP_ONE) {
+            S.put(0, min(sp, GPR_MAX_DEPTH - 1), T);
+            sp = min(sp + 1, GPR_MAX_DEPTH);
+            T = op == GPR_OP_X ? x : (op == GPR_OP_CONST ? (h ? rl_f(p.c1, li) : rl_f(p.c0, li)) : 1.0f);
+        } else if (op <= GPR_OP_RPOW) {
+            sp = max(sp - 1, 0);
+            T = op_bin(op, S.get(0, sp), T);
+        } else {
+            const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, li) : rl_f(p.c0, li)) : 0.0f;
+            T = op_un(op, T, c);
+        }
+        lv[t * 64 + l] = T;
+    }
+    return T;
+}
+
+// add the wave sum of v (padding lanes masked) to token t of the lane-distributed g0/g1
+template <bool HI>
+__device__ __forceinline__ void grad_add(double &g0, double &g1, int t, float v, bool valid) {
+    const double s = (double)wave_sum_f(valid ? v : 0.0f);
+    if (HI && t >= 64) { if (lane_id() == t - 64) g1 += s; }
+    else if (lane_id() == t) g0 += s;
+}
+
+template <bool HI>
+__device__ __forceinline__ void wave_adjoint(const WaveProg &p, const float *lv, float G, bool valid, int D0, int D1,
+                                             double &g0, double &g1) {
+    const int l = lane_id();
+    VStack<1> A;
+    int asp = 0;
+    for (int t = p.len - 1; t >= 0; t--) {
+        const int li = t & 63;
+        const bool h = HI && t >= 64;
+        const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
+        if (op <= GPR_OP_ONE) {
+            if (op == GPR_OP_CONST) grad_add<HI>(g0, g1, t, G, valid);
+            if (asp > 0) { asp--; G = A.get(0, asp); }
+            continue;
+        }
+        const float v = lv[t * 64 + l];
+        const float r = lv[(t - 1) * 64 + l];   // right operand, or the operand of a unary op
+        if (op <= GPR_OP_RPOW) {
+            const int Dt = h ? rl_i(D1, li) : rl_i(D0, li);
+            int lc = 0;
+            const uint64_t m1 = HI ? ballot(l + 64 < t && D1 == Dt) : 0;
+            if (m1) lc = 64 + (63 - __builtin_clzll(m1));
+            else {
+                const uint64_t m0 = ballot(l < t && D0 == Dt);
+                if (m0) lc = 63 - __builtin_clzll(m0);
+            }
+            const float a = lv[lc * 64 + l], b = r;
+            float ga, gb;
+            switch (op) {
+            case GPR_OP_ADD: ga = G; gb = G; break;
+            case GPR_OP_SUB: ga = G; gb = -G; break;
+            case GPR_OP_RSUB: ga = -G; gb = G; break;
+            case GPR_OP_MUL: ga = G * b; gb = G * a; break;
+            case GPR_OP_DIV: ga = G / b; gb = -(G * a) / (b * b); break;
+            case GPR_OP_RDIV: gb = G / a; ga = -(G * b) / (a * a); break;
+            case GPR_OP_POW: ga = G * b * powf(a, b - 1.0f); gb = G * v * logf(a); break;
+            default: gb = G * a * powf(b, a - 1.0f); ga = G * v * logf(b); break;  // RPOW: b^a
+            }
+            A.put(0, min(asp, GPR_MAX_DEPTH - 1), ga);
+            asp = min(asp + 1, GPR_MAX_DEPTH);
+            G = gb;
+            continue;
+        }
+        const float a = r;
+        const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, li) : rl_f(p.c0, li)) : 0.0f;
+        float ga, gc = 0.0f;
+        switch (op) {
+        case GPR_OP_NEG: ga = -G; break;
+        case GPR_OP_EXP: ga = G * v; break;
+        case GPR_OP_LOG: ga = G / a; break;
+        case GPR_OP_INV: ga = -G / (a * a); break;
+        case GPR_OP_SQRT: ga = G * 0.5f / v; break;
+        case GPR_OP_SIN: ga = G * cosf(a); break;
+        case GPR_OP_COS: ga = -G * sinf(a); break;
+        case GPR_OP_TAN: ga = G * (1.0f + v * v); break;
+        case GPR_OP_ASIN: ga = G / sqrtf(1.0f - a * a); break;
+        case GPR_OP_ACOS: ga = -G / sqrtf(1.0f - a * a); break;
+        case GPR_OP_ATAN: ga = G / (1.0f + a * a); break;
+        case GPR_OP_ABS: ga = a < 0.0f ? -G : G; break;
+        case GPR_OP_CADD: ga = G; gc = G; break;
+        case GPR_OP_CMUL: ga = G * c; gc = G * a; break;
+        case GPR_OP_CPOW: ga = G * c * powf(a, c - 1.0f); gc = G * v * logf(a); break;
+        case GPR_OP_CRPOW: ga = G * v * logf(c); gc = G * a * powf(c, a - 1.0f); break;
+        default: ga = __builtin_nanf(""); break;
+        }
+        if (op >= GPR_OP_CADD) grad_add<HI>(g0, g1, t, gc, valid);
+        G = ga;
+    }
+}
+
+// Log-likelihood and d(log posterior)/d(constant) of the wave's program.  The LL is
+// bit-identical to wave_loglik<KC, HI> for every KC: lane l visits points c*64 + l in
+// increasing c, the same per-lane order and Kahan updates.  The gradient holds the
+// wave sums of the per-point adjoints (FP32) accumulated over chunks in FP64, plus the
+// Cauchy prior's -2c/(1+c^2); token t sits at lane t of g0 (t < 64) or g1.  lv: this
+// wave's LDS slab of at least len * 64 floats.  Returns false in ok when the LL or a
+// constant's gradient is not finite.
+template <bool HI>
+__device__ double wave_loglik_grad(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
+                                   const float *dw, float *lv, double &g0, double &g1, bool &ok) {
+    const int l = lane_id();
+    int D0, D1, dm;
+    wave_depths<HI>(p, D0, D1, dm);
+    g0 = 0.0;
+    g1 = 0.0;
+    float acc = 0.0f, comp = 0.0f;
+    const bool folded = d.wconst > 0.0;
+    const float wf = (float)d.wconst;
+    const int nch = (d.n + 63) / 64;
+    for (int ch = 0; ch < nch; ch++) {
+        const int i = ch * 64 + l;
+        const bool valid = i < d.n;
+        const float x = valid ? dx[i] : 0.0f, y = valid ? dy[i] : 0.0f, w = valid ? dw[i] : 0.0f;
+        const float f = wave_eval_store<HI>(p, x, lv);
+        const float r = f - y;
+        const float term = valid ? (r * r) * w : 0.0f;
+        const float yv = term - comp;
+        const float tv = acc + yv;
+        comp = (tv - acc) - yv;
+        acc = tv;
+        // d/df of -(f - y)^2 w, with w = wconst when the weights are folded into it
+        const float G = valid ? (-2.0f * r) * (folded ? wf : w) : 0.0f;
+        wave_adjoint<HI>(p, lv, G, valid, D0, D1, g0, g1);
+    }
+    const double s = wave_sum_d((double)acc - (double)comp);
+    const bool c0 = l < p.len && gpr_has_const(p.o0), c1 = HI && l + 64 < p.len && gpr_has_const(p.o1);
+    if (c0) g0 += -2.0 * (double)p.c0 / (1.0 + (double)p.c0 * (double)p.c0);
+    if (c1) g1 += -2.0 * (double)p.c1 / (1.0 + (double)p.c1 * (double)p.c1);
+    const bool bad = ballot((c0 && !__builtin_isfinite(g0)) || (c1 && !__builtin_isfinite(g1))) != 0;
+    ok = !bad && __builtin_isfinite(s);
+    if (!__builtin_isfinite(s)) return -GPR_INF;
+    return d.lconst - (folded ? d.wconst * s : s);
+}
+
+// FP64 sum of v over the lanes where m holds, in token order (lanes of the low half,
+// then the high half), as a serial loop over tokens would add them.
+template <bool HI>
+__device__ __forceinline__ double wave_ordered_sum_d(double v0, bool m0, double v1, bool m1) {
+    double s = 0.0;
+    uint64_t b0 = ballot(m0);
+    while (b0) {
+        const int b = __builtin_ctzll(b0);
+        b0 &= b0 - 1;
+        s += rl_d(v0, b);
+    }
+    if (HI) {
+        uint64_t b1 = ballot(m1);
+        while (b1) {
+            const int b = __builtin_ctzll(b1);
+            b1 &= b1 - 1;
+            s += rl_d(v1, b);
+        }
+    }
+    return s;
+}
+
 // config accept rule: invalid current always moves; log u < a otherwise.  u lies on the
 // grid ((k + 1/2) 2^-23), so log u >= -16.64: any a < -17 rejects without drawing.
 __device__ __forceinline__ bool cfg_accept(uint64_t seed, uint32_t chain, uint32_t step, double cur, double prop,

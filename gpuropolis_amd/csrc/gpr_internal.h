@@ -38,7 +38,6 @@ struct GprMHArgs {
     int32_t *len;           // [N]
     double *prior;          // [N]
     double *ll;             // [N]
-    float *grad;            // [cap][N] MALA gradient cache (may be null)
     int nchains;            // N, the stride
     const float *dx, *dy, *dw;
     const float *dsd;       // sd per point (reference lane kernel)
@@ -59,7 +58,7 @@ struct GprMHArgs {
     unsigned long long *counters;
     int32_t block_seed;
     int rng;
-    int dmax_cap;           // largest per-dataset cap (lane kernel LDS sizing)
+    int dmax_cap;           // largest per-dataset cap (LDS sizing: lane kernel tapes, MALA slabs)
 };
 
 struct GprEvalArgs {
@@ -81,7 +80,6 @@ struct GprResampleArgs {
     int32_t *len;
     double *prior;
     double *ll;
-    float *grad;
     int nchains;
     int cap_rows;           // rows of ops/cs to copy
     const GprDataset *ds;
@@ -100,7 +98,7 @@ struct GprResampleArgs {
 };
 
 // launchers (gpr_kernels.hip)
-hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, int grid, hipStream_t s);
+hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, bool mala, int grid, hipStream_t s);
 hipError_t gpr_launch_mh_config_lane(const GprMHArgs &a, int cap_max, hipStream_t s);
 hipError_t gpr_launch_mh_ref_lane(const GprMHArgs &a, hipStream_t s);
 hipError_t gpr_launch_mh_ref_wave(const GprMHArgs &a, int kc, int grid, hipStream_t s);

@@ -18,11 +18,18 @@
 #include "gpr_device.h"
 
 // ---------------------------------------------------------------- config MH, wave schedule
-template <int KC, bool HI>
+// MALA: constant moves are Langevin proposals c' = c + h g(c) + eps z (h = eps^2/2, g the
+// gradient of the log posterior from the adjoint sweep) with the Hastings ratio of the
+// two Gaussian proposal densities; each wave keeps its token values for the reverse
+// sweep in an LDS slab of dmax_cap * 64 floats.
+extern __shared__ float gpr_lds_slab[];
+
+template <int KC, bool HI, bool MALA>
 __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
     const int N = a.nchains;
+    float *lv = MALA ? gpr_lds_slab + (size_t)wave * a.dmax_cap * 64 : nullptr;
     unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0, n_evals = 0, n_steps = 0, n_flops = 0;
     for (;;) {
         __shared__ int s_tile;
@@ -55,11 +62,16 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         // per-lane constant-proposal block, reused by four consecutive steps
         gpr_u4 wlo = {0, 0, 0, 0}, whi = {0, 0, 0, 0};
         uint32_t wblk = 0xffffffffu;
+        // MALA: gradient of the current state (lane-distributed like the constants)
+        float cg0 = 0.0f, cg1 = 0.0f, pg0 = 0.0f, pg1 = 0.0f;
+        bool grad_ok = false;
         for (int i = -1; i < a.steps; i++) {
             const uint32_t step = a.step0 + (uint32_t)i;
             const double post = prior + ll;
             WaveProg prop = cur;
             double extra = 0.0;
+            bool have_pll = false;
+            double pll_m = 0.0;
             const bool structural = i >= 0 && !a.fixed_structure && (step % (uint32_t)a.struct_every) == 0;
             if (i < 0) {
             } else if (structural) {
@@ -139,11 +151,39 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                     if (HI) whi = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 1);
                 }
                 const int wi = (int)(step & 3);
-                if (k0) prop.c0 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(wlo, wi))), cur.c0);
-                if (HI && k1) prop.c1 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(whi, wi))), cur.c1);
+                if (!MALA) {
+                    if (k0) prop.c0 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(wlo, wi))), cur.c0);
+                    if (HI && k1) prop.c1 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(whi, wi))), cur.c1);
+                } else {
+                    if (!grad_ok) {
+                        double h0, h1;
+                        bool gok;
+                        (void)wave_loglik_grad<HI>(cur, d, dx, dy, dw, lv, h0, h1, gok);
+                        cg0 = gok ? (float)h0 : 0.0f;
+                        cg1 = gok ? (float)h1 : 0.0f;
+                        grad_ok = true;
+                    }
+                    const float e = a.eps, h = 0.5f * e * e;
+                    const double inv2e2 = 2.0 * (double)e * (double)e;
+                    const float m0 = gpr_axpy(h, cg0, cur.c0), m1 = gpr_axpy(h, cg1, cur.c1);
+                    if (k0) prop.c0 = gpr_axpy(e, gpr_normal(gpr_u01(gpr_u4_word(wlo, wi))), m0);
+                    if (HI && k1) prop.c1 = gpr_axpy(e, gpr_normal(gpr_u01(gpr_u4_word(whi, wi))), m1);
+                    const double f0 = (double)prop.c0 - (double)m0, f1 = (double)prop.c1 - (double)m1;
+                    const double fq = wave_ordered_sum_d<HI>(-f0 * f0 / inv2e2, k0, -f1 * f1 / inv2e2, k1);
+                    double h0, h1;
+                    bool pok;
+                    pll_m = wave_loglik_grad<HI>(prop, d, dx, dy, dw, lv, h0, h1, pok);
+                    have_pll = true;
+                    pg0 = pok ? (float)h0 : 0.0f;
+                    pg1 = pok ? (float)h1 : 0.0f;
+                    const float b0 = gpr_axpy(h, pg0, prop.c0), b1 = gpr_axpy(h, pg1, prop.c1);
+                    const double r0 = (double)cur.c0 - (double)b0, r1 = (double)cur.c1 - (double)b1;
+                    const double bq = wave_ordered_sum_d<HI>(-r0 * r0 / inv2e2, k0, -r1 * r1 / inv2e2, k1);
+                    extra = pok ? bq - fq : -GPR_INF;
+                }
             }
             const double pprior = wave_prior<HI>(prop, a.lnsym);
-            const double pll = wave_loglik<KC, HI>(prop, d, dx, dy, dw, c0, nchunks);
+            const double pll = have_pll ? pll_m : wave_loglik<KC, HI>(prop, d, dx, dy, dw, c0, nchunks);
             n_evals += (unsigned long long)prop.len * (unsigned long long)d.n;
             {
                 const int nleaf = __popcll(ballot(l < prop.len && prop.o0 <= GPR_OP_ONE)) +
@@ -158,7 +198,8 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                 cur = prop;
                 prior = pprior;
                 ll = pll;
-                if (structural) n_sacc++; else n_cacc++;
+                if (structural) { n_sacc++; grad_ok = false; }
+                else { n_cacc++; cg0 = pg0; cg1 = pg1; }
             }
         }
         n_steps += (unsigned long long)a.steps;
@@ -181,13 +222,25 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
     }
 }
 
-hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, int grid, hipStream_t s) {
-#define GPR_L(K, H) hipLaunchKernelGGL((k_mh_config_wave<K, H>), dim3(grid), dim3(GPR_WG), 0, s, a)
+template <int KC, bool HI, bool MALA>
+static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
+    size_t sh = 0;
+    if (MALA) {
+        sh = sizeof(float) * 64 * (size_t)a.dmax_cap * GPR_WAVES_PER_WG;
+        hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_mh_config_wave<KC, HI, MALA>), dim3(grid), dim3(GPR_WG), sh, s, a);
+    return hipGetLastError();
+}
+
+hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, bool mala, int grid, hipStream_t s) {
+#define GPR_L(K, H) return mala ? launch_mh_wave<K, H, true>(a, grid, s) : launch_mh_wave<K, H, false>(a, grid, s)
     if (kc == 1) { if (hi) GPR_L(1, true); else GPR_L(1, false); }
     else if (kc == 2) { if (hi) GPR_L(2, true); else GPR_L(2, false); }
     else { if (hi) GPR_L(4, true); else GPR_L(4, false); }
 #undef GPR_L
-    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- config MH, lane schedule
@@ -435,6 +488,46 @@ hipError_t gpr_launch_eval_wave(const GprEvalArgs &a, hipStream_t s) {
     else if (kc == 2) { if (hi) GPR_L(2, true); else GPR_L(2, false); }
     else { if (hi) GPR_L(4, true); else GPR_L(4, false); }
 #undef GPR_L
+    return hipGetLastError();
+}
+
+// Gradient of the log posterior of each tape (one wave per tape, LDS slab of cap * 64
+// floats per wave for the token values).  grad rows t < cap: d/dc_t, 0 where token t
+// has no constant or t >= len.  ll receives the log posterior.
+template <bool HI>
+__global__ void __launch_bounds__(GPR_WG) k_grad_wave(GprEvalArgs a) {
+    const int wave = uni((int)(threadIdx.x >> 6));
+    const int l = lane_id();
+    const int chain = uni((int)blockIdx.x * GPR_WAVES_PER_WG + wave);
+    if (chain >= a.nchains) return;
+    const int N = a.nchains;
+    float *lv = gpr_lds_slab + (size_t)wave * a.cap * 64;
+    WaveProg p;
+    p.len = uni(a.len[chain]);
+    p.o0 = l < p.len ? a.ops[(size_t)l * N + chain] : 0;
+    p.c0 = l < p.len ? a.cs[(size_t)l * N + chain] : 0.0f;
+    p.o1 = (HI && l + 64 < p.len) ? a.ops[(size_t)(l + 64) * N + chain] : 0;
+    p.c1 = (HI && l + 64 < p.len) ? a.cs[(size_t)(l + 64) * N + chain] : 0.0f;
+    double g0, g1;
+    bool ok;
+    const float *dx = a.dx + a.d.off, *dy = a.dy + a.d.off, *dw = a.dw + a.d.off;
+    const double ll = wave_loglik_grad<HI>(p, a.d, dx, dy, dw, lv, g0, g1, ok);
+    const double pr = wave_prior<HI>(p, a.lnsym);
+    if (l < a.cap) a.grad[(size_t)l * N + chain] = (l < p.len && gpr_has_const(p.o0)) ? (float)g0 : 0.0f;
+    if (HI && l + 64 < a.cap)
+        a.grad[(size_t)(l + 64) * N + chain] = (l + 64 < p.len && gpr_has_const(p.o1)) ? (float)g1 : 0.0f;
+    if (l == 0) a.ll[chain] = pr + ll;
+}
+
+hipError_t gpr_launch_grad_wave(const GprEvalArgs &a, hipStream_t s) {
+    const int grid = (a.nchains + GPR_WAVES_PER_WG - 1) / GPR_WAVES_PER_WG;
+    const size_t sh = sizeof(float) * 64 * (size_t)a.cap * GPR_WAVES_PER_WG;
+    const bool hi = a.cap > 64;
+    const void *f = hi ? (const void *)k_grad_wave<true> : (const void *)k_grad_wave<false>;
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+    if (hi) hipLaunchKernelGGL(k_grad_wave<true>, dim3(grid), dim3(GPR_WG), sh, s, a);
+    else hipLaunchKernelGGL(k_grad_wave<false>, dim3(grid), dim3(GPR_WG), sh, s, a);
     return hipGetLastError();
 }
 

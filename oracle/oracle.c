@@ -544,6 +544,33 @@ double gpo_tape_prior(const uint8_t *ops, const float *cs, int len, double log_n
     return -(double)dl * log_nsym + p;
 }
 
+/* Log-likelihood plus the Cauchy(0,1) terms of the constant tokens, everything in FP64
+ * with FP64 constants: the function whose central differences check the adjoints.
+ * Parity pinned by the closed-form tests of tests/test_oracle_sampler.py (linear and
+ * rational programs with analytic derivatives). */
+double gpo_tape_logpost_f64(const uint8_t *ops, const double *cs, int len,
+                            const float *x, const float *y, const float *sd, int n) {
+    double lp = 0.0;
+    for (int i = 0; i < n; i++) {
+        double st[256];
+        int sp = 0;
+        for (int t = 0; t < len; t++) {
+            int o = ops[t];
+            if (o == GPO_X) st[sp++] = (double)x[i];
+            else if (o == GPO_CONST) st[sp++] = cs[t];
+            else if (o == GPO_ONE) st[sp++] = 1.0;
+            else if (gpo_arity(o) == 2) { double b = st[--sp]; double a = st[sp - 1]; st[sp - 1] = tape_op_f64(o, a, b, cs[t]); }
+            else st[sp - 1] = tape_op_f64(o, st[sp - 1], 0.0, cs[t]);
+        }
+        double r = st[0] - (double)y[i];
+        double s = (double)sd[i];
+        lp += -(r * r) / (2.0 * s * s) - log(s) - HALF_LOG2PI_D;
+    }
+    for (int t = 0; t < len; t++)
+        if (ops[t] == GPO_CONST || ops[t] >= GPO_CADD) lp += -LOGPI_D - log1p(cs[t] * cs[t]);
+    return lp;
+}
+
 /* Reverse-mode adjoint over the tape in FP64.  grad[t] = d lp / d cs[t]. */
 int gpo_tape_grad(const uint8_t *ops, const float *cs, int len,
                   const float *x, const float *y, const float *sd, int n,

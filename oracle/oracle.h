@@ -94,6 +94,9 @@ double gpo_tape_loglik(const uint8_t *ops, const float *cs, int len,
                        const float *x, const float *y, const float *sd, int n, int mode);
 double gpo_tape_prior(const uint8_t *ops, const float *cs, int len, double log_nsym,
                       int cauchy_all_slots_count);
+/* log-likelihood + Cauchy terms of the constant tokens, FP64 nodes and FP64 constants */
+double gpo_tape_logpost_f64(const uint8_t *ops, const double *cs, int len,
+                            const float *x, const float *y, const float *sd, int n);
 /* d log posterior / d cs[t] for each token t (0 for non-constant tokens), FP64 */
 int gpo_tape_grad(const uint8_t *ops, const float *cs, int len,
                   const float *x, const float *y, const float *sd, int n,

@@ -93,6 +93,7 @@ def lib():
         "gpo_tape_loglik": (C.c_double, [_u8p, _f32p, C.c_int, _f32p, _f32p, _f32p, C.c_int, C.c_int]),
         "gpo_tape_prior": (C.c_double, [_u8p, _f32p, C.c_int, C.c_double, C.c_int]),
         "gpo_tape_grad": (C.c_int, [_u8p, _f32p, C.c_int, _f32p, _f32p, _f32p, C.c_int, _f64p, _f64p]),
+        "gpo_tape_logpost_f64": (C.c_double, [_u8p, _f64p, C.c_int, _f32p, _f32p, _f32p, C.c_int]),
         "gpo_cfg_gen_table": (None, [_u8p, _f64p]),
         "gpo_cfg_init_chain": (C.c_int, [C.POINTER(GpoCfg), C.c_uint32, _u8p, _f32p]),
         "gpo_cfg_run_chain": (None, [C.POINTER(GpoCfg), C.c_uint32, C.c_uint64, C.c_int, _u8p, _f32p,
@@ -191,6 +192,29 @@ def tape_grad(ops, cs, x, y, sd):
     lp = np.zeros(1, dtype=np.float64)
     rc = lib().gpo_tape_grad(u8(ops), f32(cs), len(ops), f32(x), f32(y), f32(sd), len(x), f64(g), f64(lp))
     return rc, g, float(lp[0])
+
+
+def tape_logpost_f64(ops, cs64, x, y, sd):
+    ops = np.ascontiguousarray(ops, dtype=np.uint8)
+    cs64 = np.ascontiguousarray(cs64, dtype=np.float64)
+    x, y, sd = (np.ascontiguousarray(v, dtype=np.float32) for v in (x, y, sd))
+    return lib().gpo_tape_logpost_f64(u8(ops), f64(cs64), len(ops), f32(x), f32(y), f32(sd), len(x))
+
+
+def central_diff_grad(ops, cs, x, y, sd, rel=1e-6):
+    """FP64 central differences of tape_logpost_f64 at the constant tokens; also the
+    same at half the step, so callers can see how well the difference has converged."""
+    c64 = np.asarray(cs, dtype=np.float64)
+    g, g2 = np.zeros(len(ops)), np.zeros(len(ops))
+    for t in range(len(ops)):
+        if not (ops[t] == OP["CONST"] or ops[t] >= OP["CADD"]):
+            continue
+        for out, h in ((g, rel * max(1.0, abs(c64[t]))), (g2, 0.5 * rel * max(1.0, abs(c64[t])))):
+            cp, cm = c64.copy(), c64.copy()
+            cp[t] += h
+            cm[t] -= h
+            out[t] = (tape_logpost_f64(ops, cp, x, y, sd) - tape_logpost_f64(ops, cm, x, y, sd)) / (2 * h)
+    return g, g2
 
 
 def ref_decode(hop, hc):

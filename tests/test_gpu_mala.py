@@ -1,0 +1,114 @@
+"""MALA constant proposals in the wave MH kernel.
+
+* Target invariance: on a one-constant program c*x with Gaussian data the posterior of
+  c (Gaussian likelihood x Cauchy prior) is known by quadrature; MALA and random-walk
+  chains both reproduce its mean and standard deviation (independent chains, so the
+  standard error comes from the spread of per-chain means).  A correct sampler misses
+  a 2-standard-error band in one run of 20 (over 20 seeds the oracle's z-scores here
+  have mean 0.17 and sd 0.86), so the band is 3 standard errors; the same seed on the
+  oracle gives the same estimate to a small fraction of a standard error.
+* Same algorithm as the oracle: from the same start, device chains shadow the oracle's
+  MALA chains (FP64 gradients there, FP32 adjoint sums here) on the rational workload.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+
+def toy():
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0.0, 1.0, 20).astype(np.float32)
+    y = (1.3 * x + rng.normal(0.0, 0.1, 20)).astype(np.float32)
+    sd = np.full(20, 0.1, dtype=np.float32)
+    return x, y, sd
+
+
+def quadrature_mean_sd(x, y, sd):
+    c = np.linspace(0.5, 2.1, 200001)
+    xd, yd, s = (v.astype(np.float64) for v in (x, y, sd))
+    lp = -np.sum((c[:, None] * xd[None, :] - yd[None, :]) ** 2 / (2 * s * s), axis=1) - np.log1p(c * c)
+    w = np.exp(lp - lp.max())
+    m = np.sum(w * c) / np.sum(w)
+    return m, np.sqrt(np.sum(w * (c - m) ** 2) / np.sum(w))
+
+
+@pytest.mark.parametrize("proposal", ["mala", "rw"])
+def test_posterior_mean_matches_quadrature(oracle, gpr, proposal):
+    x, y, sd = toy()
+    m_true, s_true = quadrature_mean_sd(x, y, sd)
+    nch, blocks, burn, steps = 4096, 40, 5, 10
+    ops, cs = oracle.tape_arrays(["X", ("CONST", 0.7), "MUL"])
+    O = np.zeros((32, nch), dtype=np.uint8)
+    V = np.zeros((32, nch), dtype=np.float32)
+    O[:3] = ops[:, None]
+    V[:3] = cs[:, None]
+    L = np.full(nch, 3, dtype=np.int32)
+    prop = gpr.PROP_MALA if proposal == "mala" else gpr.PROP_RW
+    with gpr.Context(cap=32, seed=3, proposal=prop, eps=0.04, sigma=0.06, fixed_structure=1,
+                     resample=gpr.RS_NONE) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(O, V, L)
+        say(f"{proposal}: {nch} chains x {blocks} blocks x {steps} steps")
+        samples = []
+        for b in range(blocks):
+            ctx.run_block(steps)
+            if b >= burn:
+                samples.append(ctx.read_state()[1][1].astype(np.float64))
+        st = ctx.stats()
+    S = np.array(samples)                      # [draws, chains]
+    chain_means = S.mean(axis=0)
+    m = chain_means.mean()
+    se = chain_means.std(ddof=1) / np.sqrt(nch)
+    acc = st.const_accepted / st.const_proposed
+    say(f"{proposal}: mean {m:.6f} quadrature {m_true:.6f} se {se:.2e} sd {S.std():.5f} vs {s_true:.5f} "
+        f"acceptance {acc:.3f}")
+    assert abs(m - m_true) < 3 * se
+    assert abs(S.std() / s_true - 1) < 0.03
+    assert 0.2 < acc < 0.99
+    # the oracle's chains from the same seed
+    cfg = oracle.make_cfg(cap=32, proposal=1 if proposal == "mala" else 0, eps=0.04, sigma=0.06,
+                          fixed_structure=1, seed=3)
+    o_S = []
+    for b in range(blocks):
+        oracle.cfg_run_block(cfg, O, V, L, x, y, sd, b * steps, steps)
+        if b >= burn:
+            o_S.append(V[1].astype(np.float64).copy())
+    m_o = np.array(o_S).mean()
+    say(f"{proposal}: oracle mean {m_o:.6f}")
+    assert abs(m - m_o) < 0.25 * se
+
+
+def test_mala_shadows_oracle_on_rational_workload(oracle, gpr):
+    from gpuropolis_amd.workloads import dataset_D, rational_tape
+    (x, y, sd), b = dataset_D(0)
+    rng = np.random.default_rng(4)
+    nch, steps, eps = 64, 40, 1e-3
+    O = np.zeros((32, nch), dtype=np.uint8)
+    V = np.zeros((32, nch), dtype=np.float32)
+    for c in range(nch):
+        o, v = rational_tape(b * (1 + 0.002 * rng.normal(size=7)))
+        O[:len(o), c] = o
+        V[:len(o), c] = v
+    L = np.full(nch, len(o), dtype=np.int32)
+    cfg = oracle.make_cfg(cap=32, proposal=1, eps=eps, fixed_structure=1, seed=9)
+    o_ops, o_cs, o_ln = O.copy(), V.copy(), L.copy()
+    p_o, l_o, cnt = oracle.cfg_run_block(cfg, o_ops, o_cs, o_ln, x, y, sd, 0, steps)
+    with gpr.Context(cap=32, seed=9, proposal=gpr.PROP_MALA, eps=eps, fixed_structure=1,
+                     resample=gpr.RS_NONE) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(O, V, L)
+        say("mala shadowing: 64 chains x 40 steps, rational workload")
+        ctx.run_block(steps)
+        g_ops, g_cs, g_ln, g_pr, g_ll = ctx.read_state()
+        st = ctx.stats()
+    rel = np.max(np.abs(g_cs[:len(o)].astype(np.float64) - o_cs[:len(o)]) /
+                 np.maximum(np.abs(o_cs[:len(o)].astype(np.float64)), 1e-3), axis=0)
+    close = rel < 1e-3
+    say("chains within 1e-3 of the oracle's MALA chain:", int(close.sum()), "of", nch,
+        "acceptance gpu", st.const_accepted / st.const_proposed, "oracle", cnt[3] / max(cnt[2], 1))
+    assert close.sum() >= int(0.9 * nch)
+    assert cnt[3] > 0.3 * cnt[2]
+    assert abs(st.const_accepted - cnt[3]) <= 0.1 * cnt[3] + 5

@@ -122,6 +122,27 @@ def test_adjoint_matches_central_differences():
         assert abs(fd - g[t]) <= 1e-4 * max(1.0, abs(g[t])), (t, fd, g[t])
 
 
+def test_fp64_logpost_and_central_differences_closed_form():
+    """c0*x + c1 with Gaussian data: log posterior and its gradient in closed form."""
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-2, 3, 40).astype(np.float32)
+    sd = rng.uniform(0.2, 0.6, 40).astype(np.float32)
+    y = (0.7 * x - 1.1 + rng.normal(0, 0.3, 40)).astype(np.float32)
+    ops, _ = O.tape_arrays(["X", ("CONST", 0.0), "MUL", ("CONST", 0.0), "ADD"])
+    for c0, c1 in [(0.5, -0.9), (1.3, 0.2), (-2.0, 4.0)]:
+        cs = np.array([0.0, c0, 0.0, c1, 0.0])
+        xd, yd, s = (v.astype(np.float64) for v in (x, y, sd))
+        r = c0 * xd + c1 - yd
+        lp = np.sum(-r * r / (2 * s * s) - np.log(s) - 0.5 * np.log(2 * np.pi)) - 2 * np.log(np.pi) \
+            - np.log1p(c0 * c0) - np.log1p(c1 * c1)
+        assert abs(O.tape_logpost_f64(ops, cs, x, y, sd) - lp) <= 1e-12 * abs(lp)
+        g0 = np.sum(-r * xd / (s * s)) - 2 * c0 / (1 + c0 * c0)
+        g1 = np.sum(-r / (s * s)) - 2 * c1 / (1 + c1 * c1)
+        fd, fd2 = O.central_diff_grad(ops, cs, x, y, sd)
+        assert abs(fd[1] - g0) <= 1e-7 * max(1, abs(g0)) and abs(fd[3] - g1) <= 1e-7 * max(1, abs(g1))
+        assert fd[0] == fd[2] == fd[4] == 0.0
+
+
 def test_map_resample_rule():
     nch, cap = 500, 8
     rng = np.random.default_rng(4)
