@@ -53,6 +53,12 @@ struct gpr_ctx {
     int32_t *d_good = nullptr, *d_ngood = nullptr, *d_map_chain = nullptr, *d_nres = nullptr;
     double *d_map_post = nullptr, *d_cutoff = nullptr;
     uint32_t step = 0, block = 0;
+    // device copy of {step, block} read by graph replays, and its host staging
+    uint32_t *d_clock = nullptr;
+    uint32_t h_clock[2] = {0, 0};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_nblocks = 0, g_steps = 0;
     bool have_data = false, have_state = false;
     double last_ms = 0.0;
     int sched_used = GPR_SCHED_WAVE;
@@ -84,16 +90,25 @@ extern "C" int gpr_config_default(gpr_config *c) {
     return GPR_OK;
 }
 
+static void drop_graph(gpr_ctx *c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->g_nblocks = c->g_steps = 0;
+}
+
 static void free_state(gpr_ctx *c) {
+    drop_graph(c);
     void *ps[] = {c->d_ds, c->d_x, c->d_y, c->d_w, c->d_sd, c->d_lnc, c->d_ops, c->d_cs, c->d_len, c->d_prior, c->d_ll,
                   c->d_tiles, c->d_tile_counter, c->d_counters, c->d_good, c->d_ngood, c->d_map_chain,
-                  c->d_nres, c->d_map_post, c->d_cutoff};
+                  c->d_nres, c->d_map_post, c->d_cutoff, c->d_clock};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     c->d_ds = nullptr; c->d_x = c->d_y = c->d_w = c->d_sd = nullptr; c->d_lnc = nullptr; c->d_ops = nullptr; c->d_cs = nullptr;
     c->d_len = nullptr; c->d_prior = c->d_ll = nullptr; c->d_tiles = nullptr;
     c->d_tile_counter = nullptr; c->d_counters = nullptr; c->d_good = c->d_ngood = c->d_map_chain = nullptr;
-    c->d_nres = nullptr; c->d_map_post = c->d_cutoff = nullptr;
+    c->d_nres = nullptr; c->d_map_post = c->d_cutoff = nullptr; c->d_clock = nullptr;
     c->have_data = c->have_state = false;
 }
 
@@ -241,6 +256,7 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     ALLOC(c->d_ngood, (size_t)nd);
     ALLOC(c->d_map_chain, (size_t)nd);
     ALLOC(c->d_map_post, (size_t)nd);
+    ALLOC(c->d_clock, 2);
     ALLOC(c->d_cutoff, (size_t)nd);
     ALLOC(c->d_nres, 1);
     HIP_TRY(hipMemcpy(c->d_ds, c->hds.data(), sizeof(GprDataset) * nd, hipMemcpyHostToDevice));
@@ -352,8 +368,12 @@ extern "C" int gpr_read_state(gpr_ctx *c, uint8_t *ops, float *cs, int32_t *len,
     return GPR_OK;
 }
 
-static int launch_block(gpr_ctx *c, int steps, int32_t block_seed) {
+// One MCMC block on the context stream.  on_device: step and block come from the
+// device clock, which a final kernel advances (the form captured into graphs); else
+// from the host counters, advanced here.
+static int launch_block(gpr_ctx *c, int steps, int32_t block_seed, bool on_device = false) {
     GprMHArgs a = mh_args(c, steps, block_seed);
+    if (on_device) a.clock = c->d_clock;
     const bool ref = c->cfg.semantics == GPR_SEM_REFERENCE;
     int sched = c->cfg.schedule;
     if (sched == GPR_SCHED_AUTO) sched = (ref && c->cfg.rng == GPR_RNG_LCG) ? GPR_SCHED_LANE : GPR_SCHED_WAVE;
@@ -383,8 +403,12 @@ static int launch_block(gpr_ctx *c, int steps, int32_t block_seed) {
         }
     }
     c->sched_used = sched;
-    c->step += (uint32_t)steps;
-    c->block++;
+    if (on_device) {
+        HIP_TRY(gpr_launch_clock_advance(c->d_clock, (uint32_t)steps, c->stream));
+    } else {
+        c->step += (uint32_t)steps;
+        c->block++;
+    }
     return GPR_OK;
 }
 
@@ -430,24 +454,73 @@ static GprResampleArgs rs_args(gpr_ctx *c) {
     return r;
 }
 
-extern "C" int gpr_resample(gpr_ctx *c) {
-    if (!c || !c->have_state) return fail(GPR_E_STATE, "no chain state");
-    if (c->cfg.resample == GPR_RS_NONE) return GPR_OK;
-    HIP_TRY(hipSetDevice(c->cfg.device));
+static int launch_resample(gpr_ctx *c, bool on_device) {
+    GprResampleArgs r = rs_args(c);
+    if (on_device) r.clock = c->d_clock;
     HIP_TRY(gpr_launch_fill_i32(c->d_nres, 1, 0, c->stream));
     if (c->cfg.resample == GPR_RS_QUANTILE) {
         if (c->cfg.semantics != GPR_SEM_REFERENCE)
             return fail(GPR_E_ARG, "the quantile re-draw rule draws heap programs: reference semantics only");
-        HIP_TRY(gpr_launch_resample_quantile(rs_args(c), c->stream));
+        HIP_TRY(gpr_launch_resample_quantile(r, c->stream));
     } else {
-        HIP_TRY(gpr_launch_resample(rs_args(c), c->stream));
+        HIP_TRY(gpr_launch_resample(r, c->stream));
     }
+    return GPR_OK;
+}
+
+extern "C" int gpr_resample(gpr_ctx *c) {
+    if (!c || !c->have_state) return fail(GPR_E_STATE, "no chain state");
+    if (c->cfg.resample == GPR_RS_NONE) return GPR_OK;
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    const int r = launch_resample(c, false);
+    if (r) return r;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return GPR_OK;
 }
 
+// Capture nblocks x (MH block, clock advance, resampling) once per (nblocks, steps)
+// and replay it; the device clock carries the step and block index across replays, so
+// a replay continues the chains exactly as the same eager calls would.  Only kernels
+// are recorded (counters are reset by fill kernels, never by memset nodes).
 extern "C" int gpr_run_blocks_graph(gpr_ctx *c, int nblocks, int steps) {
-    return fail(GPR_E_UNSUPPORTED, "graph capture not built yet");
+    if (!c || !c->have_state) return fail(GPR_E_STATE, "no chain state: call gpr_init_chains or gpr_write_state");
+    if (nblocks < 1 || steps < 1) return fail(GPR_E_ARG, "nblocks and steps must be > 0");
+    if (c->cfg.rng == GPR_RNG_LCG)
+        return fail(GPR_E_ARG, "LCG blocks are seeded from the host's rand(): run them with gpr_run_block");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    if (!c->gexec || c->g_nblocks != nblocks || c->g_steps != steps) {
+        drop_graph(c);
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        int r = GPR_OK;
+        for (int b = 0; b < nblocks && r == GPR_OK; b++) {
+            r = launch_block(c, steps, 0, true);
+            if (r == GPR_OK && c->cfg.resample != GPR_RS_NONE) r = launch_resample(c, true);
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r != GPR_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        if (e != hipSuccess) return fail(GPR_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        c->graph = g;
+        HIP_TRY(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->g_nblocks = nblocks;
+        c->g_steps = steps;
+    }
+    c->h_clock[0] = c->step;
+    c->h_clock[1] = c->block;
+    HIP_TRY(hipMemcpyAsync(c->d_clock, c->h_clock, sizeof(c->h_clock), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms / nblocks;
+    c->step += (uint32_t)(nblocks * steps);
+    c->block += (uint32_t)nblocks;
+    return GPR_OK;
 }
 
 extern "C" int gpr_get_stats(const gpr_ctx *c, gpr_stats *o) {

@@ -57,6 +57,9 @@ struct GprMHArgs {
     int fixed_structure;
     unsigned long long *counters;
     int32_t block_seed;
+    // device clock {step, block}: when set, step0 is read from clock[0] at launch (graph
+    // replays advance it on the device); null: step0 as given
+    const uint32_t *clock;
     int rng;
     int dmax_cap;           // largest per-dataset cap (LDS sizing: lane kernel tapes, MALA slabs)
 };
@@ -91,6 +94,7 @@ struct GprResampleArgs {
     double *cutoff;         // [ndatasets] quantile rule
     uint64_t seed;
     uint32_t block;
+    const uint32_t *clock;  // when set, block is read from clock[1]
     double nats;
     float quantile;
     int semantics;
@@ -110,6 +114,7 @@ hipError_t gpr_launch_init_ref(const GprMHArgs &a, hipStream_t s);
 hipError_t gpr_launch_resample(const GprResampleArgs &a, hipStream_t s);
 hipError_t gpr_launch_resample_quantile(const GprResampleArgs &a, hipStream_t s);
 hipError_t gpr_launch_fill_i32(int *p, int n, int v, hipStream_t s);
+hipError_t gpr_launch_clock_advance(uint32_t *clock, uint32_t steps, hipStream_t s);
 hipError_t gpr_launch_test_ref_decode(const uint8_t *hop, const float *hc, int nheaps, uint8_t *ops, float *cs,
                                       int32_t *meta, hipStream_t s);
 hipError_t gpr_launch_test_draws(const uint32_t *words, int n, uint64_t seed, float *out, hipStream_t s);

@@ -30,6 +30,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
     const int l = lane_id();
     const int N = a.nchains;
     float *lv = MALA ? gpr_lds_slab + (size_t)wave * a.dmax_cap * 64 : nullptr;
+    const uint32_t step0 = a.clock ? (uint32_t)uni((int)a.clock[0]) : a.step0;
     unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0, n_evals = 0, n_steps = 0, n_flops = 0;
     for (;;) {
         __shared__ int s_tile;
@@ -66,7 +67,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         float cg0 = 0.0f, cg1 = 0.0f, pg0 = 0.0f, pg1 = 0.0f;
         bool grad_ok = false;
         for (int i = -1; i < a.steps; i++) {
-            const uint32_t step = a.step0 + (uint32_t)i;
+            const uint32_t step = step0 + (uint32_t)i;
             const double post = prior + ll;
             WaveProg prop = cur;
             double extra = 0.0;
@@ -316,6 +317,7 @@ __device__ double lane_prior(LaneTape &tp, int len, double lnsym) {
 }
 
 __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
+    const uint32_t step0 = a.clock ? a.clock[0] : a.step0;
     extern __shared__ unsigned char smem[];
     const int tid = threadIdx.x;
     const int chain = blockIdx.x * GPR_CL_WG + tid;
@@ -341,7 +343,7 @@ __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
     double ll = lane_loglik(tp, len, stk, d, dx, dy, dw);
     unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0, n_evals = 0, n_flops = 0;
     for (int i = 0; i < a.steps; i++) {
-        const uint32_t step = a.step0 + (uint32_t)i;
+        const uint32_t step = step0 + (uint32_t)i;
         const double post = prior + ll;
         double extra = 0.0;
         int plen = len;
@@ -727,7 +729,7 @@ __global__ void k_rs_map_gather(GprResampleArgs a) {
     const double p = post_of(a, c);
     const int ng = a.ngood[di];
     if (ng == 0 || p >= a.map_post[di] - a.nats) return;
-    const uint32_t r = gpr_first(a.seed, (uint32_t)c, a.block, GPR_P_RESAMPLE, 0);
+    const uint32_t r = gpr_first(a.seed, (uint32_t)c, a.clock ? a.clock[1] : a.block, GPR_P_RESAMPLE, 0);
     const int j = a.good[d.chain0 + (int)(((uint64_t)r * (uint64_t)ng) >> 32)];
     const int N = a.nchains;
     for (int t = 0; t < a.cap_rows; t++) {
@@ -755,5 +757,17 @@ __global__ void k_fill_i32(int *p, int n, int v) {
 }
 hipError_t gpr_launch_fill_i32(int *p, int n, int v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+// one MCMC block done: the next block starts `steps` later and has the next index
+__global__ void k_clock_advance(uint32_t *clock, uint32_t steps) {
+    if (threadIdx.x == 0) {
+        clock[0] += steps;
+        clock[1] += 1u;
+    }
+}
+hipError_t gpr_launch_clock_advance(uint32_t *clock, uint32_t steps, hipStream_t s) {
+    hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, s, clock, steps);
     return hipGetLastError();
 }

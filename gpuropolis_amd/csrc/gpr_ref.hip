@@ -293,6 +293,7 @@ __device__ __forceinline__ bool r_accept(S &s, double ppost, double cur, double 
 
 template <int RNG>
 __global__ void __launch_bounds__(128) k_mh_ref_lane(GprMHArgs a) {
+    const uint32_t step0 = a.clock ? a.clock[0] : a.step0;
     const int chain = blockIdx.x * blockDim.x + threadIdx.x;
     if (chain >= a.nchains) return;
     int di = 0;
@@ -316,7 +317,7 @@ __global__ void __launch_bounds__(128) k_mh_ref_lane(GprMHArgs a) {
     double forward = 0.0, backward = 0.0;
     unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0;
     for (int mcmci = 0; mcmci < a.steps; mcmci++) {
-        const uint32_t step = a.step0 + (uint32_t)mcmci;
+        const uint32_t step = step0 + (uint32_t)mcmci;
         if (mcmci % 5 == 0) {
             n_sprop++;
             float length;
@@ -488,6 +489,7 @@ __device__ __forceinline__ double r_prior_wave(int dl, float hc) {
 
 template <int KC>
 __global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
+    const uint32_t step0 = a.clock ? a.clock[0] : a.step0;
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
     const int N = a.nchains;
@@ -516,7 +518,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
         double cur_post = cur_prior + cur_ll;
         double forward = 0.0, backward = 0.0;
         for (int mcmci = 0; mcmci < a.steps; mcmci++) {
-            const uint32_t step = a.step0 + (uint32_t)mcmci;
+            const uint32_t step = step0 + (uint32_t)mcmci;
             int php = hop;
             float phc = hc;
             RefTape prop = cur;
@@ -669,7 +671,7 @@ __global__ void k_rs_quantile_redraw(GprResampleArgs a) {
     int di = 0;
     while (di + 1 < a.ndatasets && a.ds[di + 1].chain0 <= c) di++;
     if (!(a.prior[c] + a.ll[c] < a.cutoff[di])) return;   // NaN posteriors never resampled
-    r_draw_chain(a.seed, (uint32_t)c, a.block, GPR_P_RESAMPLE, a.ops, a.cs, a.nchains);
+    r_draw_chain(a.seed, (uint32_t)c, a.clock ? a.clock[1] : a.block, GPR_P_RESAMPLE, a.ops, a.cs, a.nchains);
     atomicAdd(a.nresampled, 1);
 }
 

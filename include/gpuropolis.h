@@ -134,7 +134,12 @@ int gpr_run_block_async(gpr_ctx *ctx, int steps, int32_t block_seed);
  * initial distribution (the reference rule). */
 int gpr_resample(gpr_ctx *ctx);
 
-/* Capture `nblocks` x (run_block + resample) as one hipGraph and replay it. */
+/* `nblocks` x (gpr_run_block + gpr_resample) as one hipGraph: captured on the first call
+ * with these (nblocks, steps), replayed on every call.  The step and block index live
+ * on the device inside the graph, so each replay continues the chains exactly as the
+ * same eager calls would (bit-identical state).  Not for GPR_RNG_LCG (its block seeds
+ * come from the host's rand()).  last_block_ms = graph time / nblocks.  The run loop of
+ * main.cu:1156-1239 without its per-block host round trip. */
 int gpr_run_blocks_graph(gpr_ctx *ctx, int nblocks, int steps);
 
 int gpr_get_stats(const gpr_ctx *ctx, gpr_stats *out);

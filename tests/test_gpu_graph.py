@@ -1,0 +1,64 @@
+"""MCMC blocks captured as hipGraphs: every replay (with eager reads of the state in
+between) leaves exactly the state and counters the same eager gpr_run_block +
+gpr_resample calls leave, for the config sampler (random walk and MALA) and the
+reference sampler on the wave schedule."""
+import numpy as np
+import pytest
+
+from gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("config-rw", dict(cap=64, seed=4, resample_nats=5.0)),
+         ("config-mala", dict(cap=32, seed=6, proposal=1, eps=0.01, resample_nats=5.0)),
+         ("reference", dict(semantics=1, seed=8, schedule=2, resample=2))]
+
+
+def data():
+    from gpuropolis_amd import workloads as W
+    return W.dataset_S(0)
+
+
+def same_state(a, b):
+    return all(np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8)) for u, v in zip(a, b))
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_graph_replays_match_eager(gpr, name, kw):
+    nch, steps, reps = 1000, 40, 3
+    x, y, sd = data()
+    eager = []
+    with gpr.Context(**kw) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.init_chains()
+        for _ in range(reps):
+            ctx.run_block(steps)
+            ctx.resample()
+            eager.append(ctx.read_state())
+        st_e = ctx.stats()
+    with gpr.Context(**kw) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.init_chains()
+        for r in range(reps):
+            say(f"{name}: graph replay {r}")
+            ctx.run_blocks_graph(1, steps)
+            assert same_state(ctx.read_state(), eager[r]), (name, r)
+        st_g = ctx.stats()
+    for f in ("struct_proposed", "struct_accepted", "const_proposed", "const_accepted", "steps", "node_evals"):
+        assert getattr(st_e, f) == getattr(st_g, f), f
+    with gpr.Context(**kw) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.init_chains()
+        say(f"{name}: one graph of {reps} blocks")
+        ctx.run_blocks_graph(reps, steps)
+        assert same_state(ctx.read_state(), eager[-1])
+        ctx.run_block(steps)            # eager work after a replay continues the same chains
+        ctx.resample()
+        s1 = ctx.read_state()
+    with gpr.Context(**kw) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.init_chains()
+        for _ in range(reps + 1):
+            ctx.run_block(steps)
+            ctx.resample()
+        assert same_state(ctx.read_state(), s1)
