@@ -26,9 +26,18 @@ static int fail(int code, const std::string &msg) {
         if (_e != hipSuccess) return fail(GPR_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
 
+// Size classes of datasets: n <= 64, n <= 128, n > 128, each with cap <= 64 or > 64.
+// Each class has a launch shape (tuned or default) and its own persistent launch.
+static int cls_of(int n) { return n <= 64 ? 0 : (n <= 128 ? 1 : 2); }
+static const int kClassN[3] = {1, 2, 4};
+struct ClassTune {
+    int kc = 0, wpg = 4, wgpc = 8;   // points per lane per chunk, waves per workgroup, workgroups per CU
+    double ms = 0.0;                 // measured by the autotuner
+};
 struct KernelBucket {
-    int kc;
+    int cls;
     bool hi;
+    int kc, wpg, wgpc;
     int tile0, ntiles;
 };
 
@@ -47,6 +56,8 @@ struct gpr_ctx {
     double *d_prior = nullptr, *d_ll = nullptr;
     std::vector<GprTile> htiles;
     std::vector<KernelBucket> buckets;
+    ClassTune ct[3][2];
+    int tuned_sched = -1;            // schedule picked by the autotuner for GPR_SCHED_AUTO
     GprTile *d_tiles = nullptr;
     int *d_tile_counter = nullptr;  // one per bucket
     unsigned long long *d_counters = nullptr;
@@ -59,6 +70,7 @@ struct gpr_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int g_nblocks = 0, g_steps = 0;
+    bool gexec_stale = false;        // launch shapes changed since the capture
     bool have_data = false, have_state = false;
     double last_ms = 0.0;
     int sched_used = GPR_SCHED_WAVE;
@@ -166,6 +178,34 @@ static int dev_alloc(T **p, size_t count) {
 }
 #define ALLOC(p, n) do { int _r = dev_alloc(&(p), (n)); if (_r) return _r; } while (0)
 
+// Tiles of wpg chains of one dataset, grouped by size class (one persistent launch per
+// class); larger datasets first within a class (longest-processing-time order).
+static void build_tiles(gpr_ctx *c) {
+    c->htiles.clear();
+    c->buckets.clear();
+    for (int k = 2; k >= 0; k--)
+        for (int hi = 1; hi >= 0; hi--) {
+            const ClassTune &t = c->ct[k][hi];
+            KernelBucket b{k, hi != 0, t.kc, t.wpg, t.wgpc, (int)c->htiles.size(), 0};
+            std::vector<int> order;
+            for (int d = 0; d < c->nd; d++)
+                if (cls_of(c->hds[d].n) == k && ((c->hds[d].cap > 64) == (hi != 0))) order.push_back(d);
+            std::stable_sort(order.begin(), order.end(), [&](int a, int bb) { return c->hds[a].n > c->hds[bb].n; });
+            for (int d : order)
+                for (int s0 = 0; s0 < c->hds[d].nchains; s0 += t.wpg)
+                    c->htiles.push_back(GprTile{d, c->hds[d].chain0 + s0, std::min(t.wpg, c->hds[d].nchains - s0), 0});
+            b.ntiles = (int)c->htiles.size() - b.tile0;
+            if (b.ntiles) c->buckets.push_back(b);
+        }
+}
+
+static int upload_tiles(gpr_ctx *c) {
+    HIP_TRY(hipMemcpyAsync(c->d_tiles, c->htiles.data(), sizeof(GprTile) * c->htiles.size(), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
+}
+
 extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x, const float *y, const float *sd,
                             const int32_t *chains, const int32_t *caps) {
     if (!c || nd < 1 || !n || !x || !y || !sd || !chains) return fail(GPR_E_ARG, "null or empty dataset arguments");
@@ -220,23 +260,10 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     c->nd = nd;
     c->N = (int)total_chains;
     c->rows = ref ? GPR_REF_SLOTS : c->cfg.cap;
-    // tiles of GPR_WAVES_PER_WG chains of one dataset, grouped by kernel bucket
-    c->htiles.clear();
-    c->buckets.clear();
-    for (int kc : {4, 2, 1})
-        for (int hi = 1; hi >= 0; hi--) {
-            KernelBucket b{kc, hi != 0, (int)c->htiles.size(), 0};
-            // larger datasets first within a bucket: longest-processing-time order
-            std::vector<int> order;
-            for (int d = 0; d < nd; d++)
-                if (c->hds[d].kc == kc && ((c->hds[d].cap > 64) == (hi != 0))) order.push_back(d);
-            std::stable_sort(order.begin(), order.end(), [&](int a, int bb) { return c->hds[a].n > c->hds[bb].n; });
-            for (int d : order)
-                for (int s = 0; s < c->hds[d].nchains; s += GPR_WAVES_PER_WG)
-                    c->htiles.push_back(GprTile{d, c->hds[d].chain0 + s, std::min(GPR_WAVES_PER_WG, c->hds[d].nchains - s), 0});
-            b.ntiles = (int)c->htiles.size() - b.tile0;
-            if (b.ntiles) c->buckets.push_back(b);
-        }
+    for (int k = 0; k < 3; k++)
+        for (int h = 0; h < 2; h++) c->ct[k][h] = ClassTune{kClassN[k], 4, 8, 0.0};
+    c->tuned_sched = -1;
+    build_tiles(c);
     const size_t N = (size_t)c->N;
     ALLOC(c->d_ds, (size_t)nd);
     ALLOC(c->d_x, (size_t)total_pts);
@@ -249,8 +276,8 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     ALLOC(c->d_len, N);
     ALLOC(c->d_prior, N);
     ALLOC(c->d_ll, N);
-    ALLOC(c->d_tiles, c->htiles.size());
-    ALLOC(c->d_tile_counter, std::max<size_t>(1, c->buckets.size()));
+    ALLOC(c->d_tiles, N);                  // enough for one chain per tile
+    ALLOC(c->d_tile_counter, 6);
     ALLOC(c->d_counters, (size_t)GPR_NCNT);
     ALLOC(c->d_good, N);
     ALLOC(c->d_ngood, (size_t)nd);
@@ -266,6 +293,7 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     HIP_TRY(hipMemcpy(c->d_sd, sd, sizeof(float) * total_pts, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_lnc, lnc.data(), sizeof(double) * total_pts, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_tiles, c->htiles.data(), sizeof(GprTile) * c->htiles.size(), hipMemcpyHostToDevice));
+    c->gexec_stale = true;
     HIP_TRY(hipMemset(c->d_counters, 0, sizeof(unsigned long long) * GPR_NCNT));
     HIP_TRY(hipMemset(c->d_ops, 0, (size_t)c->rows * N));
     HIP_TRY(hipMemset(c->d_cs, 0, sizeof(float) * c->rows * N));
@@ -368,6 +396,20 @@ extern "C" int gpr_read_state(gpr_ctx *c, uint8_t *ops, float *cs, int32_t *len,
     return GPR_OK;
 }
 
+// The persistent wave-schedule launch of one size class (its tile counter must be 0).
+static hipError_t launch_bucket(gpr_ctx *c, const GprMHArgs &a, int b) {
+    const KernelBucket &kb = c->buckets[b];
+    GprMHArgs ab = a;
+    ab.tiles = c->d_tiles + kb.tile0;
+    ab.ntiles = kb.ntiles;
+    ab.tile_counter = c->d_tile_counter + b;
+    ab.dmax_cap = c->rows;
+    ab.wpg = kb.wpg;
+    const int grid = std::max(1, std::min(kb.ntiles, c->cus * kb.wgpc));
+    if (c->cfg.semantics == GPR_SEM_REFERENCE) return gpr_launch_mh_ref_wave(ab, kb.kc, grid, c->stream);
+    return gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, c->cfg.proposal == GPR_PROP_MALA, grid, c->stream);
+}
+
 // One MCMC block on the context stream.  on_device: step and block come from the
 // device clock, which a final kernel advances (the form captured into graphs); else
 // from the host counters, advanced here.
@@ -376,7 +418,9 @@ static int launch_block(gpr_ctx *c, int steps, int32_t block_seed, bool on_devic
     if (on_device) a.clock = c->d_clock;
     const bool ref = c->cfg.semantics == GPR_SEM_REFERENCE;
     int sched = c->cfg.schedule;
-    if (sched == GPR_SCHED_AUTO) sched = (ref && c->cfg.rng == GPR_RNG_LCG) ? GPR_SCHED_LANE : GPR_SCHED_WAVE;
+    if (sched == GPR_SCHED_AUTO)
+        sched = (ref && c->cfg.rng == GPR_RNG_LCG) ? GPR_SCHED_LANE
+                                                   : (c->tuned_sched > 0 ? c->tuned_sched : GPR_SCHED_WAVE);
     if (c->cfg.rng == GPR_RNG_LCG && sched != GPR_SCHED_LANE)
         return fail(GPR_E_ARG, "the LCG stream is sequential per chain: it needs the lane schedule");
     if (sched == GPR_SCHED_LANE) {
@@ -390,17 +434,7 @@ static int launch_block(gpr_ctx *c, int steps, int32_t block_seed, bool on_devic
         }
     } else {
         HIP_TRY(gpr_launch_fill_i32(c->d_tile_counter, (int)c->buckets.size(), 0, c->stream));
-        for (size_t b = 0; b < c->buckets.size(); b++) {
-            const KernelBucket &kb = c->buckets[b];
-            GprMHArgs ab = a;
-            ab.tiles = c->d_tiles + kb.tile0;
-            ab.ntiles = kb.ntiles;
-            ab.tile_counter = c->d_tile_counter + b;
-            const int grid = std::min(kb.ntiles, c->cus * 8);
-            ab.dmax_cap = c->rows;
-            if (ref) HIP_TRY(gpr_launch_mh_ref_wave(ab, kb.kc, grid, c->stream));
-            else HIP_TRY(gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, c->cfg.proposal == GPR_PROP_MALA, grid, c->stream));
-        }
+        for (size_t b = 0; b < c->buckets.size(); b++) HIP_TRY(launch_bucket(c, a, (int)b));
     }
     c->sched_used = sched;
     if (on_device) {
@@ -488,7 +522,8 @@ extern "C" int gpr_run_blocks_graph(gpr_ctx *c, int nblocks, int steps) {
     if (c->cfg.rng == GPR_RNG_LCG)
         return fail(GPR_E_ARG, "LCG blocks are seeded from the host's rand(): run them with gpr_run_block");
     HIP_TRY(hipSetDevice(c->cfg.device));
-    if (!c->gexec || c->g_nblocks != nblocks || c->g_steps != steps) {
+    if (!c->gexec || c->g_nblocks != nblocks || c->g_steps != steps || c->gexec_stale) {
+        c->gexec_stale = false;
         drop_graph(c);
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
         int r = GPR_OK;
@@ -521,6 +556,151 @@ extern "C" int gpr_run_blocks_graph(gpr_ctx *c, int nblocks, int steps) {
     c->step += (uint32_t)(nblocks * steps);
     c->block += (uint32_t)nblocks;
     return GPR_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+static int valid_tune(const gpr_tune &t) {
+    return (t.n_class == 1 || t.n_class == 2 || t.n_class == 4) && (t.hi == 0 || t.hi == 1) &&
+           (t.kc == 1 || t.kc == 2 || t.kc == 4) && (t.waves_per_wg == 1 || t.waves_per_wg == 2 || t.waves_per_wg == 4) &&
+           t.wg_per_cu >= 1 && t.wg_per_cu <= 64;
+}
+static int class_index(int n_class) { return n_class == 1 ? 0 : (n_class == 2 ? 1 : 2); }
+
+extern "C" int gpr_set_tuning(gpr_ctx *c, const gpr_tune *t, int n, int schedule) {
+    if (!c || !c->have_data) return fail(GPR_E_STATE, "gpr_set_data must come first");
+    if (n < 0 || (n > 0 && !t)) return fail(GPR_E_ARG, "null tuning table");
+    if (schedule != GPR_SCHED_AUTO && schedule != GPR_SCHED_LANE && schedule != GPR_SCHED_WAVE)
+        return fail(GPR_E_ARG, "bad schedule");
+    for (int i = 0; i < n; i++)
+        if (!valid_tune(t[i])) return fail(GPR_E_ARG, "tuning entry " + std::to_string(i) + " out of range");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    for (int i = 0; i < n; i++)
+        c->ct[class_index(t[i].n_class)][t[i].hi] = ClassTune{t[i].kc, t[i].waves_per_wg, t[i].wg_per_cu, t[i].ms};
+    c->tuned_sched = schedule == GPR_SCHED_AUTO ? -1 : schedule;
+    build_tiles(c);
+    c->gexec_stale = true;
+    return upload_tiles(c);
+}
+
+extern "C" int gpr_get_tuning(const gpr_ctx *c, gpr_tune *out, int max_out, int *nout, int *schedule) {
+    if (!c || !nout) return fail(GPR_E_ARG, "null argument");
+    int k = 0;
+    for (const KernelBucket &b : c->buckets) {
+        if (out && k < max_out) {
+            const ClassTune &t = c->ct[b.cls][b.hi];
+            out[k] = gpr_tune{kClassN[b.cls], b.hi ? 1 : 0, t.kc, t.wpg, t.wgpc, t.ms};
+        }
+        k++;
+    }
+    *nout = k;
+    if (schedule) *schedule = c->tuned_sched > 0 ? c->tuned_sched : GPR_SCHED_AUTO;
+    return GPR_OK;
+}
+
+// Time every candidate shape of each size class present on the context's own chains:
+// each candidate runs `steps` MH steps of that class from the same saved state, which
+// is restored at the end (chains, counters and clock are as before the call).
+extern "C" int gpr_autotune(gpr_ctx *c, int steps, gpr_tune *out, int max_out, int *nout) {
+    if (!c || !c->have_state) return fail(GPR_E_STATE, "no chain state: call gpr_init_chains or gpr_write_state");
+    if (steps < 1) return fail(GPR_E_ARG, "steps must be > 0");
+    if (c->cfg.rng == GPR_RNG_LCG) return fail(GPR_E_ARG, "LCG chains run on the lane schedule only: nothing to tune");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    const size_t N = (size_t)c->N, R = (size_t)c->rows;
+    uint8_t *s_ops = nullptr;
+    float *s_cs = nullptr;
+    int32_t *s_len = nullptr;
+    double *s_pr = nullptr, *s_ll = nullptr;
+    unsigned long long *s_cnt = nullptr;
+    auto release = [&]() {
+        void *ps[] = {s_ops, s_cs, s_len, s_pr, s_ll, s_cnt};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+    };
+#define TUNE_TRY(expr)                                                                         \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) { release(); return fail(GPR_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); } \
+    } while (0)
+    TUNE_TRY(hipMalloc((void **)&s_ops, R * N));
+    TUNE_TRY(hipMalloc((void **)&s_cs, sizeof(float) * R * N));
+    TUNE_TRY(hipMalloc((void **)&s_len, sizeof(int32_t) * N));
+    TUNE_TRY(hipMalloc((void **)&s_pr, sizeof(double) * N));
+    TUNE_TRY(hipMalloc((void **)&s_ll, sizeof(double) * N));
+    TUNE_TRY(hipMalloc((void **)&s_cnt, sizeof(unsigned long long) * GPR_NCNT));
+    auto copy_all = [&](bool save) -> hipError_t {
+        hipError_t e = hipSuccess;
+        auto cp = [&](void *dst, void *src, size_t bytes) {
+            if (e == hipSuccess) e = hipMemcpyAsync(save ? dst : src, save ? src : dst, bytes, hipMemcpyDeviceToDevice, c->stream);
+        };
+        cp(s_ops, c->d_ops, R * N);
+        cp(s_cs, c->d_cs, sizeof(float) * R * N);
+        cp(s_len, c->d_len, sizeof(int32_t) * N);
+        cp(s_pr, c->d_prior, sizeof(double) * N);
+        cp(s_ll, c->d_ll, sizeof(double) * N);
+        cp(s_cnt, c->d_counters, sizeof(unsigned long long) * GPR_NCNT);
+        return e;
+    };
+    TUNE_TRY(copy_all(true));
+    const GprMHArgs a = mh_args(c, steps, 0);
+    const bool mala = c->cfg.proposal == GPR_PROP_MALA && c->cfg.semantics == GPR_SEM_CONFIG;
+    double wave_total = 0.0;
+    // the classes present, with their current shapes
+    std::vector<std::pair<int, int>> classes;
+    for (const KernelBucket &b : c->buckets) classes.push_back({b.cls, b.hi ? 1 : 0});
+    for (auto [k, hi] : classes) {
+        ClassTune best;
+        best.ms = 1e30;
+        for (int kc : {1, 2, 4})
+            for (int wpg : {1, 2, 4})
+                for (int wpc : {8, 16, 32}) {
+                    if (c->cfg.semantics == GPR_SEM_REFERENCE && hi) continue;
+                    const ClassTune cand{kc, wpg, wpc / wpg, 0.0};
+                    c->ct[k][hi] = cand;
+                    build_tiles(c);
+                    int bi = -1;
+                    for (size_t i = 0; i < c->buckets.size(); i++)
+                        if (c->buckets[i].cls == k && c->buckets[i].hi == (hi != 0)) bi = (int)i;
+                    if (upload_tiles(c)) { release(); return GPR_E_HIP; }
+                    TUNE_TRY(copy_all(false));
+                    TUNE_TRY(gpr_launch_fill_i32(c->d_tile_counter, 6, 0, c->stream));
+                    TUNE_TRY(hipEventRecord(c->ev0, c->stream));
+                    TUNE_TRY(launch_bucket(c, a, bi));
+                    TUNE_TRY(hipEventRecord(c->ev1, c->stream));
+                    TUNE_TRY(hipEventSynchronize(c->ev1));
+                    float ms = 0.0f;
+                    TUNE_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                    if (ms < best.ms) { best = cand; best.ms = ms; }
+                }
+        c->ct[k][hi] = best;
+        wave_total += best.ms;
+    }
+    build_tiles(c);
+    if (upload_tiles(c)) { release(); return GPR_E_HIP; }
+    // schedule for GPR_SCHED_AUTO: the lane kernel over every chain against the tuned wave launches
+    c->tuned_sched = GPR_SCHED_WAVE;
+    if (c->cfg.schedule == GPR_SCHED_AUTO && !mala) {
+        TUNE_TRY(copy_all(false));
+        TUNE_TRY(hipEventRecord(c->ev0, c->stream));
+        if (c->cfg.semantics == GPR_SEM_REFERENCE) {
+            TUNE_TRY(gpr_launch_mh_ref_lane(a, c->stream));
+        } else {
+            int capmax = 1;
+            for (const GprDataset &g : c->hds) capmax = std::max(capmax, g.cap);
+            TUNE_TRY(gpr_launch_mh_config_lane(a, capmax, c->stream));
+        }
+        TUNE_TRY(hipEventRecord(c->ev1, c->stream));
+        TUNE_TRY(hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        TUNE_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (ms < wave_total) c->tuned_sched = GPR_SCHED_LANE;
+    }
+    TUNE_TRY(copy_all(false));
+    TUNE_TRY(hipStreamSynchronize(c->stream));
+#undef TUNE_TRY
+    release();
+    c->gexec_stale = true;
+    int k = 0;
+    return gpr_get_tuning(c, out, max_out, nout ? nout : &k, nullptr);
 }
 
 extern "C" int gpr_get_stats(const gpr_ctx *c, gpr_stats *o) {

@@ -16,7 +16,7 @@ struct GprDataset {
     double wconst;  // 1/(2 sd^2) in FP64 when sd is constant (then w[] holds 1), else 0
 };
 
-// A unit of work for the wave kernels: up to GPR_WAVES_PER_WG chains of one dataset.
+// A unit of work for the wave kernels: up to waves-per-workgroup chains of one dataset.
 struct GprTile {
     int dataset;
     int chain0;
@@ -62,6 +62,7 @@ struct GprMHArgs {
     const uint32_t *clock;
     int rng;
     int dmax_cap;           // largest per-dataset cap (LDS sizing: lane kernel tapes, MALA slabs)
+    int wpg;                // waves (= chains of a tile) per workgroup of the wave kernels: 1, 2 or 4
 };
 
 struct GprEvalArgs {

@@ -227,12 +227,12 @@ template <int KC, bool HI, bool MALA>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
     size_t sh = 0;
     if (MALA) {
-        sh = sizeof(float) * 64 * (size_t)a.dmax_cap * GPR_WAVES_PER_WG;
+        sh = sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg;
         hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_mh_config_wave<KC, HI, MALA>), dim3(grid), dim3(GPR_WG), sh, s, a);
+    hipLaunchKernelGGL((k_mh_config_wave<KC, HI, MALA>), dim3(grid), dim3(64 * a.wpg), sh, s, a);
     return hipGetLastError();
 }
 

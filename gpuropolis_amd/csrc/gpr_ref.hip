@@ -582,9 +582,10 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
 }
 
 hipError_t gpr_launch_mh_ref_wave(const GprMHArgs &a, int kc, int grid, hipStream_t s) {
-    if (kc == 1) hipLaunchKernelGGL((k_mh_ref_wave<1>), dim3(grid), dim3(GPR_WG), 0, s, a);
-    else if (kc == 2) hipLaunchKernelGGL((k_mh_ref_wave<2>), dim3(grid), dim3(GPR_WG), 0, s, a);
-    else hipLaunchKernelGGL((k_mh_ref_wave<4>), dim3(grid), dim3(GPR_WG), 0, s, a);
+    const dim3 wg(64 * a.wpg);
+    if (kc == 1) hipLaunchKernelGGL((k_mh_ref_wave<1>), dim3(grid), wg, 0, s, a);
+    else if (kc == 2) hipLaunchKernelGGL((k_mh_ref_wave<2>), dim3(grid), wg, 0, s, a);
+    else hipLaunchKernelGGL((k_mh_ref_wave<4>), dim3(grid), wg, 0, s, a);
     return hipGetLastError();
 }
 

@@ -30,7 +30,8 @@ MAX_CAP, MAX_DEPTH = 128, 16
 EXPORTS = ["gpr_config_default", "gpr_create", "gpr_destroy", "gpr_last_error", "gpr_version", "gpr_set_data",
            "gpr_init_chains", "gpr_write_state", "gpr_read_state", "gpr_run_block", "gpr_run_block_async",
            "gpr_resample", "gpr_run_blocks_graph", "gpr_get_stats", "gpr_reset_stats", "gpr_sync", "gpr_stream",
-           "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws"]
+           "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws",
+           "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning"]
 
 
 class Config(C.Structure):
@@ -46,6 +47,14 @@ class Stats(C.Structure):
                 ("const_proposed", C.c_longlong), ("const_accepted", C.c_longlong), ("steps", C.c_longlong),
                 ("node_evals", C.c_longlong), ("alg_flops", C.c_longlong), ("last_block_ms", C.c_double), ("schedule_used", C.c_int),
                 ("map_chain", C.c_int), ("map_posterior", C.c_double)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("n_class", C.c_int), ("hi", C.c_int), ("kc", C.c_int), ("waves_per_wg", C.c_int),
+                ("wg_per_cu", C.c_int), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class GprError(RuntimeError):
@@ -84,6 +93,9 @@ def load():
         "gpr_stream": (vp, [vp]),
         "gpr_eval_loglik": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp]),
         "gpr_grad_logpost": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, vp, vp]),
+        "gpr_autotune": (C.c_int, [vp, C.c_int, C.POINTER(Tune), C.c_int, C.POINTER(C.c_int)]),
+        "gpr_set_tuning": (C.c_int, [vp, C.POINTER(Tune), C.c_int, C.c_int]),
+        "gpr_get_tuning": (C.c_int, [vp, C.POINTER(Tune), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "gpr_ref_decode": (C.c_int, [vp, vp, C.c_size_t, vp, vp]),
         "gpr_test_ref_decode": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, vp]),
         "gpr_test_draws": (C.c_int, [vp, vp, C.c_int, C.c_uint64, vp]),
@@ -214,6 +226,27 @@ class Context:
 
     def test_draws_dev(self, d_words, n, seed, d_out):
         check(load().gpr_test_draws(self.h, d_words, n, seed, d_out))
+
+    def autotune(self, steps=50):
+        """Tune the launch shape of every size class on the current chains; returns the
+        table as a list of dicts and the schedule picked for GPR_SCHED_AUTO."""
+        out = (Tune * 6)()
+        n = C.c_int(0)
+        check(load().gpr_autotune(self.h, steps, out, 6, C.byref(n)))
+        return self.tuning()
+
+    def tuning(self):
+        out = (Tune * 6)()
+        n, sched = C.c_int(0), C.c_int(0)
+        check(load().gpr_get_tuning(self.h, out, 6, C.byref(n), C.byref(sched)))
+        return [out[i].as_dict() for i in range(n.value)], sched.value
+
+    def set_tuning(self, entries, schedule=SCHED_AUTO):
+        arr = (Tune * max(1, len(entries)))()
+        for i, e in enumerate(entries):
+            for k, v in e.items():
+                setattr(arr[i], k, v)
+        check(load().gpr_set_tuning(self.h, arr, len(entries), schedule))
 
     def grad_logpost_dev(self, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp):
         check(load().gpr_grad_logpost(self.h, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp))

@@ -142,6 +142,27 @@ int gpr_resample(gpr_ctx *ctx);
  * main.cu:1156-1239 without its per-block host round trip. */
 int gpr_run_blocks_graph(gpr_ctx *ctx, int nblocks, int steps);
 
+/* ---------------------------------------------------------------- autotuner
+ * No reference counterpart (the reference launches one fixed grid, main.cu:1022-1023).
+ * Datasets fall into size classes (n <= 64, n <= 128, n > 128; cap <= 64 or > 64),
+ * each run by its own persistent launch of the wave kernels with a launch shape:
+ * points per lane per chunk (the data tile, 64*kc points), waves = chains per
+ * workgroup, and workgroups per CU.  gpr_autotune times every candidate shape of each
+ * class present on the context's own chains (steps MH steps each, from the same
+ * state, which is restored afterwards) and keeps the fastest; with GPR_SCHED_AUTO it
+ * also picks the lane or wave schedule.  gpr_set_tuning applies a saved table. */
+typedef struct {
+    int n_class;        /* 1: n <= 64, 2: n <= 128, 4: n > 128 */
+    int hi;             /* 1: cap > 64 */
+    int kc;             /* points per lane per chunk: 1, 2 or 4 */
+    int waves_per_wg;   /* chains per workgroup: 1, 2 or 4 */
+    int wg_per_cu;      /* persistent workgroups per CU: 1..64 */
+    double ms;          /* device time of the class's launch when tuned */
+} gpr_tune;
+int gpr_autotune(gpr_ctx *ctx, int steps, gpr_tune *out, int max_out, int *nout);
+int gpr_set_tuning(gpr_ctx *ctx, const gpr_tune *t, int n, int schedule);
+int gpr_get_tuning(const gpr_ctx *ctx, gpr_tune *out, int max_out, int *nout, int *schedule);
+
 int gpr_get_stats(const gpr_ctx *ctx, gpr_stats *out);
 int gpr_reset_stats(gpr_ctx *ctx);
 int gpr_sync(gpr_ctx *ctx);
