@@ -67,7 +67,7 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_mib": mib, "bytes_per_launch": mib << 20}), flush=True)
 
 
-def time_blocks(G, ds, args, sched, blocks, warmup):
+def time_blocks(G, ds, args, sched, blocks, warmup, tune=False):
     ctx = G.Context(cap=args.cap, seed=args.seed, schedule=sched)
     ctx.set_data([ds], [args.chains])
     ctx.init_chains()
@@ -75,6 +75,9 @@ def time_blocks(G, ds, args, sched, blocks, warmup):
         ctx.run_block(args.mh_steps)
         ctx.resample()
     ctx.sync()
+    table = None
+    if tune:
+        table, _ = ctx.autotune(args.tune_steps)
     ctx.reset_stats()
     kernel_ms = 0.0
     t0 = time.perf_counter()
@@ -86,7 +89,118 @@ def time_blocks(G, ds, args, sched, blocks, warmup):
     wall = time.perf_counter() - t0
     st = ctx.stats()
     ctx.close()
-    return st, wall, kernel_ms
+    return st, wall, kernel_ms, table
+
+
+def ess_multichain(x):
+    """Effective sample size of one scalar from x[chains, draws]: the chain-averaged
+    autocovariance against the pooled variance (as split-R-hat ESS), truncated by Geyer's
+    initial monotone sequence."""
+    import numpy as np
+    m, n = x.shape
+    xc = x - x.mean(axis=1, keepdims=True)
+    f = np.fft.rfft(xc, n=2 * n, axis=1)
+    acov = np.fft.irfft(f * np.conj(f), axis=1)[:, :n].mean(axis=0) / n
+    w = acov[0] * n / (n - 1)
+    var_plus = (n - 1) / n * w + x.mean(axis=1).var(ddof=1)
+    if not var_plus > 0:
+        return float(m * n)
+    rho = 1.0 - (w - acov * n / (n - 1)) / var_plus
+    rho[0] = 1.0
+    tau, prev = -1.0, np.inf
+    for k in range(0, n - 1, 2):
+        p = rho[k] + rho[k + 1]
+        if p < 0:
+            break
+        p = min(p, prev)
+        prev = p
+        tau += 2 * p
+    return float(m * n / max(tau, 1e-9))
+
+
+def ess_rational(G, seed=0, nch=16384, thin=20, blocks=80):
+    """The differentiation workload: the fixed rational function with 7 constants, n = 37.
+    Random-walk and MALA constant moves (structure fixed), each at the step size that
+    gives it the most ESS per second among a few pilot values.  ESS per second = the
+    smallest ESS over the 7 constants (all chains pooled) / device time of the draws."""
+    import numpy as np
+    from gpuropolis_amd import workloads as W
+    (x, y, sd), b = W.dataset_D(seed)
+    rng = np.random.default_rng(seed + 100)
+    O = np.zeros((32, nch), dtype=np.uint8)
+    V = np.zeros((32, nch), dtype=np.float32)
+    for c in range(nch):
+        o, v = W.rational_tape(b * (1 + 0.002 * rng.normal(size=7)))
+        O[:len(o), c] = o
+        V[:len(o), c] = v
+    L = np.full(nch, len(o), dtype=np.int32)
+    rows = np.flatnonzero(o == G.OP["CONST"])
+
+    def run(prop, step, nblocks, burn):
+        kw = dict(eps=step) if prop == G.PROP_MALA else dict(sigma=step)
+        ctx = G.Context(cap=32, seed=seed + 7, proposal=prop, fixed_structure=1, resample=G.RS_NONE,
+                        schedule=G.SCHED_WAVE, **kw)
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(O, V, L)
+        draws, ms = [], 0.0
+        for i in range(nblocks):
+            ctx.run_block(thin)
+            if i >= burn:
+                ms += ctx.stats().last_block_ms
+                draws.append(ctx.read_state()[1][rows].astype(np.float64))
+        st = ctx.stats()
+        ctx.close()
+        D = np.array(draws)                                   # [draws, consts, chains]
+        ess = [ess_multichain(D[:, j, :].T) for j in range(len(rows))]
+        return dict(step=step, ess_min=min(ess), ess_per_s=min(ess) / (ms / 1e3), device_s=ms / 1e3,
+                    acceptance=st.const_accepted / max(1, st.const_proposed),
+                    mean=D.mean(axis=(0, 2)), sd=D.std(axis=(0, 2)), ess=np.array(ess))
+
+    out = {}
+    for name, prop, steps in (("rw", G.PROP_RW, [2e-4, 5e-4, 1e-3, 2e-3]), ("mala", G.PROP_MALA, [2.5e-4, 5e-4, 1e-3, 2e-3])):
+        pilots = [run(prop, h, 24, 8) for h in steps]
+        best = max(pilots, key=lambda r: r["ess_per_s"])
+        out[name] = run(prop, best["step"], blocks, blocks // 4)
+    # posterior means of the two samplers agree within 2 MC standard errors (sd / sqrt(ESS))
+    se = np.sqrt(out["rw"]["sd"] ** 2 / out["rw"]["ess"] + out["mala"]["sd"] ** 2 / out["mala"]["ess"])
+    z = np.abs(out["rw"]["mean"] - out["mala"]["mean"]) / se
+    res = {"workload": f"differentiation: rational function, 7 constants, n=37, {nch} chains, "
+                       f"{blocks} draws thinned by {thin} MH steps, first quarter discarded",
+           "max_z_means": float(z.max())}
+    for k in ("rw", "mala"):
+        r = out[k]
+        res[k] = {"step": r["step"], "acceptance": r["acceptance"], "ess_min": r["ess_min"],
+                  "ess_per_s": r["ess_per_s"], "device_s": r["device_s"]}
+    res["mala_over_rw_ess_per_s"] = out["mala"]["ess_per_s"] / out["rw"]["ess_per_s"]
+    return res
+
+
+def batched(G, seed=0, blocks=3, mh_steps=1000):
+    """The batched workload: 64 datasets (n log-uniform in [6, 250]), 4096 chains each,
+    caps 32/64/128 round robin; blocks of MH steps + MAP resampling replayed as hipGraphs
+    after autotuning."""
+    from gpuropolis_amd import workloads as W
+    dsets, caps, _ = W.datasets_B(seed)
+    ctx = G.Context(cap=128, seed=seed, schedule=G.SCHED_WAVE)
+    ctx.set_data(dsets, [4096] * len(dsets), caps)
+    ctx.init_chains()
+    ctx.run_blocks_graph(1, mh_steps)
+    table, _ = ctx.autotune(50)
+    ctx.run_blocks_graph(1, mh_steps)                       # capture with the tuned shapes
+    ctx.reset_stats()
+    ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(blocks):
+        ctx.run_blocks_graph(1, mh_steps)
+        ms += ctx.stats().last_block_ms
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.close()
+    return {"workload": "batched: 64 datasets, n log-uniform [6, 250], 4096 chains each, caps 32/64/128, "
+                        f"blocks of {mh_steps} MH steps + MAP-100 resample as hipGraph replays",
+            "blocks": blocks, "mh_steps_per_s": st.steps / wall, "graph_ms_per_block": ms / blocks,
+            "alg_tflops": st.alg_flops / (ms / 1e3) / 1e12,
+            "frac_valu_peak": st.alg_flops / (ms / 1e3) / VALU_LANE_OP_PEAK, "launch_shape": table}
 
 
 def main():
@@ -104,6 +218,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--other-schedule-blocks", type=int, default=2,
                     help="blocks timed on the other evaluator schedule for comparison (0: skip)")
+    ap.add_argument("--batched-blocks", type=int, default=3, help="timed blocks of the batched workload (0: skip)")
+    ap.add_argument("--no-ess", dest="ess", action="store_false", help="skip the MALA vs random-walk ESS run")
+    ap.add_argument("--tune-steps", type=int, default=50,
+                    help="autotune the wave launch shapes on this many MH steps after warm-up (0: defaults)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -121,7 +239,8 @@ def main():
 
     ds, _target = W.dataset_L(args.seed)
     sched = G.SCHED_WAVE if args.schedule == "wave" else G.SCHED_LANE
-    st, wall, kernel_ms = time_blocks(G, ds, args, sched, args.steps, args.warmup)
+    st, wall, kernel_ms, table = time_blocks(G, ds, args, sched, args.steps, args.warmup,
+                                             tune=args.tune_steps > 0 and args.schedule == "wave")
     mh_steps = st.steps
     value = mh_steps / wall
     evals_per_s = st.node_evals / wall
@@ -147,6 +266,7 @@ def main():
                    "chains": args.chains, "cap": args.cap, "n": 250, "mh_steps_per_block": args.mh_steps,
                    "schedule": args.schedule},
         "node_evals_per_s": evals_per_s,
+        "launch_shape": table,
         "acceptance": {"structure": st.struct_accepted / max(1, st.struct_proposed),
                        "constants": st.const_accepted / max(1, st.const_proposed)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OP_PEAK / 1e12,
@@ -163,13 +283,17 @@ def main():
         out["roofline"]["hbm_frac"] = traffic / avg_launch_s / HBM_PEAK
     if args.other_schedule_blocks > 0:
         other = "lane" if args.schedule == "wave" else "wave"
-        o_st, o_wall, o_ms = time_blocks(G, ds, args, G.SCHED_LANE if other == "lane" else G.SCHED_WAVE,
+        o_st, o_wall, o_ms, _ = time_blocks(G, ds, args, G.SCHED_LANE if other == "lane" else G.SCHED_WAVE,
                                          args.other_schedule_blocks, 1)
         out["schedules"] = {
             args.schedule: {"mh_steps_per_s": value, "kernel_ms_per_block": avg_launch_s * 1e3},
             other: {"mh_steps_per_s": o_st.steps / o_wall, "blocks": args.other_schedule_blocks,
                     "kernel_ms_per_block": o_ms / args.other_schedule_blocks,
                     "alg_tflops": o_st.alg_flops / (o_ms / 1e3) / 1e12}}
+    if args.batched_blocks > 0:
+        out["batched"] = batched(G, args.seed, args.batched_blocks, args.mh_steps)
+    if args.ess:
+        out["ess"] = ess_rational(G, args.seed)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ds, args.cap, args.seed, args.cpu_chains, args.cpu_steps)
     print(json.dumps(out), flush=True)

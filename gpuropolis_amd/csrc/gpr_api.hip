@@ -31,7 +31,7 @@ static int fail(int code, const std::string &msg) {
 static int cls_of(int n) { return n <= 64 ? 0 : (n <= 128 ? 1 : 2); }
 static const int kClassN[3] = {1, 2, 4};
 struct ClassTune {
-    int kc = 0, wpg = 4, wgpc = 8;   // points per lane per chunk, waves per workgroup, workgroups per CU
+    int kc = 0, wpg = 1, wgpc = 16;  // points per lane per chunk, waves per workgroup, workgroups per CU
     double ms = 0.0;                 // measured by the autotuner
 };
 struct KernelBucket {
@@ -261,7 +261,7 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     c->N = (int)total_chains;
     c->rows = ref ? GPR_REF_SLOTS : c->cfg.cap;
     for (int k = 0; k < 3; k++)
-        for (int h = 0; h < 2; h++) c->ct[k][h] = ClassTune{kClassN[k], 4, 8, 0.0};
+        for (int h = 0; h < 2; h++) c->ct[k][h] = ClassTune{kClassN[k], 1, 16, 0.0};
     c->tuned_sched = -1;
     build_tiles(c);
     const size_t N = (size_t)c->N;
