@@ -141,7 +141,12 @@ int gpo_cfg_resample(int nchains, uint32_t chain0, uint32_t block, uint64_t seed
                      double threshold, uint8_t *ops, float *cs, int32_t *len,
                      double *prior, double *ll, int32_t *src_out);
 
-/* ---------------- reference-mode sampler (main.cu:679-813) */
+/* ---------------- reference-mode sampler (main.cu:679-813)
+ * Parity unpinned at the level of the reference's quirks (stale forward/backward sums,
+ * the integer constant proposal, all-slot structure choice, main.cu:695-806): the
+ * reference needs nvcc and cuda_runtime.h, so no golden trajectories of it exist here.
+ * Pinned: its draw transforms (known answers, moments), its folds and densities
+ * (closed forms), its init distribution and its median resampling rule. */
 typedef struct {
     int rng;            /* 0 = Philox substreams, 1 = reference LCG */
     uint64_t seed;
