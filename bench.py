@@ -118,11 +118,13 @@ def ess_multichain(x):
     return float(m * n / max(tau, 1e-9))
 
 
-def ess_rational(G, seed=0, nch=16384, thin=20, blocks=80):
+def ess_rational(G, seed=0, nch=16384, thin=2, blocks=160):
     """The differentiation workload: the fixed rational function with 7 constants, n = 37.
-    Random-walk and MALA constant moves (structure fixed), each at the step size that
-    gives it the most ESS per second among a few pilot values.  ESS per second = the
-    smallest ESS over the 7 constants (all chains pooled) / device time of the draws."""
+    Warm-up adapts a proposal metric (Cholesky factor of the chains' cross-chain
+    covariance, three rounds).  Then random-walk and MALA constant moves run with that
+    metric, each at the step size giving it the most ESS per second among pilot values.
+    ESS per second = the smallest ESS over the 7 constants (all chains pooled) per MH
+    step x MH steps per second of device time, with the cost per MH step taken from a 1000-step block."""
     import numpy as np
     from gpuropolis_amd import workloads as W
     (x, y, sd), b = W.dataset_D(seed)
@@ -136,41 +138,63 @@ def ess_rational(G, seed=0, nch=16384, thin=20, blocks=80):
     L = np.full(nch, len(o), dtype=np.int32)
     rows = np.flatnonzero(o == G.OP["CONST"])
 
-    def run(prop, step, nblocks, burn):
+    def context(prop, step, cs, chol):
         kw = dict(eps=step) if prop == G.PROP_MALA else dict(sigma=step)
         ctx = G.Context(cap=32, seed=seed + 7, proposal=prop, fixed_structure=1, resample=G.RS_NONE,
                         schedule=G.SCHED_WAVE, **kw)
         ctx.set_data([(x, y, sd)], [nch])
-        ctx.write_state(O, V, L)
-        draws, ms = [], 0.0
+        if chol is not None:
+            ctx.set_metric(0, rows, chol)
+        ctx.write_state(O, cs, L)
+        return ctx
+
+    # warm-up: identity metric, then twice the metric of the chains' spread
+    cs, chol = V.copy(), None
+    for step, nsteps in ((1e-3, 1000), (1.0, 500), (1.0, 500)):
+        ctx = context(G.PROP_MALA, step, cs, chol)
+        ctx.run_block(nsteps)
+        cs = ctx.read_state()[1]
+        ctx.close()
+        cov = np.cov(cs[rows].astype(np.float64))
+        chol = np.linalg.cholesky(cov + 1e-12 * np.eye(len(rows)) * np.trace(cov))
+
+    def run(prop, step, nblocks, burn):
+        ctx = context(prop, step, cs, chol)
+        draws = []
         for i in range(nblocks):
             ctx.run_block(thin)
             if i >= burn:
-                ms += ctx.stats().last_block_ms
                 draws.append(ctx.read_state()[1][rows].astype(np.float64))
         st = ctx.stats()
+        # cost per MH step from one long block (short blocks pay the per-launch start-up)
+        ctx.reset_stats()
+        ctx.run_block(1000)
+        rate = ctx.stats().steps / (ctx.stats().last_block_ms / 1e3)
         ctx.close()
         D = np.array(draws)                                   # [draws, consts, chains]
         ess = [ess_multichain(D[:, j, :].T) for j in range(len(rows))]
-        return dict(step=step, ess_min=min(ess), ess_per_s=min(ess) / (ms / 1e3), device_s=ms / 1e3,
+        chain_steps = (nblocks - burn) * thin * nch
+        return dict(step=step, ess_min=min(ess), ess_per_s=min(ess) / chain_steps * rate,
+                    mh_steps_per_s=rate, ess_per_step=min(ess) / chain_steps,
                     acceptance=st.const_accepted / max(1, st.const_proposed),
                     mean=D.mean(axis=(0, 2)), sd=D.std(axis=(0, 2)), ess=np.array(ess))
 
     out = {}
-    for name, prop, steps in (("rw", G.PROP_RW, [2e-4, 5e-4, 1e-3, 2e-3]), ("mala", G.PROP_MALA, [2.5e-4, 5e-4, 1e-3, 2e-3])):
-        pilots = [run(prop, h, 24, 8) for h in steps]
+    for name, prop, steps in (("rw", G.PROP_RW, [0.3, 0.5, 0.8, 1.2]), ("mala", G.PROP_MALA, [0.6, 0.9, 1.2, 1.6])):
+        pilots = [run(prop, h, 60, 20) for h in steps]
         best = max(pilots, key=lambda r: r["ess_per_s"])
         out[name] = run(prop, best["step"], blocks, blocks // 4)
     # posterior means of the two samplers agree within 2 MC standard errors (sd / sqrt(ESS))
     se = np.sqrt(out["rw"]["sd"] ** 2 / out["rw"]["ess"] + out["mala"]["sd"] ** 2 / out["mala"]["ess"])
     z = np.abs(out["rw"]["mean"] - out["mala"]["mean"]) / se
-    res = {"workload": f"differentiation: rational function, 7 constants, n=37, {nch} chains, "
-                       f"{blocks} draws thinned by {thin} MH steps, first quarter discarded",
+    res = {"workload": f"differentiation: rational function, 7 constants, n=37, {nch} chains, proposal metric "
+                       f"adapted in warm-up, {blocks} draws thinned by {thin} MH steps, first quarter discarded",
            "max_z_means": float(z.max())}
     for k in ("rw", "mala"):
         r = out[k]
         res[k] = {"step": r["step"], "acceptance": r["acceptance"], "ess_min": r["ess_min"],
-                  "ess_per_s": r["ess_per_s"], "device_s": r["device_s"]}
+                  "ess_per_step": r["ess_per_step"], "mh_steps_per_s": r["mh_steps_per_s"],
+                  "ess_per_s": r["ess_per_s"]}
     res["mala_over_rw_ess_per_s"] = out["mala"]["ess_per_s"] / out["rw"]["ess_per_s"]
     return res
 

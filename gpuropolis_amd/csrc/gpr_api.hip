@@ -59,6 +59,9 @@ struct gpr_ctx {
     ClassTune ct[3][2];
     int tuned_sched = -1;            // schedule picked by the autotuner for GPR_SCHED_AUTO
     GprTile *d_tiles = nullptr;
+    std::vector<GprMetric> hmetric;  // per dataset; k = 0: identity
+    GprMetric *d_metric = nullptr;
+    bool any_metric = false;
     int *d_tile_counter = nullptr;  // one per bucket
     unsigned long long *d_counters = nullptr;
     int32_t *d_good = nullptr, *d_ngood = nullptr, *d_map_chain = nullptr, *d_nres = nullptr;
@@ -114,13 +117,15 @@ static void free_state(gpr_ctx *c) {
     drop_graph(c);
     void *ps[] = {c->d_ds, c->d_x, c->d_y, c->d_w, c->d_sd, c->d_lnc, c->d_ops, c->d_cs, c->d_len, c->d_prior, c->d_ll,
                   c->d_tiles, c->d_tile_counter, c->d_counters, c->d_good, c->d_ngood, c->d_map_chain,
-                  c->d_nres, c->d_map_post, c->d_cutoff, c->d_clock};
+                  c->d_nres, c->d_map_post, c->d_cutoff, c->d_clock, c->d_metric};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     c->d_ds = nullptr; c->d_x = c->d_y = c->d_w = c->d_sd = nullptr; c->d_lnc = nullptr; c->d_ops = nullptr; c->d_cs = nullptr;
     c->d_len = nullptr; c->d_prior = c->d_ll = nullptr; c->d_tiles = nullptr;
     c->d_tile_counter = nullptr; c->d_counters = nullptr; c->d_good = c->d_ngood = c->d_map_chain = nullptr;
-    c->d_nres = nullptr; c->d_map_post = c->d_cutoff = nullptr; c->d_clock = nullptr;
+    c->d_nres = nullptr; c->d_map_post = c->d_cutoff = nullptr; c->d_clock = nullptr; c->d_metric = nullptr;
+    c->hmetric.clear();
+    c->any_metric = false;
     c->have_data = c->have_state = false;
 }
 
@@ -284,6 +289,9 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     ALLOC(c->d_map_chain, (size_t)nd);
     ALLOC(c->d_map_post, (size_t)nd);
     ALLOC(c->d_clock, 2);
+    ALLOC(c->d_metric, (size_t)nd);
+    c->hmetric.assign(nd, GprMetric{});
+    for (GprMetric &m : c->hmetric) memset(m.slot, -1, sizeof(m.slot));
     ALLOC(c->d_cutoff, (size_t)nd);
     ALLOC(c->d_nres, 1);
     HIP_TRY(hipMemcpy(c->d_ds, c->hds.data(), sizeof(GprDataset) * nd, hipMemcpyHostToDevice));
@@ -314,6 +322,7 @@ static GprMHArgs mh_args(gpr_ctx *c, int steps, int32_t block_seed) {
     a.dsd = c->d_sd;
     a.lnc = c->d_lnc;
     a.ds = c->d_ds;
+    a.metric = c->any_metric ? c->d_metric : nullptr;
     a.tiles = c->d_tiles;
     a.ntiles = (int)c->htiles.size();
     a.tile_counter = c->d_tile_counter;
@@ -555,6 +564,61 @@ extern "C" int gpr_run_blocks_graph(gpr_ctx *c, int nblocks, int steps) {
     c->last_ms = ms / nblocks;
     c->step += (uint32_t)(nblocks * steps);
     c->block += (uint32_t)nblocks;
+    return GPR_OK;
+}
+
+// M = L L^T (FP64 sums in increasing index order, rounded to FP32) and L^-1 (FP64 forward
+// substitution): the derived forms the kernels use, in the oracle's operation order.
+static void metric_derive(GprMetric &m) {
+#pragma clang fp contract(off)
+    const int k = m.k, S = GPR_METRIC_MAX;
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++) {
+            double s = 0.0;
+            for (int q = 0; q <= std::min(i, j); q++) s = s + (double)m.L[i * S + q] * (double)m.L[j * S + q];
+            m.M[i * S + j] = (float)s;
+        }
+    for (int c = 0; c < k; c++)
+        for (int i = 0; i < k; i++) {
+            if (i < c) { m.Li[i * S + c] = 0.0; continue; }
+            double s = i == c ? 1.0 : 0.0;
+            for (int q = c; q < i; q++) s = s - (double)m.L[i * S + q] * m.Li[q * S + c];
+            m.Li[i * S + c] = s / (double)m.L[i * S + i];
+        }
+}
+
+extern "C" int gpr_set_metric(gpr_ctx *c, int dataset, int k, const int32_t *rows, const float *chol) {
+    if (!c || !c->have_data) return fail(GPR_E_STATE, "gpr_set_data must come first");
+    if (dataset < 0 || dataset >= c->nd) return fail(GPR_E_ARG, "dataset index out of range");
+    if (c->cfg.semantics != GPR_SEM_CONFIG) return fail(GPR_E_ARG, "a proposal metric needs config semantics");
+    if (k > 0 && !c->cfg.fixed_structure)
+        return fail(GPR_E_ARG, "a proposal metric names token rows: it needs fixed_structure");
+    if (k < 0 || k > GPR_METRIC_MAX) return fail(GPR_E_ARG, "metric size must be in [0, 16]");
+    if (k > 0 && (!rows || !chol)) return fail(GPR_E_ARG, "null metric rows or factor");
+    GprMetric m{};
+    memset(m.slot, -1, sizeof(m.slot));
+    m.k = k;
+    for (int i = 0; i < k; i++) {
+        if (rows[i] < 0 || rows[i] >= c->hds[dataset].cap) return fail(GPR_E_ARG, "metric row outside the cap");
+        if (m.slot[rows[i]] >= 0) return fail(GPR_E_ARG, "metric rows must be distinct");
+        m.rows[i] = rows[i];
+        m.slot[rows[i]] = (int8_t)i;
+        for (int j = 0; j < GPR_METRIC_MAX; j++) {
+            const float v = j <= i && j < k ? chol[i * k + j] : 0.0f;
+            if (!std::isfinite(v)) return fail(GPR_E_ARG, "metric factor must be finite");
+            m.L[i * GPR_METRIC_MAX + j] = v;
+        }
+        if (!(m.L[i * GPR_METRIC_MAX + i] > 0.0f)) return fail(GPR_E_ARG, "metric factor needs a positive diagonal");
+    }
+    metric_derive(m);
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    c->hmetric[dataset] = m;
+    c->any_metric = false;
+    for (const GprMetric &g : c->hmetric) c->any_metric = c->any_metric || g.k > 0;
+    HIP_TRY(hipMemcpyAsync(c->d_metric + dataset, &c->hmetric[dataset], sizeof(GprMetric), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->gexec_stale = true;
     return GPR_OK;
 }
 

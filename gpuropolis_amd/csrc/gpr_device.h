@@ -518,6 +518,49 @@ __device__ __forceinline__ double wave_ordered_sum_d(double v0, bool m0, double 
     return s;
 }
 
+// ---------------------------------------------------------------- proposal metric
+// Lanes 0..k-1 hold metric rows; L is the row-major lower factor in global memory.
+// Products accumulate in FP32 in increasing index order, the density's forward
+// substitution in FP64, both without contraction (the oracle's operation order).
+
+// d_i = sum_{j <= i} L_ij v_j, v and d at lanes 0..k-1
+__device__ __forceinline__ float metric_Lv(const float *L, int k, float v) {
+    const int l = lane_id();
+    float acc = 0.0f;
+    for (int j = 0; j < k; j++) {
+        const float vj = rl_f(v, j);
+        if (l < k && j <= l) acc = gpr_axpy(L[l * GPR_METRIC_MAX + j], vj, acc);
+    }
+    return acc;
+}
+// v_i = sum_j M_ij g_j, g and v at lanes 0..k-1
+__device__ __forceinline__ float metric_Mv(const float *M, int k, float g) {
+    const int l = lane_id();
+    float acc = 0.0f;
+    for (int j = 0; j < k; j++) {
+        const float gj = rl_f(g, j);
+        if (l < k) acc = gpr_axpy(M[l * GPR_METRIC_MAX + j], gj, acc);
+    }
+    return acc;
+}
+// log density of a Gaussian proposal with covariance e^2 L L^T at offset r (lanes
+// 0..k-1), up to its constant: -|w|^2 / (2 e^2) with w = L^-1 r, summed in row order
+__device__ __forceinline__ double metric_logq(const double *Li, int k, double r, double two_e2) {
+#pragma clang fp contract(off)
+    const int l = lane_id();
+    double w = 0.0;
+    for (int j = 0; j < k; j++) {
+        const double rj = rl_d(r, j);
+        if (l < k && j <= l) w = w + Li[l * GPR_METRIC_MAX + j] * rj;
+    }
+    double q = 0.0;
+    for (int i = 0; i < k; i++) {
+        const double wi = rl_d(w, i);
+        q += -wi * wi / two_e2;
+    }
+    return q;
+}
+
 // config accept rule: invalid current always moves; log u < a otherwise.  u lies on the
 // grid ((k + 1/2) 2^-23), so log u >= -16.64: any a < -17 rejects without drawing.
 __device__ __forceinline__ bool cfg_accept(uint64_t seed, uint32_t chain, uint32_t step, double cur, double prop,

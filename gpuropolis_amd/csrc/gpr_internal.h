@@ -16,6 +16,19 @@ struct GprDataset {
     double wconst;  // 1/(2 sd^2) in FP64 when sd is constant (then w[] holds 1), else 0
 };
 
+// Proposal metric of one dataset's constant moves: tokens rows[i] (i < k <= 16) move
+// with the lower-triangular factor L (row-major 16 x 16); slot[t] is the metric row of
+// token t or -1.  k = 0: the identity metric.  M and Li are derived from L on the host.
+#define GPR_METRIC_MAX 16
+struct GprMetric {
+    int k;
+    int rows[GPR_METRIC_MAX];
+    int8_t slot[GPR_MAX_CAP];
+    float L[GPR_METRIC_MAX * GPR_METRIC_MAX];    // lower factor: proposal noise L z
+    float M[GPR_METRIC_MAX * GPR_METRIC_MAX];    // L L^T (FP64 sums rounded): Langevin drift M g
+    double Li[GPR_METRIC_MAX * GPR_METRIC_MAX];  // L^-1 (FP64): proposal density
+};
+
 // A unit of work for the wave kernels: up to waves-per-workgroup chains of one dataset.
 struct GprTile {
     int dataset;
@@ -43,6 +56,7 @@ struct GprMHArgs {
     const float *dsd;       // sd per point (reference lane kernel)
     const double *lnc;      // 0.5*(logf(2 pi) + 2 logf(sd)) per point, host libm (main.cu:300)
     const GprDataset *ds;
+    const GprMetric *metric;  // [ndatasets] or null (identity metric everywhere)
     const GprTile *tiles;
     int ntiles;
     int *tile_counter;      // persistent work queue (reset by the host before launch)

@@ -57,6 +57,20 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         Chunk<KC> c0;
         c0.load(dx, dy, dw, d.n, 0);
 
+        // proposal metric of this dataset; a chain whose metric rows are not all constant
+        // tokens moves every constant with the identity
+        const GprMetric *mt = a.metric ? a.metric + tile.dataset : nullptr;
+        int mk = mt ? uni(mt->k) : 0;
+        const int mrow = (l < mk) ? mt->rows[l] : 0;
+        if (mk) {
+            const int orow = gather_i(cur.o0, cur.o1, mrow, HI);
+            if (ballot(l < mk && (mrow >= cur.len || orow != GPR_OP_CONST))) mk = 0;
+        }
+        const int slot0 = (mk && l < cap) ? mt->slot[l] : -1;
+        const int slot1 = (HI && mk && l + 64 < cap) ? mt->slot[l + 64] : -1;
+        const float *mL = mk ? mt->L : nullptr, *mM = mk ? mt->M : nullptr;
+        const double *mLi = mk ? mt->Li : nullptr;
+
         // iteration -1 evaluates the current state (main.cu:683-684); the prior and
         // likelihood have a single call site so the kernel body stays small
         double prior = 0.0, ll = 0.0;
@@ -152,9 +166,28 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                     if (HI) whi = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 1);
                 }
                 const int wi = (int)(step & 3);
+                // identity-metric constants; metric rows move together below
+                const bool i0 = k0 && slot0 < 0, i1 = k1 && slot1 < 0;
+                const float z0 = gpr_normal(gpr_u01(gpr_u4_word(wlo, wi)));
+                const float z1 = HI ? gpr_normal(gpr_u01(gpr_u4_word(whi, wi))) : 0.0f;
+                float zr = 0.0f, cr = 0.0f, dr = 0.0f;
+                if (mk) {
+                    zr = gather_f(z0, z1, mrow, HI);
+                    cr = gather_f(cur.c0, cur.c1, mrow, HI);
+                    dr = metric_Lv(mL, mk, zr);
+                }
                 if (!MALA) {
-                    if (k0) prop.c0 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(wlo, wi))), cur.c0);
-                    if (HI && k1) prop.c1 = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(whi, wi))), cur.c1);
+                    if (i0) prop.c0 = gpr_axpy(a.sigma, z0, cur.c0);
+                    if (HI && i1) prop.c1 = gpr_axpy(a.sigma, z1, cur.c1);
+                    if (mk) {
+                        const float cn = gpr_axpy(a.sigma, dr, cr);
+                        const float v0 = shfl_all(cn, max(slot0, 0));
+                        if (slot0 >= 0) prop.c0 = v0;
+                        if (HI) {
+                            const float v1 = shfl_all(cn, max(slot1, 0));
+                            if (slot1 >= 0) prop.c1 = v1;
+                        }
+                    }
                 } else {
                     if (!grad_ok) {
                         double h0, h1;
@@ -167,10 +200,24 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                     const float e = a.eps, h = 0.5f * e * e;
                     const double inv2e2 = 2.0 * (double)e * (double)e;
                     const float m0 = gpr_axpy(h, cg0, cur.c0), m1 = gpr_axpy(h, cg1, cur.c1);
-                    if (k0) prop.c0 = gpr_axpy(e, gpr_normal(gpr_u01(gpr_u4_word(wlo, wi))), m0);
-                    if (HI && k1) prop.c1 = gpr_axpy(e, gpr_normal(gpr_u01(gpr_u4_word(whi, wi))), m1);
+                    if (i0) prop.c0 = gpr_axpy(e, z0, m0);
+                    if (HI && i1) prop.c1 = gpr_axpy(e, z1, m1);
                     const double f0 = (double)prop.c0 - (double)m0, f1 = (double)prop.c1 - (double)m1;
-                    const double fq = wave_ordered_sum_d<HI>(-f0 * f0 / inv2e2, k0, -f1 * f1 / inv2e2, k1);
+                    double fq = wave_ordered_sum_d<HI>(-f0 * f0 / inv2e2, i0, -f1 * f1 / inv2e2, i1);
+                    float cn = 0.0f;
+                    if (mk) {
+                        // c' = c + h M g + e L z over the metric rows
+                        const float gr = gather_f(cg0, cg1, mrow, HI);
+                        const float mean = gpr_axpy(h, metric_Mv(mM, mk, gr), cr);
+                        cn = gpr_axpy(e, dr, mean);
+                        fq += metric_logq(mLi, mk, l < mk ? (double)cn - (double)mean : 0.0, inv2e2);
+                        const float v0 = shfl_all(cn, max(slot0, 0));
+                        if (slot0 >= 0) prop.c0 = v0;
+                        if (HI) {
+                            const float v1 = shfl_all(cn, max(slot1, 0));
+                            if (slot1 >= 0) prop.c1 = v1;
+                        }
+                    }
                     double h0, h1;
                     bool pok;
                     pll_m = wave_loglik_grad<HI>(prop, d, dx, dy, dw, lv, h0, h1, pok);
@@ -179,7 +226,12 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                     pg1 = pok ? (float)h1 : 0.0f;
                     const float b0 = gpr_axpy(h, pg0, prop.c0), b1 = gpr_axpy(h, pg1, prop.c1);
                     const double r0 = (double)cur.c0 - (double)b0, r1 = (double)cur.c1 - (double)b1;
-                    const double bq = wave_ordered_sum_d<HI>(-r0 * r0 / inv2e2, k0, -r1 * r1 / inv2e2, k1);
+                    double bq = wave_ordered_sum_d<HI>(-r0 * r0 / inv2e2, i0, -r1 * r1 / inv2e2, i1);
+                    if (mk) {
+                        const float gr = gather_f(pg0, pg1, mrow, HI);
+                        const float mb = gpr_axpy(h, metric_Mv(mM, mk, gr), cn);
+                        bq += metric_logq(mLi, mk, l < mk ? (double)cr - (double)mb : 0.0, inv2e2);
+                    }
                     extra = pok ? bq - fq : -GPR_INF;
                 }
             }

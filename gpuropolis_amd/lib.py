@@ -31,7 +31,7 @@ EXPORTS = ["gpr_config_default", "gpr_create", "gpr_destroy", "gpr_last_error", 
            "gpr_init_chains", "gpr_write_state", "gpr_read_state", "gpr_run_block", "gpr_run_block_async",
            "gpr_resample", "gpr_run_blocks_graph", "gpr_get_stats", "gpr_reset_stats", "gpr_sync", "gpr_stream",
            "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws",
-           "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning"]
+           "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning", "gpr_set_metric"]
 
 
 class Config(C.Structure):
@@ -94,6 +94,7 @@ def load():
         "gpr_eval_loglik": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp]),
         "gpr_grad_logpost": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, vp, vp]),
         "gpr_autotune": (C.c_int, [vp, C.c_int, C.POINTER(Tune), C.c_int, C.POINTER(C.c_int)]),
+        "gpr_set_metric": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
         "gpr_set_tuning": (C.c_int, [vp, C.POINTER(Tune), C.c_int, C.c_int]),
         "gpr_get_tuning": (C.c_int, [vp, C.POINTER(Tune), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "gpr_ref_decode": (C.c_int, [vp, vp, C.c_size_t, vp, vp]),
@@ -247,6 +248,13 @@ class Context:
             for k, v in e.items():
                 setattr(arr[i], k, v)
         check(load().gpr_set_tuning(self.h, arr, len(entries), schedule))
+
+    def set_metric(self, dataset, rows, chol):
+        """Proposal metric of dataset's constant moves: token rows and the lower Cholesky
+        factor of the proposal covariance (rows=[] removes it)."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        L = np.ascontiguousarray(np.tril(np.asarray(chol, dtype=np.float64)), dtype=np.float32) if len(r) else None
+        check(load().gpr_set_metric(self.h, dataset, len(r), _p(r) if len(r) else None, _p(L)))
 
     def grad_logpost_dev(self, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp):
         check(load().gpr_grad_logpost(self.h, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp))

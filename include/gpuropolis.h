@@ -142,6 +142,16 @@ int gpr_resample(gpr_ctx *ctx);
  * main.cu:1156-1239 without its per-block host round trip. */
 int gpr_run_blocks_graph(gpr_ctx *ctx, int nblocks, int steps);
 
+/* Proposal metric of one dataset's constant moves (no reference counterpart).  With
+ * fixed_structure, token rows[i] (i < k <= 16) of every chain of the dataset hold
+ * constants; chol is the k x k lower-triangular Cholesky factor L (row-major, upper
+ * part ignored) of the proposal covariance M = L L^T.  Random walk: c' = c + sigma L z on
+ * those rows; MALA: c' = c + (eps^2/2) M g + eps L z with the matching Gaussian
+ * densities in the Hastings ratio.  Other constant tokens keep the identity; a chain
+ * whose rows are not all constant tokens uses the identity for every constant.  k = 0
+ * removes the metric. */
+int gpr_set_metric(gpr_ctx *ctx, int dataset, int k, const int32_t *rows, const float *chol);
+
 /* ---------------------------------------------------------------- autotuner
  * No reference counterpart (the reference launches one fixed grid, main.cu:1022-1023).
  * Datasets fall into size classes (n <= 64, n <= 128, n > 128; cap <= 64 or > 64),

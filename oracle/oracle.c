@@ -730,6 +730,60 @@ static int cfg_accept(const gpo_cfg *cfg, uint32_t chain, uint32_t step, double 
     return (double)gpo_logf(u) < a;
 }
 
+/* The proposal metric M = L L^T of the constant tokens cfg->mrows (no reference
+ * counterpart).  Derived once per chain: M rounded to FP32 from FP64 sums, and L^-1 in
+ * FP64 by forward substitution.  The proposal uses d = L z and the drift M g (FP32 sums
+ * in increasing index order); the proposal density uses w = L^-1 r (FP64). */
+typedef struct {
+    int k;
+    float L[256], M[256];
+    double Li[256];
+} metric_t;
+static void metric_derive(const gpo_cfg *cfg, int k, metric_t *m) {
+    m->k = k;
+    memcpy(m->L, cfg->mL, sizeof(m->L));
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++) {
+            double s = 0.0;
+            for (int q = 0; q <= (i < j ? i : j); q++) s = s + (double)m->L[i * 16 + q] * (double)m->L[j * 16 + q];
+            m->M[i * 16 + j] = (float)s;
+        }
+    for (int c = 0; c < k; c++)
+        for (int i = 0; i < k; i++) {
+            if (i < c) { m->Li[i * 16 + c] = 0.0; continue; }
+            double s = i == c ? 1.0 : 0.0;
+            for (int q = c; q < i; q++) s = s - (double)m->L[i * 16 + q] * m->Li[q * 16 + c];
+            m->Li[i * 16 + c] = s / (double)m->L[i * 16 + i];
+        }
+}
+static int metric_slot(const gpo_cfg *cfg, int k, int t) {
+    for (int i = 0; i < k; i++) if (cfg->mrows[i] == t) return i;
+    return -1;
+}
+static void metric_L(const metric_t *m, const float *z, float *d) {           /* d = L z */
+    for (int i = 0; i < m->k; i++) {
+        float acc = 0.0f;
+        for (int j = 0; j <= i; j++) acc = acc + m->L[i * 16 + j] * z[j];
+        d[i] = acc;
+    }
+}
+static void metric_M(const metric_t *m, const float *g, float *v) {           /* v = M g */
+    for (int i = 0; i < m->k; i++) {
+        float acc = 0.0f;
+        for (int j = 0; j < m->k; j++) acc = acc + m->M[i * 16 + j] * g[j];
+        v[i] = acc;
+    }
+}
+static double metric_logq(const metric_t *m, const double *r, double two_e2) {  /* -|L^-1 r|^2 / (2 e^2) */
+    double q = 0.0;
+    for (int i = 0; i < m->k; i++) {
+        double w = 0.0;
+        for (int j = 0; j <= i; j++) w = w + m->Li[i * 16 + j] * r[j];
+        q += -w * w / two_e2;
+    }
+    return q;
+}
+
 void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int steps,
                        uint8_t *ops, float *cs, int *lenp,
                        const float *x, const float *y, const float *sd, int n,
@@ -783,6 +837,19 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
             stt->n_const_prop++;
             memcpy(pcs, cs, sizeof(float) * (size_t)len);
             double a_extra = 0.0, pgrad[256];
+            /* metric rows: their z, current constants and (MALA) gradients.  A chain whose
+             * metric rows are not all constant tokens moves every constant with the identity. */
+            int K = cfg->mk;
+            for (int i = 0; i < cfg->mk; i++)
+                if (cfg->mrows[i] >= len || ops[cfg->mrows[i]] != GPO_CONST) K = 0;
+            metric_t mt;
+            metric_derive(cfg, K, &mt);
+            float mz[16], md[16], mc[16];
+            for (int i = 0; i < K; i++) {
+                mz[i] = gpo_normal_icdf(gpo_u01(cprop_word(cfg->seed, chain, step, cfg->mrows[i])));
+                mc[i] = cs[cfg->mrows[i]];
+            }
+            metric_L(&mt, mz, md);
             if (cfg->proposal == 1) {
                 if (!grad_ok) {
                     if (gpo_tape_grad(ops, cs, len, x, y, sd, n, grad, NULL) != 0) {
@@ -791,30 +858,53 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
                     grad_ok = 1;
                 }
                 const float e = cfg->eps, h = 0.5f * e * e;
+                const double two_e2 = 2.0 * (double)e * (double)e;
                 double fq = 0.0;
                 for (int t = 0; t < len; t++) {
-                    if (ops[t] != GPO_CONST) continue;
+                    if (ops[t] != GPO_CONST || metric_slot(cfg, K, t) >= 0) continue;
                     float z = gpo_normal_icdf(gpo_u01(cprop_word(cfg->seed, chain, step, t)));
                     float mean = cs[t] + h * (float)grad[t];
                     pcs[t] = mean + e * z;
                     double d = (double)pcs[t] - (double)mean;
-                    fq += -d * d / (2.0 * (double)e * (double)e);
+                    fq += -d * d / two_e2;
+                }
+                float mg[16], mv[16], mmean[16];
+                double r[16];
+                if (K > 0) {
+                    for (int i = 0; i < K; i++) mg[i] = (float)grad[cfg->mrows[i]];
+                    metric_M(&mt, mg, mv);
+                    for (int i = 0; i < K; i++) {
+                        mmean[i] = mc[i] + h * mv[i];
+                        pcs[cfg->mrows[i]] = mmean[i] + e * md[i];
+                        r[i] = (double)pcs[cfg->mrows[i]] - (double)mmean[i];
+                    }
+                    fq += metric_logq(&mt, r, two_e2);
                 }
                 int pg_ok = gpo_tape_grad(ops, pcs, len, x, y, sd, n, pgrad, NULL) == 0;
                 double bq = 0.0;
                 for (int t = 0; t < len; t++) {
-                    if (ops[t] != GPO_CONST) continue;
+                    if (ops[t] != GPO_CONST || metric_slot(cfg, K, t) >= 0) continue;
                     float mean = pcs[t] + h * (float)(pg_ok ? pgrad[t] : 0.0);
                     double d = (double)cs[t] - (double)mean;
-                    bq += -d * d / (2.0 * (double)e * (double)e);
+                    bq += -d * d / two_e2;
+                }
+                if (K > 0) {
+                    for (int i = 0; i < K; i++) mg[i] = (float)(pg_ok ? pgrad[cfg->mrows[i]] : 0.0);
+                    metric_M(&mt, mg, mv);
+                    for (int i = 0; i < K; i++) {
+                        float mb = pcs[cfg->mrows[i]] + h * mv[i];
+                        r[i] = (double)mc[i] - (double)mb;
+                    }
+                    bq += metric_logq(&mt, r, two_e2);
                 }
                 a_extra = pg_ok ? bq - fq : -INFINITY;
             } else {
                 for (int t = 0; t < len; t++) {
-                    if (ops[t] != GPO_CONST) continue;
+                    if (ops[t] != GPO_CONST || metric_slot(cfg, K, t) >= 0) continue;
                     float z = gpo_normal_icdf(gpo_u01(cprop_word(cfg->seed, chain, step, t)));
                     pcs[t] = cs[t] + cfg->sigma * z;
                 }
+                for (int i = 0; i < K; i++) pcs[cfg->mrows[i]] = mc[i] + cfg->sigma * md[i];
             }
             double pprior = gpo_tape_prior(ops, pcs, len, lns, 0);
             double pll = gpo_tape_loglik(ops, pcs, len, x, y, sd, n, 2);

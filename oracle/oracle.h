@@ -113,6 +113,11 @@ typedef struct {
     int struct_every;   /* structure move when step % struct_every == 0 */
     int fixed_structure;/* 1: never propose structure */
     uint64_t seed;
+    /* proposal metric of the constant moves: tokens mrows[i] (i < mk <= 16) move with the
+     * lower-triangular factor mL (row-major 16 x 16); other constants use the identity */
+    int mk;
+    int mrows[16];
+    float mL[256];
 } gpo_cfg;
 
 /* generation table: 64 buckets of the 6 high bits of a Philox word */

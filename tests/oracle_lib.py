@@ -37,7 +37,8 @@ class GpoStream(C.Structure):
 class GpoCfg(C.Structure):
     _fields_ = [("cap", C.c_int), ("dmax", C.c_int), ("nsym", C.c_int), ("proposal", C.c_int),
                 ("sigma", C.c_float), ("eps", C.c_float), ("struct_every", C.c_int),
-                ("fixed_structure", C.c_int), ("seed", C.c_uint64)]
+                ("fixed_structure", C.c_int), ("seed", C.c_uint64), ("mk", C.c_int), ("mrows", C.c_int * 16),
+                ("mL", C.c_float * 256)]
 
 
 class GpoRefCfg(C.Structure):
@@ -228,8 +229,17 @@ def ref_decode(hop, hc):
 
 
 def make_cfg(cap=32, dmax=16, nsym=10, proposal=0, sigma=0.1, eps=0.05, struct_every=5,
-             fixed_structure=0, seed=0):
-    return GpoCfg(cap, dmax, nsym, proposal, sigma, eps, struct_every, fixed_structure, seed)
+             fixed_structure=0, seed=0, metric=None):
+    """metric: None or (rows, L) with L a K x K lower-triangular factor (K <= 16)."""
+    cfg = GpoCfg(cap, dmax, nsym, proposal, sigma, eps, struct_every, fixed_structure, seed)
+    if metric is not None:
+        rows, L = metric
+        cfg.mk = len(rows)
+        for i, r in enumerate(rows):
+            cfg.mrows[i] = int(r)
+            for j in range(i + 1):
+                cfg.mL[i * 16 + j] = float(np.float32(L[i][j]))
+    return cfg
 
 
 def cfg_init(cfg, nchains, chain0=0):

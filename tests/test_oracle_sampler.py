@@ -99,6 +99,89 @@ def test_constant_posterior_rw_and_mala(proposal, scale):
     assert 0.2 < acc < 0.95
 
 
+def line_toy():
+    """c0*x + c1 on x in [5, 7]: slope and intercept correlated at about -0.99."""
+    rng = np.random.default_rng(21)
+    x = rng.uniform(5.0, 7.0, 30).astype(np.float32)
+    y = (0.8 * x - 1.5 + rng.normal(0.0, 0.2, 30)).astype(np.float32)
+    sd = np.full(30, 0.2, dtype=np.float32)
+    ops, _ = O.tape_arrays(["X", ("CONST", 0.0), "MUL", ("CONST", 0.0), "ADD"])
+    return x, y, sd, ops
+
+
+def line_posterior(x, y, sd):
+    """Mean and covariance of (c0, c1) by quadrature on a grid (Gaussian likelihood x
+    Cauchy priors), and the Laplace covariance from the likelihood alone."""
+    xd, yd, s = (v.astype(np.float64) for v in (x, y, sd))
+    A = np.stack([xd, np.ones_like(xd)], 1) / s[:, None]
+    cov0 = np.linalg.inv(A.T @ A)
+    mu0 = cov0 @ (A.T @ (yd / s))
+    sds = np.sqrt(np.diag(cov0))
+    g0 = np.linspace(mu0[0] - 7 * sds[0], mu0[0] + 7 * sds[0], 701)
+    g1 = np.linspace(mu0[1] - 7 * sds[1], mu0[1] + 7 * sds[1], 701)
+    C0, C1 = np.meshgrid(g0, g1, indexing="ij")
+    r = C0[..., None] * xd + C1[..., None] - yd
+    lp = -np.sum(r * r / (2 * s * s), axis=-1) - np.log1p(C0 ** 2) - np.log1p(C1 ** 2)
+    w = np.exp(lp - lp.max())
+    w /= w.sum()
+    m = np.array([(w * C0).sum(), (w * C1).sum()])
+    cov = np.array([[(w * (C0 - m[0]) ** 2).sum(), (w * (C0 - m[0]) * (C1 - m[1])).sum()],
+                    [0.0, (w * (C1 - m[1]) ** 2).sum()]])
+    cov[1, 0] = cov[0, 1]
+    return m, cov, cov0
+
+
+@pytest.mark.parametrize("proposal,step", [(0, 1.0), (1, 1.2)])
+def test_metric_proposals_sample_the_posterior(proposal, step):
+    """Random walk and MALA with the metric L L^T = the Laplace covariance: posterior
+    mean and covariance of a strongly correlated pair by quadrature."""
+    x, y, sd, ops = line_toy()
+    m, cov, cov0 = line_posterior(x, y, sd)
+    L = np.linalg.cholesky(cov0)
+    nch = 4000
+    cfg = O.make_cfg(cap=8, seed=2, proposal=proposal, sigma=step, eps=step, fixed_structure=1,
+                     metric=([1, 3], L))
+    O_ = np.zeros((8, nch), dtype=np.uint8)
+    O_[:5] = ops[:, None]
+    V = np.zeros((8, nch), dtype=np.float32)
+    V[1] = m[0]
+    V[3] = m[1]
+    ln = np.full(nch, 5, dtype=np.int32)
+    draws = []
+    for b in range(30):
+        _, _, cnt = O.cfg_run_block(cfg, O_, V, ln, x, y, sd, b * 20, 20)
+        if b >= 5:
+            draws.append(V[[1, 3]].astype(np.float64).copy())
+    D = np.array(draws)                                  # [draws, 2, chains]
+    cm = D.mean(axis=0)                                  # chain means [2, chains]
+    se = cm.std(axis=1, ddof=1) / np.sqrt(nch)
+    est = D.transpose(1, 0, 2).reshape(2, -1)
+    assert np.all(np.abs(cm.mean(axis=1) - m) < 4 * se), (cm.mean(axis=1), m, se)
+    ecov = np.cov(est)
+    assert np.allclose(np.sqrt(np.diag(ecov)), np.sqrt(np.diag(cov)), rtol=0.05)
+    assert abs(ecov[0, 1] / np.sqrt(ecov[0, 0] * ecov[1, 1]) - cov[0, 1] / np.sqrt(cov[0, 0] * cov[1, 1])) < 0.01
+    assert 0.2 < cnt[3] / cnt[2] < 0.95
+
+
+@pytest.mark.parametrize("proposal", [0, 1])
+def test_unit_metric_is_the_identity_proposal(proposal):
+    """A one-row metric with L = [[1]] proposes exactly what the identity metric does."""
+    x, y, sd, ops = line_toy()
+    ops1, _ = O.tape_arrays(["X", ("CONST", 0.0), "MUL"])
+    runs = []
+    for metric in (None, ([1], [[1.0]])):
+        cfg = O.make_cfg(cap=4, seed=4, proposal=proposal, sigma=0.01, eps=0.005, fixed_structure=1,
+                         metric=metric)
+        Ops = np.zeros((4, 64), dtype=np.uint8)
+        Ops[:3] = ops1[:, None]
+        V = np.zeros((4, 64), dtype=np.float32)
+        V[1] = 0.5
+        ln = np.full(64, 3, dtype=np.int32)
+        O.cfg_run_block(cfg, Ops, V, ln, x, y, sd, 0, 200)
+        runs.append(V.copy())
+    assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))
+
+
 def test_adjoint_matches_central_differences():
     from gpuropolis_amd.workloads import dataset_D, rational_tape
     (x, y, sd), b = dataset_D(0)
