@@ -27,7 +27,19 @@ enum { GPR_P_STRUCT = 1, GPR_P_CPROP = 2, GPR_P_ACCEPT = 3, GPR_P_INIT = 4, GPR_
 // Salmon et al., SC'11: two 32x32->64 multiplies per round, ten rounds, Weyl key.
 struct gpr_u4 { uint32_t x, y, z, w; };
 
+// The key is the same at every call site of a kernel, so the compiler hoists its ten-
+// round schedule out of every loop and keeps twenty scalar registers live (spilled to
+// vector lanes in the large kernels).  On the device the key passes through an empty
+// asm, so each call recomputes the schedule with ten scalar adds instead.
+__host__ inline uint32_t gpr_keep(uint32_t k) { return k; }
+__device__ __forceinline__ uint32_t gpr_keep(uint32_t k) {
+    __asm__ volatile("" : "+s"(k));
+    return k;
+}
+
 GPR_HD gpr_u4 gpr_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+    k0 = gpr_keep(k0);
+    k1 = gpr_keep(k1);
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         uint64_t p0 = (uint64_t)0xD2511F53u * c0;
@@ -80,19 +92,19 @@ GPR_HD uint32_t gpr_f_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
 GPR_HD float gpr_u01(uint32_t r) { return ((float)(r >> 9) + 0.5f) * 0x1p-23f; }
 
 // natural log: m*2^e, m in (sqrt(1/2), sqrt(2)], log m = 2 atanh((m-1)/(m+1)); fused
-// multiply-adds are written as fmaf so host and device round identically
+// multiply-adds are written as fmaf so host and device round identically.  No branches:
+// the main path runs for every input and special inputs select their result at the end
+// (on the GPU a data-dependent branch costs more than the whole polynomial).
 GPR_HD float gpr_logf(float x) {
 #pragma clang fp contract(off)
-    if (x != x) return x;
-    if (x < 0.0f) return __builtin_nanf("");
-    if (x == 0.0f) return -__builtin_inff();
-    if (x == __builtin_inff()) return x;
-    uint32_t b = gpr_f_bits(x);
-    int e = (int)((b >> 23) & 0xffu);
-    if (e == 0) { x = x * 0x1p23f; b = gpr_f_bits(x); e = (int)((b >> 23) & 0xffu) - 23; }
-    e -= 127;
+    const bool sub = x > 0.0f && x < 0x1p-126f;
+    const float xs = sub ? x * 0x1p23f : x;
+    const uint32_t b = gpr_f_bits(xs);
+    int e = (int)((b >> 23) & 0xffu) - (sub ? 23 : 0) - 127;
     float m = gpr_bits_f((b & 0x007fffffu) | 0x3f800000u);
-    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e += 1; }
+    const bool big = m > 0x1.6a09e6p+0f;
+    m = big ? m * 0.5f : m;
+    e += big ? 1 : 0;
     const float s = (m - 1.0f) / (m + 1.0f);
     const float s2 = s * s;
     float p = __builtin_fmaf(s2, 0x1.745d18p-4f, 0x1.c71c72p-4f);
@@ -102,7 +114,11 @@ GPR_HD float gpr_logf(float x) {
     p = p * s2;
     const float lm = __builtin_fmaf(s, p, s) * 2.0f;
     const float ef = (float)e;
-    return __builtin_fmaf(ef, 0x1.62e4p-1f, __builtin_fmaf(ef, 0x1.7f7d1cp-20f, lm));
+    float r = __builtin_fmaf(ef, 0x1.62e4p-1f, __builtin_fmaf(ef, 0x1.7f7d1cp-20f, lm));
+    r = x == __builtin_inff() ? x : r;
+    r = x == 0.0f ? -__builtin_inff() : r;
+    r = x < 0.0f ? __builtin_nanf("") : r;
+    return x != x ? x : r;
 }
 
 GPR_HD float gpr_pow2i(int k) { return gpr_bits_f((uint32_t)(k + 127) << 23); }
@@ -152,36 +168,33 @@ GPR_HD float gpr_sin2pi(float v) {
 }
 
 // standard normal quantile sqrt(2) erfinv(2u-1), erfinv by M. Giles (GPU Computing
-// Gems, 2011) single-precision polynomials; odd in 2u-1.
+// Gems, 2011) single-precision polynomials; odd in 2u-1.  Both branches are evaluated
+// and selected (w < 5 for all but the outer 0.7 % of u), so lanes never diverge.
 GPR_HD float gpr_normal(float u) {
 #pragma clang fp contract(off)
     const float x = __builtin_fmaf(2.0f, u, -1.0f);
-    float w = -gpr_logf((1.0f - x) * (1.0f + x));
-    float p;
-    if (w < 5.0f) {
-        w = w - 2.5f;
-        p = 2.81022636e-08f;
-        p = __builtin_fmaf(p, w, 3.43273939e-07f);
-        p = __builtin_fmaf(p, w, -3.5233877e-06f);
-        p = __builtin_fmaf(p, w, -4.39150654e-06f);
-        p = __builtin_fmaf(p, w, 0.00021858087f);
-        p = __builtin_fmaf(p, w, -0.00125372503f);
-        p = __builtin_fmaf(p, w, -0.00417768164f);
-        p = __builtin_fmaf(p, w, 0.246640727f);
-        p = __builtin_fmaf(p, w, 1.50140941f);
-    } else {
-        w = __builtin_sqrtf(w) - 3.0f;
-        p = -0.000200214257f;
-        p = __builtin_fmaf(p, w, 0.000100950558f);
-        p = __builtin_fmaf(p, w, 0.00134934322f);
-        p = __builtin_fmaf(p, w, -0.00367342844f);
-        p = __builtin_fmaf(p, w, 0.00573950773f);
-        p = __builtin_fmaf(p, w, -0.0076224613f);
-        p = __builtin_fmaf(p, w, 0.00943887047f);
-        p = __builtin_fmaf(p, w, 1.00167406f);
-        p = __builtin_fmaf(p, w, 2.83297682f);
-    }
-    return (p * x) * 1.41421356f;
+    const float w = -gpr_logf((1.0f - x) * (1.0f + x));
+    const float wc = w - 2.5f;
+    float p = 2.81022636e-08f;
+    p = __builtin_fmaf(p, wc, 3.43273939e-07f);
+    p = __builtin_fmaf(p, wc, -3.5233877e-06f);
+    p = __builtin_fmaf(p, wc, -4.39150654e-06f);
+    p = __builtin_fmaf(p, wc, 0.00021858087f);
+    p = __builtin_fmaf(p, wc, -0.00125372503f);
+    p = __builtin_fmaf(p, wc, -0.00417768164f);
+    p = __builtin_fmaf(p, wc, 0.246640727f);
+    p = __builtin_fmaf(p, wc, 1.50140941f);
+    const float wt = __builtin_sqrtf(w) - 3.0f;
+    float q = -0.000200214257f;
+    q = __builtin_fmaf(q, wt, 0.000100950558f);
+    q = __builtin_fmaf(q, wt, 0.00134934322f);
+    q = __builtin_fmaf(q, wt, -0.00367342844f);
+    q = __builtin_fmaf(q, wt, 0.00573950773f);
+    q = __builtin_fmaf(q, wt, -0.0076224613f);
+    q = __builtin_fmaf(q, wt, 0.00943887047f);
+    q = __builtin_fmaf(q, wt, 1.00167406f);
+    q = __builtin_fmaf(q, wt, 2.83297682f);
+    return ((w < 5.0f ? p : q) * x) * 1.41421356f;
 }
 
 // Words of the batched substreams: a Philox block at counter step>>2 serves four

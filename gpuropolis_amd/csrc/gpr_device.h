@@ -164,7 +164,11 @@ struct VStack {
     } while (0)
 
 // Interpret the tape at KC points per lane.  op is wave-uniform: every branch below is
-// a scalar branch and only the selected operation executes.
+// a scalar branch and only the selected operation executes.  Taken branches cost far
+// more than VALU work here, so the two common token classes take one branch each and
+// no nested switch: leaves (x, constant, one) select their value, and +, -, * compute
+// a + s*b (s = +-1, exact, so one rounding like a +- b) or a*b and select.  Every other
+// op goes through the full switch.
 template <int KC, bool HI>
 __device__ __forceinline__ void wave_eval(const WaveProg &p, const float (&xs)[KC], float (&T)[KC]) {
     VStack<KC> S;
@@ -173,54 +177,195 @@ __device__ __forceinline__ void wave_eval(const WaveProg &p, const float (&xs)[K
     for (int k = 0; k < KC; k++) T[k] = 0.0f;
     for (int t = 0; t < p.len; t++) {
         const int l = t & 63;
-        const int op = (HI && t >= 64) ? rl_i(p.o1, l) : rl_i(p.o0, l);
+        const bool h = HI && t >= 64;
+        const int op = h ? rl_i(p.o1, l) : rl_i(p.o0, l);
         if (op <= GPR_OP_ONE) {
             // the write index is bounded for malformed tapes; valid ones (depth <= 16) never hit it
             const int wi = min(sp, GPR_MAX_DEPTH - 1);
 #pragma unroll
             for (int k = 0; k < KC; k++) S.put(k, wi, T[k]);
             sp = min(sp + 1, GPR_MAX_DEPTH);
-            if (op == GPR_OP_X) {
-                GPR_KLOOP(xs[k]);
-            } else {
-                const float c = op == GPR_OP_CONST ? ((HI && t >= 64) ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 1.0f;
-                GPR_KLOOP(c);
+            const float cv = h ? rl_f(p.c1, l) : rl_f(p.c0, l);
+            const float sv = op == GPR_OP_CONST ? cv : 1.0f;
+            const bool isx = op == GPR_OP_X;
+            GPR_KLOOP(isx ? xs[k] : sv);
+        } else if (op <= GPR_OP_MUL) {
+            sp = max(sp - 1, 0);
+            const bool mul = op == GPR_OP_MUL;
+            const float sgn = op == GPR_OP_SUB ? -1.0f : 1.0f;
+#pragma unroll
+            for (int k = 0; k < KC; k++) {
+                const float a = S.get(k, sp);
+                const float pm = __builtin_fmaf(sgn, T[k], a);
+                const float pr = a * T[k];
+                T[k] = mul ? pr : pm;
             }
-            continue;
-        }
-        if (op <= GPR_OP_RPOW) {
+        } else if (op <= GPR_OP_RPOW) {
             sp = max(sp - 1, 0);
             switch (op) {
-            case GPR_OP_ADD: GPR_KLOOP(S.get(k, sp) + T[k]); break;
-            case GPR_OP_SUB: GPR_KLOOP(S.get(k, sp) - T[k]); break;
-            case GPR_OP_MUL: GPR_KLOOP(S.get(k, sp) * T[k]); break;
             case GPR_OP_DIV: GPR_KLOOP(S.get(k, sp) / T[k]); break;
             case GPR_OP_RSUB: GPR_KLOOP(T[k] - S.get(k, sp)); break;
             case GPR_OP_RDIV: GPR_KLOOP(T[k] / S.get(k, sp)); break;
             case GPR_OP_POW: GPR_KLOOP(powf(S.get(k, sp), T[k])); break;
             default: GPR_KLOOP(powf(T[k], S.get(k, sp))); break;
             }
-            continue;
+        } else {
+            const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 0.0f;
+            switch (op) {
+            case GPR_OP_NEG: GPR_KLOOP(-T[k]); break;
+            case GPR_OP_EXP: GPR_KLOOP(expf(T[k])); break;
+            case GPR_OP_LOG: GPR_KLOOP(logf(T[k])); break;
+            case GPR_OP_INV: GPR_KLOOP(1.0f / T[k]); break;
+            case GPR_OP_SQRT: GPR_KLOOP(sqrtf(T[k])); break;
+            case GPR_OP_SIN: GPR_KLOOP(sinf(T[k])); break;
+            case GPR_OP_ASIN: GPR_KLOOP(asinf(T[k])); break;
+            case GPR_OP_COS: GPR_KLOOP(cosf(T[k])); break;
+            case GPR_OP_ACOS: GPR_KLOOP(acosf(T[k])); break;
+            case GPR_OP_TAN: GPR_KLOOP(tanf(T[k])); break;
+            case GPR_OP_ATAN: GPR_KLOOP(atanf(T[k])); break;
+            case GPR_OP_ABS: GPR_KLOOP(fabsf(T[k])); break;
+            case GPR_OP_CADD: GPR_KLOOP(T[k] + c); break;
+            case GPR_OP_CMUL: GPR_KLOOP(c * T[k]); break;
+            case GPR_OP_CPOW: GPR_KLOOP(powf(T[k], c)); break;
+            case GPR_OP_CRPOW: GPR_KLOOP(powf(c, T[k])); break;
+            default: GPR_KLOOP(__builtin_nanf("")); break;
+            }
         }
-        const float c = op >= GPR_OP_CADD ? ((HI && t >= 64) ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 0.0f;
-        switch (op) {
-        case GPR_OP_NEG: GPR_KLOOP(-T[k]); break;
-        case GPR_OP_EXP: GPR_KLOOP(expf(T[k])); break;
-        case GPR_OP_LOG: GPR_KLOOP(logf(T[k])); break;
-        case GPR_OP_INV: GPR_KLOOP(1.0f / T[k]); break;
-        case GPR_OP_SQRT: GPR_KLOOP(sqrtf(T[k])); break;
-        case GPR_OP_SIN: GPR_KLOOP(sinf(T[k])); break;
-        case GPR_OP_ASIN: GPR_KLOOP(asinf(T[k])); break;
-        case GPR_OP_COS: GPR_KLOOP(cosf(T[k])); break;
-        case GPR_OP_ACOS: GPR_KLOOP(acosf(T[k])); break;
-        case GPR_OP_TAN: GPR_KLOOP(tanf(T[k])); break;
-        case GPR_OP_ATAN: GPR_KLOOP(atanf(T[k])); break;
-        case GPR_OP_ABS: GPR_KLOOP(fabsf(T[k])); break;
-        case GPR_OP_CADD: GPR_KLOOP(T[k] + c); break;
-        case GPR_OP_CMUL: GPR_KLOOP(c * T[k]); break;
-        case GPR_OP_CPOW: GPR_KLOOP(powf(T[k], c)); break;
-        case GPR_OP_CRPOW: GPR_KLOOP(powf(c, T[k])); break;
-        default: GPR_KLOOP(__builtin_nanf("")); break;
+    }
+}
+
+// The same interpreter with the value stack in LDS: the top two entries stay in VGPRs
+// (T, U) and the rest live in this wave's slab ls[level][lane][KC] (GPR_MAX_DEPTH
+// levels).  A push spills U to the slab, a binary op refills U from it; both are one
+// 16-byte LDS access per lane at KC = 4.  Saves the 16*KC VGPRs of the indexed stack
+// (occupancy) and the indexing-mode switches.  Same operations on the same operands
+// as wave_eval, so the values are bit-identical.
+template <int KC>
+__device__ __forceinline__ void ls_store(float *q, const float (&v)[KC]) {
+    if (KC == 4) *(float4 *)q = make_float4(v[0], v[1], v[2], v[3]);
+    else if (KC == 2) *(float2 *)q = make_float2(v[0], v[KC > 1 ? 1 : 0]);
+    else q[0] = v[0];
+}
+template <int KC>
+__device__ __forceinline__ void ls_load(const float *q, float (&v)[KC]) {
+    if (KC == 4) { const float4 f = *(const float4 *)q; v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w; }
+    else if (KC == 2) { const float2 f = *(const float2 *)q; v[0] = f.x; v[KC > 1 ? 1 : 0] = f.y; }
+    else v[0] = q[0];
+}
+
+// Token classes as wave-uniform bit masks (bit t of the low word, bit t-64 of the high
+// word): the common classes dispatch on a scalar bit test, with no vector-to-scalar
+// read of the op on their path.
+// (x and constant leaves; +, -, *; every other op takes the generic path)
+struct OpMasks {
+    uint64_t leaf[2], x[2], cheap[2], mul[2], sub[2];
+};
+template <bool HI>
+__device__ __forceinline__ OpMasks op_masks(const WaveProg &p) {
+    const int l = lane_id();
+    OpMasks m;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const bool v = (h == 0 || HI) && l + 64 * h < p.len;
+        const int o = h ? p.o1 : p.o0;
+        m.leaf[h] = ballot(v && o <= GPR_OP_CONST);
+        m.x[h] = ballot(v && o == GPR_OP_X);
+        m.cheap[h] = ballot(v && o >= GPR_OP_ADD && o <= GPR_OP_MUL);
+        m.mul[h] = ballot(v && o == GPR_OP_MUL);
+        m.sub[h] = ballot(v && o == GPR_OP_SUB);
+    }
+    return m;
+}
+
+template <int KC, bool HI>
+__device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs)[KC], float (&T)[KC], float *ls,
+                                             const OpMasks &m) {
+    float U[KC];
+    float *lane = ls + lane_id() * KC;
+    int sp = 0;   // entries on the stack, T and U included
+#pragma unroll
+    for (int k = 0; k < KC; k++) { T[k] = 0.0f; U[k] = 0.0f; }
+    for (int t = 0; t < p.len; t++) {
+        const int l = t & 63;
+        const int h = (HI && t >= 64) ? 1 : 0;
+        const uint64_t bit = 1ull << l;
+        if (m.leaf[h] & bit) {
+            // level sp-2 receives U when U holds an entry (bounded for malformed tapes)
+            if (sp >= 2) ls_store<KC>(lane + min(sp - 2, GPR_MAX_DEPTH - 1) * 64 * KC, U);
+#pragma unroll
+            for (int k = 0; k < KC; k++) U[k] = T[k];
+            sp = min(sp + 1, GPR_MAX_DEPTH + 1);
+            const float cv = h ? rl_f(p.c1, l) : rl_f(p.c0, l);
+            const bool isx = (m.x[h] & bit) != 0;
+            GPR_KLOOP(isx ? xs[k] : cv);
+        } else if (m.cheap[h] & bit) {
+            const bool mul = (m.mul[h] & bit) != 0;
+            const float sgn = (m.sub[h] & bit) ? -1.0f : 1.0f;
+#pragma unroll
+            for (int k = 0; k < KC; k++) {
+                const float pm = __builtin_fmaf(sgn, T[k], U[k]);
+                const float pr = U[k] * T[k];
+                T[k] = mul ? pr : pm;
+            }
+            if (sp >= 3) ls_load<KC>(lane + min(sp - 3, GPR_MAX_DEPTH - 1) * 64 * KC, U);
+            sp = max(sp - 1, 0);
+            continue;
+        } else {
+        const int op = h ? rl_i(p.o1, l) : rl_i(p.o0, l);
+        if (op <= GPR_OP_ONE) {
+            // level sp-2 receives U when U holds an entry (bounded for malformed tapes)
+            if (sp >= 2) ls_store<KC>(lane + min(sp - 2, GPR_MAX_DEPTH - 1) * 64 * KC, U);
+#pragma unroll
+            for (int k = 0; k < KC; k++) U[k] = T[k];
+            sp = min(sp + 1, GPR_MAX_DEPTH + 1);
+            const float cv = h ? rl_f(p.c1, l) : rl_f(p.c0, l);
+            const float sv = op == GPR_OP_CONST ? cv : 1.0f;
+            const bool isx = op == GPR_OP_X;
+            GPR_KLOOP(isx ? xs[k] : sv);
+        } else if (op <= GPR_OP_RPOW) {
+            if (op <= GPR_OP_MUL) {
+                const bool mul = op == GPR_OP_MUL;
+                const float sgn = op == GPR_OP_SUB ? -1.0f : 1.0f;
+#pragma unroll
+                for (int k = 0; k < KC; k++) {
+                    const float pm = __builtin_fmaf(sgn, T[k], U[k]);
+                    const float pr = U[k] * T[k];
+                    T[k] = mul ? pr : pm;
+                }
+            } else {
+                switch (op) {
+                case GPR_OP_DIV: GPR_KLOOP(U[k] / T[k]); break;
+                case GPR_OP_RSUB: GPR_KLOOP(T[k] - U[k]); break;
+                case GPR_OP_RDIV: GPR_KLOOP(T[k] / U[k]); break;
+                case GPR_OP_POW: GPR_KLOOP(powf(U[k], T[k])); break;
+                default: GPR_KLOOP(powf(T[k], U[k])); break;
+                }
+            }
+            // the new second entry is level sp-3, when there is one
+            if (sp >= 3) ls_load<KC>(lane + min(sp - 3, GPR_MAX_DEPTH - 1) * 64 * KC, U);
+            sp = max(sp - 1, 0);
+        } else {
+            const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 0.0f;
+            switch (op) {
+            case GPR_OP_NEG: GPR_KLOOP(-T[k]); break;
+            case GPR_OP_EXP: GPR_KLOOP(expf(T[k])); break;
+            case GPR_OP_LOG: GPR_KLOOP(logf(T[k])); break;
+            case GPR_OP_INV: GPR_KLOOP(1.0f / T[k]); break;
+            case GPR_OP_SQRT: GPR_KLOOP(sqrtf(T[k])); break;
+            case GPR_OP_SIN: GPR_KLOOP(sinf(T[k])); break;
+            case GPR_OP_ASIN: GPR_KLOOP(asinf(T[k])); break;
+            case GPR_OP_COS: GPR_KLOOP(cosf(T[k])); break;
+            case GPR_OP_ACOS: GPR_KLOOP(acosf(T[k])); break;
+            case GPR_OP_TAN: GPR_KLOOP(tanf(T[k])); break;
+            case GPR_OP_ATAN: GPR_KLOOP(atanf(T[k])); break;
+            case GPR_OP_ABS: GPR_KLOOP(fabsf(T[k])); break;
+            case GPR_OP_CADD: GPR_KLOOP(T[k] + c); break;
+            case GPR_OP_CMUL: GPR_KLOOP(c * T[k]); break;
+            case GPR_OP_CPOW: GPR_KLOOP(powf(T[k], c)); break;
+            case GPR_OP_CRPOW: GPR_KLOOP(powf(c, T[k])); break;
+            default: GPR_KLOOP(__builtin_nanf("")); break;
+            }
+        }
         }
     }
 }
@@ -244,16 +389,20 @@ struct Chunk {
 
 // Gaussian log-likelihood of the wave's program: lconst - sum_i (f_i - y_i)^2 w_i.
 // Per-lane Kahan sums, wave tree reduction; -inf if any term is not finite.
-template <int KC, bool HI>
+// LS: the value stack in LDS, ls = this wave's slab of GPR_MAX_DEPTH * 64 * KC floats,
+// pm = the program's token-class masks; else the VGPR stack.
+template <int KC, bool HI, bool LS = false>
 __device__ double wave_loglik(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
-                              const float *dw, const Chunk<KC> &c0, int nchunks) {
+                              const float *dw, const Chunk<KC> &c0, int nchunks, float *ls = nullptr,
+                              const OpMasks *pm = nullptr) {
     float acc = 0.0f, comp = 0.0f;
     const int l = lane_id();
     for (int ch = 0; ch < nchunks; ch++) {
         Chunk<KC> c;
         if (nchunks == 1) c = c0; else c.load(dx, dy, dw, d.n, ch * 64 * KC);
         float T[KC];
-        wave_eval<KC, HI>(p, c.x, T);
+        if (LS) wave_eval_ls<KC, HI>(p, c.x, T, ls, *pm);
+        else wave_eval<KC, HI>(p, c.x, T);
 #pragma unroll
         for (int k = 0; k < KC; k++) {
             const float r = T[k] - c.y[k];

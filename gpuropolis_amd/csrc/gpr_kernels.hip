@@ -29,9 +29,13 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
     const int N = a.nchains;
-    float *lv = MALA ? gpr_lds_slab + (size_t)wave * a.dmax_cap * 64 : nullptr;
+    // dynamic LDS: the value stacks of the wpg waves, then (MALA) their token-value slabs
+    float *ls = gpr_lds_slab + (size_t)wave * (GPR_MAX_DEPTH * 64 * KC);
+    float *lv = MALA ? gpr_lds_slab + (size_t)a.wpg * (GPR_MAX_DEPTH * 64 * KC) + (size_t)wave * a.dmax_cap * 64
+                     : nullptr;
     const uint32_t step0 = a.clock ? (uint32_t)uni((int)a.clock[0]) : a.step0;
-    unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0, n_evals = 0, n_steps = 0, n_flops = 0;
+    // launch counters, lane k holds counter k (folded in once per tile)
+    unsigned long long cnt = 0;
     for (;;) {
         __shared__ int s_tile;
         if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1);
@@ -80,17 +84,26 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         // MALA: gradient of the current state (lane-distributed like the constants)
         float cg0 = 0.0f, cg1 = 0.0f, pg0 = 0.0f, pg1 = 0.0f;
         bool grad_ok = false;
+        // step % struct_every, carried as a counter (no integer division per step)
+        OpMasks cmask = op_masks<HI>(cur);
+        // per-tile counts: structure proposed/accepted, constant proposed/accepted, and over
+        // evaluated programs the sums of tokens, of (operator tokens + 4) and of constants
+        uint32_t c_sp = 0, c_sa = 0, c_cp = 0, c_ca = 0, c_len = 0, c_ops = 0, c_nc = 0;
+        const uint32_t se = (uint32_t)a.struct_every;
+        uint32_t srem = step0 % se;
         for (int i = -1; i < a.steps; i++) {
             const uint32_t step = step0 + (uint32_t)i;
+            const bool struct_slot = i >= 0 && srem == 0;
+            if (i >= 0) srem = srem + 1 == se ? 0 : srem + 1;
             const double post = prior + ll;
             WaveProg prop = cur;
             double extra = 0.0;
             bool have_pll = false;
             double pll_m = 0.0;
-            const bool structural = i >= 0 && !a.fixed_structure && (step % (uint32_t)a.struct_every) == 0;
+            const bool structural = struct_slot && !a.fixed_structure;
             if (i < 0) {
             } else if (structural) {
-                n_sprop++;
+                c_sp++;
                 gpr_rstream st;
                 st.init(a.seed, chain, step, GPR_P_STRUCT, 0);
                 const uint32_t r = st.next();
@@ -159,7 +172,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                 const bool k1 = HI && l + 64 < cur.len && cur.o1 == GPR_OP_CONST;
                 const uint64_t mk0 = ballot(k0), mk1 = HI ? ballot(k1) : 0;
                 if ((mk0 | mk1) == 0) continue;
-                n_cprop++;
+                c_cp++;
                 if ((step >> 2) != wblk) {
                     wblk = step >> 2;
                     wlo = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 0);
@@ -236,14 +249,17 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                 }
             }
             const double pprior = wave_prior<HI>(prop, a.lnsym);
-            const double pll = have_pll ? pll_m : wave_loglik<KC, HI>(prop, d, dx, dy, dw, c0, nchunks);
-            n_evals += (unsigned long long)prop.len * (unsigned long long)d.n;
+            // token classes: the current program's, or the structural proposal's
+            const OpMasks pmask = structural ? op_masks<HI>(prop) : cmask;
+            const double pll = have_pll ? pll_m : wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &pmask);
             {
                 const int nleaf = __popcll(ballot(l < prop.len && prop.o0 <= GPR_OP_ONE)) +
                                   (HI ? __popcll(ballot(l + 64 < prop.len && prop.o1 <= GPR_OP_ONE)) : 0);
                 const int nconst = __popcll(ballot(l < prop.len && gpr_has_const(prop.o0))) +
                                    (HI ? __popcll(ballot(l + 64 < prop.len && gpr_has_const(prop.o1))) : 0);
-                n_flops += (unsigned long long)(prop.len - nleaf + 4) * (unsigned long long)d.n + (unsigned long long)nconst;
+                c_len += (uint32_t)prop.len;
+                c_ops += (uint32_t)(prop.len - nleaf + 4);
+                c_nc += (uint32_t)nconst;
             }
             if (i < 0) { prior = pprior; ll = pll; continue; }
             const double ppost = pprior + pll;
@@ -251,11 +267,22 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                 cur = prop;
                 prior = pprior;
                 ll = pll;
-                if (structural) { n_sacc++; grad_ok = false; }
-                else { n_cacc++; cg0 = pg0; cg1 = pg1; }
+                if (structural) { c_sa++; grad_ok = false; cmask = pmask; }
+                else { c_ca++; cg0 = pg0; cg1 = pg1; }
             }
         }
-        n_steps += (unsigned long long)a.steps;
+        {
+            const unsigned long long n = (unsigned long long)d.n;
+            unsigned long long v = 0;
+            v = l == GPR_CNT_SPROP ? c_sp : v;
+            v = l == GPR_CNT_SACC ? c_sa : v;
+            v = l == GPR_CNT_CPROP ? c_cp : v;
+            v = l == GPR_CNT_CACC ? c_ca : v;
+            v = l == GPR_CNT_EVALS ? (unsigned long long)c_len * n : v;
+            v = l == GPR_CNT_STEPS ? (unsigned long long)a.steps : v;
+            v = l == GPR_CNT_FLOPS ? (unsigned long long)c_ops * n + c_nc : v;
+            cnt += v;
+        }
         // every row below the cap: tokens past len are zero
         if (l < cap) { a.ops[(size_t)l * N + chain] = (uint8_t)cur.o0; a.cs[(size_t)l * N + chain] = cur.c0; }
         if (HI && l + 64 < cap) {
@@ -264,22 +291,14 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         }
         if (l == 0) { a.len[chain] = cur.len; a.prior[chain] = prior; a.ll[chain] = ll; }
     }
-    if (l == 0 && a.counters) {
-        atomicAdd(&a.counters[GPR_CNT_SPROP], n_sprop);
-        atomicAdd(&a.counters[GPR_CNT_SACC], n_sacc);
-        atomicAdd(&a.counters[GPR_CNT_CPROP], n_cprop);
-        atomicAdd(&a.counters[GPR_CNT_CACC], n_cacc);
-        atomicAdd(&a.counters[GPR_CNT_EVALS], n_evals);
-        atomicAdd(&a.counters[GPR_CNT_STEPS], n_steps);
-        atomicAdd(&a.counters[GPR_CNT_FLOPS], n_flops);
-    }
+    if (l < GPR_NCNT && a.counters && cnt) atomicAdd(&a.counters[l], cnt);
 }
 
 template <int KC, bool HI, bool MALA>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
-    size_t sh = 0;
-    if (MALA) {
-        sh = sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg;
+    size_t sh = sizeof(float) * (size_t)a.wpg * GPR_MAX_DEPTH * 64 * KC;
+    if (MALA) sh += sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg;
+    if (sh > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         if (e != hipSuccess) return e;
