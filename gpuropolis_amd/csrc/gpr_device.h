@@ -289,6 +289,23 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
         const int l = t & 63;
         const int h = (HI && t >= 64) ? 1 : 0;
         const uint64_t bit = 1ull << l;
+        // a leaf followed by +, - or * within the same half: push and pop cancel, so
+        // T = T op leaf with the stack untouched (the same operation on the same operands)
+        if (m.leaf[h] & (m.cheap[h] >> 1) & bit) {
+            const float cv = h ? rl_f(p.c1, l) : rl_f(p.c0, l);
+            const bool isx = (m.x[h] & bit) != 0;
+            const bool mul = ((m.mul[h] >> 1) & bit) != 0;
+            const float sgn = ((m.sub[h] >> 1) & bit) ? -1.0f : 1.0f;
+#pragma unroll
+            for (int k = 0; k < KC; k++) {
+                const float b = isx ? xs[k] : cv;
+                const float pm = __builtin_fmaf(sgn, b, T[k]);
+                const float pr = T[k] * b;
+                T[k] = mul ? pr : pm;
+            }
+            t++;
+            continue;
+        }
         if (m.leaf[h] & bit) {
             // level sp-2 receives U when U holds an entry (bounded for malformed tapes)
             if (sp >= 2) ls_store<KC>(lane + min(sp - 2, GPR_MAX_DEPTH - 1) * 64 * KC, U);
