@@ -53,6 +53,8 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
 
         WaveProg cur;
         cur.len = uni(a.len[chain]);
+        const int len0 = cur.len;
+        bool changed = false;
         cur.o0 = l < cur.len ? a.ops[(size_t)l * N + chain] : 0;
         cur.c0 = l < cur.len ? a.cs[(size_t)l * N + chain] : 0.0f;
         cur.o1 = (HI && l + 64 < cur.len) ? a.ops[(size_t)(l + 64) * N + chain] : 0;
@@ -267,6 +269,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                 cur = prop;
                 prior = pprior;
                 ll = pll;
+                changed = true;
                 if (structural) { c_sa++; grad_ok = false; cmask = pmask; }
                 else { c_ca++; cg0 = pg0; cg1 = pg1; }
             }
@@ -283,11 +286,17 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
             v = l == GPR_CNT_FLOPS ? (unsigned long long)c_ops * n + c_nc : v;
             cnt += v;
         }
-        // every row below the cap: tokens past len are zero
-        if (l < cap) { a.ops[(size_t)l * N + chain] = (uint8_t)cur.o0; a.cs[(size_t)l * N + chain] = cur.c0; }
-        if (HI && l + 64 < cap) {
-            a.ops[(size_t)(l + 64) * N + chain] = (uint8_t)cur.o1;
-            a.cs[(size_t)(l + 64) * N + chain] = cur.c1;
+        // Write back only a changed program, and only its rows below the longer of the old
+        // and new lengths: the state holds zeros past a program's length (init, write-back),
+        // so rows past both lengths are already right.  One-byte column writes of a single
+        // chain cost a memory transaction each, so skipping them saves most of the traffic.
+        if (changed) {
+            const int wlen = max(len0, cur.len);
+            if (l < wlen) { a.ops[(size_t)l * N + chain] = (uint8_t)cur.o0; a.cs[(size_t)l * N + chain] = cur.c0; }
+            if (HI && l + 64 < wlen) {
+                a.ops[(size_t)(l + 64) * N + chain] = (uint8_t)cur.o1;
+                a.cs[(size_t)(l + 64) * N + chain] = cur.c1;
+            }
         }
         if (l == 0) { a.len[chain] = cur.len; a.prior[chain] = prior; a.ll[chain] = ll; }
     }
