@@ -508,129 +508,269 @@ __device__ __forceinline__ float gather_f(float v0, float v1, int src, bool hi) 
 }
 
 // ---------------------------------------------------------------- reverse-mode adjoint
-// One point per lane (KC = 1).  The forward sweep is the interpreter above and also
-// stores every token's value in this wave's LDS slab lv[t * 64 + lane]; the reverse
-// sweep walks the tape backwards with an adjoint stack: reverse postfix order visits a
-// node, then its right subtree, then its left subtree, so a binary node pushes its
-// left child's adjoint and carries the right child's.  A binary token t has its right
-// operand at t-1 and its left operand at the last u < t whose stack depth equals t's.
+// One point per lane (KC = 1), no value stack: the forward sweep stores the value of
+// every operator token in this wave's LDS slab lv[t * 64 + lane]; leaves (x, constant,
+// one) are not stored, their values come from the token, and a binary token finds its
+// left operand through a per-structure plan.  The reverse sweep walks the tape backwards
+// carrying the adjoint of the current token.  Reverse postfix order visits a node, then
+// its right subtree (ending at t-1), then its left subtree, so a binary token carries its
+// right operand's adjoint and leaves its left operand's in a mailbox: the slot of the
+// first token of its own subtree, always a leaf, which the left subtree's root reads
+// when the sweep arrives there (the next token after a leaf).  Pending mailboxes are
+// distinct slots (nested subtrees pending at once start at distinct tokens), and a
+// constant leaf's mailbox is read before the sweep reaches the leaf itself, whose slot
+// then receives the leaf's per-point gradient; a token with a constant operand (c+, c*,
+// ^c, c^) overwrites its value slot with its gradient once it is done.
+
+// Per-structure plan of the two sweeps: token-class masks (bit t of word 0, bit t-64 of
+// word 1); for every token t (lane t of the *0 words, lane t-64 of the *1 words) the left
+// operand lc of a binary token and the first token st of the subtree ending at t.
+struct GradPlan {
+    uint64_t leaf[2], x[2], cnst[2], cheap[2], mul[2], sub[2];
+    int lc0, lc1, st0, st1;
+};
+
 template <bool HI>
-__device__ __forceinline__ float wave_eval_store(const WaveProg &p, float x, float *lv) {
+__device__ __forceinline__ GradPlan grad_plan(const WaveProg &p) {
     const int l = lane_id();
-    VStack<1> S;
-    int sp = 0;
-    float T = 0.0f;
+    GradPlan g;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const bool v = (h == 0 || HI) && l + 64 * h < p.len;
+        const int o = h ? p.o1 : p.o0;
+        g.leaf[h] = ballot(v && o <= GPR_OP_ONE);
+        g.x[h] = ballot(v && o == GPR_OP_X);
+        g.cnst[h] = ballot(v && o == GPR_OP_CONST);
+        g.cheap[h] = ballot(v && o >= GPR_OP_ADD && o <= GPR_OP_MUL);
+        g.mul[h] = ballot(v && o == GPR_OP_MUL);
+        g.sub[h] = ballot(v && o == GPR_OP_SUB);
+    }
+    // replay the stack on token indices: lane k of idx / sdx holds the token whose value
+    // is at level k and the first token of its subtree (levels are bounded for malformed
+    // tapes; valid ones stay below GPR_MAX_DEPTH)
+    int idx = 0, sdx = 0, sp = 0;
+    g.lc0 = 0;
+    g.lc1 = 0;
+    g.st0 = 0;
+    g.st1 = 0;
     for (int t = 0; t < p.len; t++) {
         const int li = t & 63;
         const bool h = HI && t >= 64;
-        const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
-        if (op <= GPR_OP_ONE) {
-            S.put(0, min(sp, GPR_MAX_DEPTH - 1), T);
-            sp = min(sp + 1, GPR_MAX_DEPTH);
-            T = op == GPR_OP_X ? x : (op == GPR_OP_CONST ? (h ? rl_f(p.c1, li) : rl_f(p.c0, li)) : 1.0f);
-        } else if (op <= GPR_OP_RPOW) {
-            sp = max(sp - 1, 0);
-            T = op_bin(op, S.get(0, sp), T);
+        const int ar = gpr_arity(h ? rl_i(p.o1, li) : rl_i(p.o0, li));
+        int lev, st = t;
+        if (ar == 0) {
+            lev = min(sp, 63);
+            sp = min(sp + 1, 64);
         } else {
-            const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, li) : rl_f(p.c0, li)) : 0.0f;
-            T = op_un(op, T, c);
+            lev = max(sp - ar, 0);
+            st = rl_i(sdx, lev);
+            if (ar == 2) {
+                const int lc = rl_i(idx, lev);
+                if (h) g.lc1 = l == li ? lc : g.lc1; else g.lc0 = l == li ? lc : g.lc0;
+                sp = max(sp - 1, 1);
+            }
         }
-        lv[t * 64 + l] = T;
+        if (h) g.st1 = l == li ? st : g.st1; else g.st0 = l == li ? st : g.st0;
+        idx = l == lev ? t : idx;
+        sdx = l == lev ? st : sdx;
+    }
+    return g;
+}
+
+// value of leaf token u (x, its constant, or one) at this lane's point
+template <bool HI>
+__device__ __forceinline__ float leaf_val(const WaveProg &p, const GradPlan &g, int u, float x) {
+    const int li = u & 63;
+    const int h = (HI && u >= 64) ? 1 : 0;
+    const uint64_t bit = 1ull << li;
+    const float c = h ? rl_f(p.c1, li) : rl_f(p.c0, li);
+    return (g.x[h] & bit) ? x : ((g.cnst[h] & bit) ? c : 1.0f);
+}
+// value of token u: a leaf from the token, an operator from the slab (lvl = lv + lane)
+template <bool HI>
+__device__ __forceinline__ float tok_val(const WaveProg &p, const GradPlan &g, int u, float x, const float *lvl) {
+    const int h = (HI && u >= 64) ? 1 : 0;
+    const float s = lvl[u * 64];
+    return (g.leaf[h] & (1ull << (u & 63))) ? leaf_val<HI>(p, g, u, x) : s;
+}
+template <bool HI>
+__device__ __forceinline__ int left_of(const GradPlan &g, int t) {
+    return (HI && t >= 64) ? rl_i(g.lc1, t & 63) : rl_i(g.lc0, t & 63);
+}
+template <bool HI>
+__device__ __forceinline__ int start_of(const GradPlan &g, int t) {
+    return (HI && t >= 64) ? rl_i(g.st1, t & 63) : rl_i(g.st0, t & 63);
+}
+
+// Forward sweep: the same operations on the same operands as wave_eval, so the value is
+// bit-identical.  A leaf followed by +, - or * in the same half is applied at once.
+template <bool HI>
+__device__ __forceinline__ float grad_forward(const WaveProg &p, const GradPlan &g, float x, float *lvl) {
+    float T = 0.0f;
+    for (int t = 0; t < p.len; t++) {
+        const int li = t & 63;
+        const int h = (HI && t >= 64) ? 1 : 0;
+        const uint64_t bit = 1ull << li;
+        if (g.leaf[h] & bit) {
+            const float b = leaf_val<HI>(p, g, t, x);
+            const uint64_t nb = bit << 1;
+            if (g.cheap[h] & nb) {
+                const bool mul = (g.mul[h] & nb) != 0;
+                const float sgn = (g.sub[h] & nb) ? -1.0f : 1.0f;
+                const float pm = __builtin_fmaf(sgn, b, T), pr = T * b;
+                T = mul ? pr : pm;
+                t++;
+                lvl[t * 64] = T;
+            } else {
+                T = b;
+            }
+            continue;
+        }
+        if (g.cheap[h] & bit) {
+            const float a = tok_val<HI>(p, g, left_of<HI>(g, t), x, lvl);
+            const bool mul = (g.mul[h] & bit) != 0;
+            const float sgn = (g.sub[h] & bit) ? -1.0f : 1.0f;
+            const float pm = __builtin_fmaf(sgn, T, a), pr = a * T;
+            T = mul ? pr : pm;
+        } else {
+            const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
+            if (op <= GPR_OP_RPOW) {
+                T = op_bin(op, tok_val<HI>(p, g, left_of<HI>(g, t), x, lvl), T);
+            } else {
+                const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, li) : rl_f(p.c0, li)) : 0.0f;
+                T = op_un(op, T, c);
+            }
+        }
+        lvl[t * 64] = T;
     }
     return T;
 }
 
-// add the wave sum of v (padding lanes masked) to token t of the lane-distributed g0/g1
+// Reverse sweep from adjoint G of the program's value.
 template <bool HI>
-__device__ __forceinline__ void grad_add(double &g0, double &g1, int t, float v, bool valid) {
-    const double s = (double)wave_sum_f(valid ? v : 0.0f);
-    if (HI && t >= 64) { if (lane_id() == t - 64) g1 += s; }
-    else if (lane_id() == t) g0 += s;
-}
-
-template <bool HI>
-__device__ __forceinline__ void wave_adjoint(const WaveProg &p, const float *lv, float G, bool valid, int D0, int D1,
-                                             double &g0, double &g1) {
-    const int l = lane_id();
-    VStack<1> A;
-    int asp = 0;
+__device__ __forceinline__ void grad_reverse(const WaveProg &p, const GradPlan &g, float x, float *lvl, float G) {
     for (int t = p.len - 1; t >= 0; t--) {
         const int li = t & 63;
-        const bool h = HI && t >= 64;
-        const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
-        if (op <= GPR_OP_ONE) {
-            if (op == GPR_OP_CONST) grad_add<HI>(g0, g1, t, G, valid);
-            if (asp > 0) { asp--; G = A.get(0, asp); }
+        const int h = (HI && t >= 64) ? 1 : 0;
+        const uint64_t bit = 1ull << li;
+        if (g.leaf[h] & bit) {
+            if (g.cnst[h] & bit) lvl[t * 64] = G;
+            // the next token roots a left operand: its adjoint waits in its mailbox
+            if (t > 0) G = lvl[start_of<HI>(g, t - 1) * 64];
             continue;
         }
-        const float v = lv[t * 64 + l];
-        const float r = lv[(t - 1) * 64 + l];   // right operand, or the operand of a unary op
-        if (op <= GPR_OP_RPOW) {
-            const int Dt = h ? rl_i(D1, li) : rl_i(D0, li);
-            int lc = 0;
-            const uint64_t m1 = HI ? ballot(l + 64 < t && D1 == Dt) : 0;
-            if (m1) lc = 64 + (63 - __builtin_clzll(m1));
-            else {
-                const uint64_t m0 = ballot(l < t && D0 == Dt);
-                if (m0) lc = 63 - __builtin_clzll(m0);
+        float ga, gb;
+        if (g.cheap[h] & bit) {
+            const bool mul = (g.mul[h] & bit) != 0;
+            ga = G;
+            gb = (g.sub[h] & bit) ? -G : G;
+            const uint64_t rb = bit >> 1;
+            if (g.leaf[h] & rb) {
+                // right operand is a leaf: its adjoint is final now, and the left operand's
+                // subtree ends at t-2 and comes next, so its adjoint is carried
+                if (mul) {
+                    const float b = leaf_val<HI>(p, g, t - 1, x);
+                    const float a = tok_val<HI>(p, g, max(t - 2, 0), x, lvl);
+                    ga = G * b;
+                    gb = G * a;
+                }
+                if (g.cnst[h] & rb) lvl[(t - 1) * 64] = gb;
+                G = ga;
+                t--;
+                continue;
             }
-            const float a = lv[lc * 64 + l], b = r;
-            float ga, gb;
-            switch (op) {
-            case GPR_OP_ADD: ga = G; gb = G; break;
-            case GPR_OP_SUB: ga = G; gb = -G; break;
-            case GPR_OP_RSUB: ga = -G; gb = G; break;
-            case GPR_OP_MUL: ga = G * b; gb = G * a; break;
-            case GPR_OP_DIV: ga = G / b; gb = -(G * a) / (b * b); break;
-            case GPR_OP_RDIV: gb = G / a; ga = -(G * b) / (a * a); break;
-            case GPR_OP_POW: ga = G * b * powf(a, b - 1.0f); gb = G * v * logf(a); break;
-            default: gb = G * a * powf(b, a - 1.0f); ga = G * v * logf(b); break;  // RPOW: b^a
+            if (mul) {
+                const float a = tok_val<HI>(p, g, left_of<HI>(g, t), x, lvl);
+                const float b = tok_val<HI>(p, g, t - 1, x, lvl);
+                ga = G * b;
+                gb = G * a;
             }
-            A.put(0, min(asp, GPR_MAX_DEPTH - 1), ga);
-            asp = min(asp + 1, GPR_MAX_DEPTH);
-            G = gb;
-            continue;
+        } else {
+            const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
+            const float v = lvl[t * 64];
+            if (op <= GPR_OP_RPOW) {
+                const float a = tok_val<HI>(p, g, left_of<HI>(g, t), x, lvl);
+                const float b = tok_val<HI>(p, g, t - 1, x, lvl);
+                switch (op) {
+                case GPR_OP_DIV: ga = G / b; gb = -(G * a) / (b * b); break;
+                case GPR_OP_RSUB: ga = -G; gb = G; break;
+                case GPR_OP_RDIV: gb = G / a; ga = -(G * b) / (a * a); break;
+                case GPR_OP_POW: ga = G * b * powf(a, b - 1.0f); gb = G * v * logf(a); break;
+                default: gb = G * a * powf(b, a - 1.0f); ga = G * v * logf(b); break;  // RPOW: b^a
+                }
+            } else {
+                const float a = tok_val<HI>(p, g, t - 1, x, lvl);
+                const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, li) : rl_f(p.c0, li)) : 0.0f;
+                float gc = 0.0f;
+                switch (op) {
+                case GPR_OP_NEG: ga = -G; break;
+                case GPR_OP_EXP: ga = G * v; break;
+                case GPR_OP_LOG: ga = G / a; break;
+                case GPR_OP_INV: ga = -G / (a * a); break;
+                case GPR_OP_SQRT: ga = G * 0.5f / v; break;
+                case GPR_OP_SIN: ga = G * cosf(a); break;
+                case GPR_OP_COS: ga = -G * sinf(a); break;
+                case GPR_OP_TAN: ga = G * (1.0f + v * v); break;
+                case GPR_OP_ASIN: ga = G / sqrtf(1.0f - a * a); break;
+                case GPR_OP_ACOS: ga = -G / sqrtf(1.0f - a * a); break;
+                case GPR_OP_ATAN: ga = G / (1.0f + a * a); break;
+                case GPR_OP_ABS: ga = a < 0.0f ? -G : G; break;
+                case GPR_OP_CADD: ga = G; gc = G; break;
+                case GPR_OP_CMUL: ga = G * c; gc = G * a; break;
+                case GPR_OP_CPOW: ga = G * c * powf(a, c - 1.0f); gc = G * v * logf(a); break;
+                case GPR_OP_CRPOW: ga = G * v * logf(c); gc = G * a * powf(c, a - 1.0f); break;
+                default: ga = __builtin_nanf(""); break;
+                }
+                if (op >= GPR_OP_CADD) lvl[t * 64] = gc;
+                G = ga;
+                continue;
+            }
         }
-        const float a = r;
-        const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, li) : rl_f(p.c0, li)) : 0.0f;
-        float ga, gc = 0.0f;
-        switch (op) {
-        case GPR_OP_NEG: ga = -G; break;
-        case GPR_OP_EXP: ga = G * v; break;
-        case GPR_OP_LOG: ga = G / a; break;
-        case GPR_OP_INV: ga = -G / (a * a); break;
-        case GPR_OP_SQRT: ga = G * 0.5f / v; break;
-        case GPR_OP_SIN: ga = G * cosf(a); break;
-        case GPR_OP_COS: ga = -G * sinf(a); break;
-        case GPR_OP_TAN: ga = G * (1.0f + v * v); break;
-        case GPR_OP_ASIN: ga = G / sqrtf(1.0f - a * a); break;
-        case GPR_OP_ACOS: ga = -G / sqrtf(1.0f - a * a); break;
-        case GPR_OP_ATAN: ga = G / (1.0f + a * a); break;
-        case GPR_OP_ABS: ga = a < 0.0f ? -G : G; break;
-        case GPR_OP_CADD: ga = G; gc = G; break;
-        case GPR_OP_CMUL: ga = G * c; gc = G * a; break;
-        case GPR_OP_CPOW: ga = G * c * powf(a, c - 1.0f); gc = G * v * logf(a); break;
-        case GPR_OP_CRPOW: ga = G * v * logf(c); gc = G * a * powf(c, a - 1.0f); break;
-        default: ga = __builtin_nanf(""); break;
-        }
-        if (op >= GPR_OP_CADD) grad_add<HI>(g0, g1, t, gc, valid);
-        G = ga;
+        // binary: the left operand's adjoint goes to the mailbox, the right one is carried
+        lvl[start_of<HI>(g, t) * 64] = ga;
+        G = gb;
     }
 }
 
-// Log-likelihood and d(log posterior)/d(constant) of the wave's program.  The LL is
-// bit-identical to wave_loglik<KC, HI> for every KC: lane l visits points c*64 + l in
-// increasing c, the same per-lane order and Kahan updates.  The gradient holds the
-// wave sums of the per-point adjoints (FP32) accumulated over chunks in FP64, plus the
-// Cauchy prior's -2c/(1+c^2); token t sits at lane t of g0 (t < 64) or g1.  lv: this
-// wave's LDS slab of at least len * 64 floats.  Returns false in ok when the LL or a
-// constant's gradient is not finite.
+// FP32 wave sums of the per-point gradients in the slots of the constant-carrying tokens
+// (lanes with valid points only), placed at lane t of s0 (t < 64) or s1.  Two tokens per
+// pass so their reductions overlap.
 template <bool HI>
-__device__ double wave_loglik_grad(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
-                                   const float *dw, float *lv, double &g0, double &g1, bool &ok) {
+__device__ __forceinline__ void grad_chunk_sums(uint64_t m0, uint64_t m1, const float *lvl, bool valid, float &s0,
+                                                float &s1) {
     const int l = lane_id();
-    int D0, D1, dm;
-    wave_depths<HI>(p, D0, D1, dm);
+#pragma unroll
+    for (int h = 0; h < (HI ? 2 : 1); h++) {
+        uint64_t m = h ? m1 : m0;
+        float s = 0.0f;
+        while (m) {
+            const int ta = __builtin_ctzll(m);
+            m &= m - 1;
+            const int tb = m ? __builtin_ctzll(m) : ta;
+            if (m) m &= m - 1;
+            const float va = lvl[(ta + 64 * h) * 64], vb = lvl[(tb + 64 * h) * 64];
+            const float sa = wave_sum_f(valid ? va : 0.0f), sb = wave_sum_f(valid ? vb : 0.0f);
+            s = l == ta ? sa : s;
+            s = l == tb ? sb : s;
+        }
+        if (h) s1 = s; else s0 = s;
+    }
+}
+
+// Log-likelihood and d(log posterior)/d(constant) of the wave's program (g: its plan).
+// The LL is bit-identical to wave_loglik<KC, HI> for every KC: lane l visits points
+// c*64 + l in increasing c, the same per-lane order and Kahan updates.  The gradient
+// holds the wave sums of the per-point adjoints (FP32) accumulated over chunks in FP64,
+// plus the Cauchy prior's -2c/(1+c^2); token t sits at lane t of g0 (t < 64) or g1.
+// lv: this wave's LDS slab of at least len * 64 floats.  Returns false in ok when the
+// LL or a constant's gradient is not finite.
+template <bool HI>
+__device__ double wave_loglik_grad(const WaveProg &p, const GradPlan &g, const GprDataset &d, const float *dx,
+                                   const float *dy, const float *dw, float *lv, double &g0, double &g1, bool &ok) {
+    const int l = lane_id();
+    float *lvl = lv + l;
+    const bool c0 = l < p.len && gpr_has_const(p.o0), c1 = HI && l + 64 < p.len && gpr_has_const(p.o1);
+    const uint64_t m0 = ballot(c0), m1 = HI ? ballot(c1) : 0;
     g0 = 0.0;
     g1 = 0.0;
     float acc = 0.0f, comp = 0.0f;
@@ -641,7 +781,7 @@ __device__ double wave_loglik_grad(const WaveProg &p, const GprDataset &d, const
         const int i = ch * 64 + l;
         const bool valid = i < d.n;
         const float x = valid ? dx[i] : 0.0f, y = valid ? dy[i] : 0.0f, w = valid ? dw[i] : 0.0f;
-        const float f = wave_eval_store<HI>(p, x, lv);
+        const float f = grad_forward<HI>(p, g, x, lvl);
         const float r = f - y;
         const float term = valid ? (r * r) * w : 0.0f;
         const float yv = term - comp;
@@ -650,16 +790,44 @@ __device__ double wave_loglik_grad(const WaveProg &p, const GprDataset &d, const
         acc = tv;
         // d/df of -(f - y)^2 w, with w = wconst when the weights are folded into it
         const float G = valid ? (-2.0f * r) * (folded ? wf : w) : 0.0f;
-        wave_adjoint<HI>(p, lv, G, valid, D0, D1, g0, g1);
+        grad_reverse<HI>(p, g, x, lvl, G);
+        float s0 = 0.0f, s1 = 0.0f;
+        grad_chunk_sums<HI>(m0, m1, lvl, valid, s0, s1);
+        g0 += (double)s0;
+        if (HI) g1 += (double)s1;
     }
     const double s = wave_sum_d((double)acc - (double)comp);
-    const bool c0 = l < p.len && gpr_has_const(p.o0), c1 = HI && l + 64 < p.len && gpr_has_const(p.o1);
     if (c0) g0 += -2.0 * (double)p.c0 / (1.0 + (double)p.c0 * (double)p.c0);
     if (c1) g1 += -2.0 * (double)p.c1 / (1.0 + (double)p.c1 * (double)p.c1);
     const bool bad = ballot((c0 && !__builtin_isfinite(g0)) || (c1 && !__builtin_isfinite(g1))) != 0;
     ok = !bad && __builtin_isfinite(s);
     if (!__builtin_isfinite(s)) return -GPR_INF;
     return d.lconst - (folded ? d.wconst * s : s);
+}
+
+// Log-likelihood through the forward sweep alone, bit-identical to wave_loglik<KC, HI>
+// (same per-lane point order and Kahan updates).  lv: as for wave_loglik_grad.
+template <bool HI>
+__device__ double wave_loglik_slab(const WaveProg &p, const GradPlan &g, const GprDataset &d, const float *dx,
+                                   const float *dy, const float *dw, float *lv) {
+    const int l = lane_id();
+    float *lvl = lv + l;
+    float acc = 0.0f, comp = 0.0f;
+    const int nch = (d.n + 63) / 64;
+    for (int ch = 0; ch < nch; ch++) {
+        const int i = ch * 64 + l;
+        const bool valid = i < d.n;
+        const float x = valid ? dx[i] : 0.0f, y = valid ? dy[i] : 0.0f, w = valid ? dw[i] : 0.0f;
+        const float r = grad_forward<HI>(p, g, x, lvl) - y;
+        const float term = valid ? (r * r) * w : 0.0f;
+        const float yv = term - comp;
+        const float tv = acc + yv;
+        comp = (tv - acc) - yv;
+        acc = tv;
+    }
+    const double s = wave_sum_d((double)acc - (double)comp);
+    if (!__builtin_isfinite(s)) return -GPR_INF;
+    return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
 }
 
 // FP64 sum of v over the lanes where m holds, in token order (lanes of the low half,
@@ -685,46 +853,84 @@ __device__ __forceinline__ double wave_ordered_sum_d(double v0, bool m0, double 
 }
 
 // ---------------------------------------------------------------- proposal metric
-// Lanes 0..k-1 hold metric rows; L is the row-major lower factor in global memory.
-// Products accumulate in FP32 in increasing index order, the density's forward
-// substitution in FP64, both without contraction (the oracle's operation order).
+// Lanes 0..k-1 hold metric rows; the matrices are row-major in global memory, and each
+// lane reads its row with 16-byte loads issued together, before the products.  Products
+// accumulate in FP32 in increasing index order, the density's forward substitution in
+// FP64, both without contraction (the oracle's operation order).
+
+// lane l's row of a 16-wide float matrix, columns below k (lanes past k read row k-1)
+__device__ __forceinline__ void metric_row(const float *A, int k, float (&r)[GPR_METRIC_MAX]) {
+    const float4 *q = (const float4 *)(A + min(lane_id(), k - 1) * GPR_METRIC_MAX);
+#pragma unroll
+    for (int j = 0; j < GPR_METRIC_MAX; j += 4) {
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (j < k) v = q[j / 4];
+        r[j] = v.x; r[j + 1] = v.y; r[j + 2] = v.z; r[j + 3] = v.w;
+    }
+}
 
 // d_i = sum_{j <= i} L_ij v_j, v and d at lanes 0..k-1
 __device__ __forceinline__ float metric_Lv(const float *L, int k, float v) {
     const int l = lane_id();
+    float r[GPR_METRIC_MAX];
+    metric_row(L, k, r);
     float acc = 0.0f;
-    for (int j = 0; j < k; j++) {
-        const float vj = rl_f(v, j);
-        if (l < k && j <= l) acc = gpr_axpy(L[l * GPR_METRIC_MAX + j], vj, acc);
+#pragma unroll
+    for (int j = 0; j < GPR_METRIC_MAX; j++) {
+        if (j < k) {
+            const float vj = rl_f(v, j);
+            if (l < k && j <= l) acc = gpr_axpy(r[j], vj, acc);
+        }
     }
     return acc;
 }
 // v_i = sum_j M_ij g_j, g and v at lanes 0..k-1
 __device__ __forceinline__ float metric_Mv(const float *M, int k, float g) {
     const int l = lane_id();
+    float r[GPR_METRIC_MAX];
+    metric_row(M, k, r);
     float acc = 0.0f;
-    for (int j = 0; j < k; j++) {
-        const float gj = rl_f(g, j);
-        if (l < k) acc = gpr_axpy(M[l * GPR_METRIC_MAX + j], gj, acc);
+#pragma unroll
+    for (int j = 0; j < GPR_METRIC_MAX; j++) {
+        if (j < k) {
+            const float gj = rl_f(g, j);
+            if (l < k) acc = gpr_axpy(r[j], gj, acc);
+        }
     }
     return acc;
 }
 // log density of a Gaussian proposal with covariance e^2 L L^T at offset r (lanes
-// 0..k-1), up to its constant: -|w|^2 / (2 e^2) with w = L^-1 r, summed in row order
+// 0..k-1), up to its constant: -|w|^2 / (2 e^2) with w = L^-1 r; |w|^2 is summed in row
+// order, then divided once.  The row of Li is read in two halves of 8 columns.
 __device__ __forceinline__ double metric_logq(const double *Li, int k, double r, double two_e2) {
 #pragma clang fp contract(off)
     const int l = lane_id();
+    const double2 *q = (const double2 *)(Li + min(l, k - 1) * GPR_METRIC_MAX);
     double w = 0.0;
-    for (int j = 0; j < k; j++) {
-        const double rj = rl_d(r, j);
-        if (l < k && j <= l) w = w + Li[l * GPR_METRIC_MAX + j] * rj;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        if (half * 8 >= k) continue;
+        double row[8];
+#pragma unroll
+        for (int j = 0; j < 8; j += 2) {
+            double2 v = make_double2(0.0, 0.0);
+            if (half * 8 + j < k) v = q[(half * 8 + j) / 2];
+            row[j] = v.x;
+            row[j + 1] = v.y;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int jj = half * 8 + j;
+            if (jj < k) {
+                const double rj = rl_d(r, jj);
+                if (l < k && jj <= l) w = w + row[j] * rj;
+            }
+        }
     }
-    double q = 0.0;
-    for (int i = 0; i < k; i++) {
-        const double wi = rl_d(w, i);
-        q += -wi * wi / two_e2;
-    }
-    return q;
+    const double w2 = w * w;
+    double s = 0.0;
+    for (int i = 0; i < k; i++) s += rl_d(w2, i);
+    return -s / two_e2;
 }
 
 // config accept rule: invalid current always moves; log u < a otherwise.  u lies on the

@@ -19,14 +19,16 @@ struct GprDataset {
 // Proposal metric of one dataset's constant moves: tokens rows[i] (i < k <= 16) move
 // with the lower-triangular factor L (row-major 16 x 16); slot[t] is the metric row of
 // token t or -1.  k = 0: the identity metric.  M and Li are derived from L on the host.
+// The matrices come first and the struct is 16-byte aligned, so a row is read with
+// 16-byte loads.
 #define GPR_METRIC_MAX 16
-struct GprMetric {
-    int k;
-    int rows[GPR_METRIC_MAX];
-    int8_t slot[GPR_MAX_CAP];
+struct alignas(16) GprMetric {
+    double Li[GPR_METRIC_MAX * GPR_METRIC_MAX];  // L^-1 (FP64): proposal density
     float L[GPR_METRIC_MAX * GPR_METRIC_MAX];    // lower factor: proposal noise L z
     float M[GPR_METRIC_MAX * GPR_METRIC_MAX];    // L L^T (FP64 sums rounded): Langevin drift M g
-    double Li[GPR_METRIC_MAX * GPR_METRIC_MAX];  // L^-1 (FP64): proposal density
+    int rows[GPR_METRIC_MAX];
+    int k;
+    int8_t slot[GPR_MAX_CAP];
 };
 
 // A unit of work for the wave kernels: up to waves-per-workgroup chains of one dataset.

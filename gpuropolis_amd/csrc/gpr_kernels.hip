@@ -20,8 +20,9 @@
 // ---------------------------------------------------------------- config MH, wave schedule
 // MALA: constant moves are Langevin proposals c' = c + h g(c) + eps z (h = eps^2/2, g the
 // gradient of the log posterior from the adjoint sweep) with the Hastings ratio of the
-// two Gaussian proposal densities; each wave keeps its token values for the reverse
-// sweep in an LDS slab of dmax_cap * 64 floats.
+// two Gaussian proposal densities.  Under MALA every evaluation uses the stack-free
+// sweep at one point per lane (KC = 1): each wave keeps its token values in an LDS slab
+// of dmax_cap * 64 floats and the current structure's plan in registers.
 extern __shared__ float gpr_lds_slab[];
 
 template <int KC, bool HI, bool MALA>
@@ -29,10 +30,9 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
     const int N = a.nchains;
-    // dynamic LDS: the value stacks of the wpg waves, then (MALA) their token-value slabs
-    float *ls = gpr_lds_slab + (size_t)wave * (GPR_MAX_DEPTH * 64 * KC);
-    float *lv = MALA ? gpr_lds_slab + (size_t)a.wpg * (GPR_MAX_DEPTH * 64 * KC) + (size_t)wave * a.dmax_cap * 64
-                     : nullptr;
+    // dynamic LDS: the value stacks of the wpg waves, or under MALA their token-value slabs
+    float *ls = MALA ? nullptr : gpr_lds_slab + (size_t)wave * (GPR_MAX_DEPTH * 64 * KC);
+    float *lv = MALA ? gpr_lds_slab + (size_t)wave * a.dmax_cap * 64 : nullptr;
     const uint32_t step0 = a.clock ? (uint32_t)uni((int)a.clock[0]) : a.step0;
     // launch counters, lane k holds counter k (folded in once per tile)
     unsigned long long cnt = 0;
@@ -83,11 +83,16 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         // per-lane constant-proposal block, reused by four consecutive steps
         gpr_u4 wlo = {0, 0, 0, 0}, whi = {0, 0, 0, 0};
         uint32_t wblk = 0xffffffffu;
-        // MALA: gradient of the current state (lane-distributed like the constants)
-        float cg0 = 0.0f, cg1 = 0.0f, pg0 = 0.0f, pg1 = 0.0f;
+        // MALA: gradient of the current state (lane-distributed like the constants) and its
+        // metric drift M g at the metric rows
+        float cg0 = 0.0f, cg1 = 0.0f, pg0 = 0.0f, pg1 = 0.0f, cmg = 0.0f, pmg = 0.0f;
         bool grad_ok = false;
+        // token classes of the current program (MALA: with the sweeps' plan)
+        GradPlan gplan{};
+        OpMasks cmask{};
+        if (MALA) gplan = grad_plan<HI>(cur);
+        else cmask = op_masks<HI>(cur);
         // step % struct_every, carried as a counter (no integer division per step)
-        OpMasks cmask = op_masks<HI>(cur);
         // per-tile counts: structure proposed/accepted, constant proposed/accepted, and over
         // evaluated programs the sums of tokens, of (operator tokens + 4) and of constants
         uint32_t c_sp = 0, c_sa = 0, c_cp = 0, c_ca = 0, c_len = 0, c_ops = 0, c_nc = 0;
@@ -207,9 +212,10 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                     if (!grad_ok) {
                         double h0, h1;
                         bool gok;
-                        (void)wave_loglik_grad<HI>(cur, d, dx, dy, dw, lv, h0, h1, gok);
+                        (void)wave_loglik_grad<HI>(cur, gplan, d, dx, dy, dw, lv, h0, h1, gok);
                         cg0 = gok ? (float)h0 : 0.0f;
                         cg1 = gok ? (float)h1 : 0.0f;
+                        if (mk) cmg = metric_Mv(mM, mk, gather_f(cg0, cg1, mrow, HI));
                         grad_ok = true;
                     }
                     const float e = a.eps, h = 0.5f * e * e;
@@ -222,8 +228,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                     float cn = 0.0f;
                     if (mk) {
                         // c' = c + h M g + e L z over the metric rows
-                        const float gr = gather_f(cg0, cg1, mrow, HI);
-                        const float mean = gpr_axpy(h, metric_Mv(mM, mk, gr), cr);
+                        const float mean = gpr_axpy(h, cmg, cr);
                         cn = gpr_axpy(e, dr, mean);
                         fq += metric_logq(mLi, mk, l < mk ? (double)cn - (double)mean : 0.0, inv2e2);
                         const float v0 = shfl_all(cn, max(slot0, 0));
@@ -235,7 +240,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                     }
                     double h0, h1;
                     bool pok;
-                    pll_m = wave_loglik_grad<HI>(prop, d, dx, dy, dw, lv, h0, h1, pok);
+                    pll_m = wave_loglik_grad<HI>(prop, gplan, d, dx, dy, dw, lv, h0, h1, pok);
                     have_pll = true;
                     pg0 = pok ? (float)h0 : 0.0f;
                     pg1 = pok ? (float)h1 : 0.0f;
@@ -243,8 +248,8 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                     const double r0 = (double)cur.c0 - (double)b0, r1 = (double)cur.c1 - (double)b1;
                     double bq = wave_ordered_sum_d<HI>(-r0 * r0 / inv2e2, i0, -r1 * r1 / inv2e2, i1);
                     if (mk) {
-                        const float gr = gather_f(pg0, pg1, mrow, HI);
-                        const float mb = gpr_axpy(h, metric_Mv(mM, mk, gr), cn);
+                        pmg = metric_Mv(mM, mk, gather_f(pg0, pg1, mrow, HI));
+                        const float mb = gpr_axpy(h, pmg, cn);
                         bq += metric_logq(mLi, mk, l < mk ? (double)cr - (double)mb : 0.0, inv2e2);
                     }
                     extra = pok ? bq - fq : -GPR_INF;
@@ -252,8 +257,16 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
             }
             const double pprior = wave_prior<HI>(prop, a.lnsym);
             // token classes: the current program's, or the structural proposal's
-            const OpMasks pmask = structural ? op_masks<HI>(prop) : cmask;
-            const double pll = have_pll ? pll_m : wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &pmask);
+            GradPlan pplan{};
+            OpMasks pmask{};
+            double pll;
+            if (MALA) {
+                pplan = structural ? grad_plan<HI>(prop) : gplan;
+                pll = have_pll ? pll_m : wave_loglik_slab<HI>(prop, pplan, d, dx, dy, dw, lv);
+            } else {
+                pmask = structural ? op_masks<HI>(prop) : cmask;
+                pll = wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &pmask);
+            }
             {
                 const int nleaf = __popcll(ballot(l < prop.len && prop.o0 <= GPR_OP_ONE)) +
                                   (HI ? __popcll(ballot(l + 64 < prop.len && prop.o1 <= GPR_OP_ONE)) : 0);
@@ -270,8 +283,16 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                 prior = pprior;
                 ll = pll;
                 changed = true;
-                if (structural) { c_sa++; grad_ok = false; cmask = pmask; }
-                else { c_ca++; cg0 = pg0; cg1 = pg1; }
+                if (structural) {
+                    c_sa++;
+                    grad_ok = false;
+                    if (MALA) gplan = pplan; else cmask = pmask;
+                } else {
+                    c_ca++;
+                    cg0 = pg0;
+                    cg1 = pg1;
+                    cmg = pmg;
+                }
             }
         }
         {
@@ -305,8 +326,8 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
 
 template <int KC, bool HI, bool MALA>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
-    size_t sh = sizeof(float) * (size_t)a.wpg * GPR_MAX_DEPTH * 64 * KC;
-    if (MALA) sh += sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg;
+    const size_t sh = MALA ? sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg
+                           : sizeof(float) * (size_t)a.wpg * GPR_MAX_DEPTH * 64 * KC;
     if (sh > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
@@ -317,7 +338,9 @@ static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
 }
 
 hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, bool mala, int grid, hipStream_t s) {
-#define GPR_L(K, H) return mala ? launch_mh_wave<K, H, true>(a, grid, s) : launch_mh_wave<K, H, false>(a, grid, s)
+    // MALA evaluates one point per lane whatever the shape's kc
+    if (mala) return hi ? launch_mh_wave<1, true, true>(a, grid, s) : launch_mh_wave<1, false, true>(a, grid, s);
+#define GPR_L(K, H) return launch_mh_wave<K, H, false>(a, grid, s)
     if (kc == 1) { if (hi) GPR_L(1, true); else GPR_L(1, false); }
     else if (kc == 2) { if (hi) GPR_L(2, true); else GPR_L(2, false); }
     else { if (hi) GPR_L(4, true); else GPR_L(4, false); }
@@ -574,7 +597,7 @@ hipError_t gpr_launch_eval_wave(const GprEvalArgs &a, hipStream_t s) {
 }
 
 // Gradient of the log posterior of each tape (one wave per tape, LDS slab of cap * 64
-// floats per wave for the token values).  grad rows t < cap: d/dc_t, 0 where token t
+// floats per wave for the token values and adjoint mailboxes).  grad rows t < cap: d/dc_t, 0 where token t
 // has no constant or t >= len.  ll receives the log posterior.
 template <bool HI>
 __global__ void __launch_bounds__(GPR_WG) k_grad_wave(GprEvalArgs a) {
@@ -593,7 +616,8 @@ __global__ void __launch_bounds__(GPR_WG) k_grad_wave(GprEvalArgs a) {
     double g0, g1;
     bool ok;
     const float *dx = a.dx + a.d.off, *dy = a.dy + a.d.off, *dw = a.dw + a.d.off;
-    const double ll = wave_loglik_grad<HI>(p, a.d, dx, dy, dw, lv, g0, g1, ok);
+    const GradPlan g = grad_plan<HI>(p);
+    const double ll = wave_loglik_grad<HI>(p, g, a.d, dx, dy, dw, lv, g0, g1, ok);
     const double pr = wave_prior<HI>(p, a.lnsym);
     if (l < a.cap) a.grad[(size_t)l * N + chain] = (l < p.len && gpr_has_const(p.o0)) ? (float)g0 : 0.0f;
     if (HI && l + 64 < a.cap)

@@ -775,13 +775,13 @@ static void metric_M(const metric_t *m, const float *g, float *v) {           /*
     }
 }
 static double metric_logq(const metric_t *m, const double *r, double two_e2) {  /* -|L^-1 r|^2 / (2 e^2) */
-    double q = 0.0;
+    double s = 0.0;
     for (int i = 0; i < m->k; i++) {
         double w = 0.0;
         for (int j = 0; j <= i; j++) w = w + m->Li[i * 16 + j] * r[j];
-        q += -w * w / two_e2;
+        s += w * w;
     }
-    return q;
+    return -s / two_e2;
 }
 
 void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int steps,

@@ -112,3 +112,69 @@ def test_mala_shadows_oracle_on_rational_workload(oracle, gpr):
     assert close.sum() >= int(0.9 * nch)
     assert cnt[3] > 0.3 * cnt[2]
     assert abs(st.const_accepted - cnt[3]) <= 0.1 * cnt[3] + 5
+
+
+@pytest.mark.parametrize("cap,n", [(64, 32), (128, 70)])
+def test_mala_structure_moves_follow_oracle(oracle, gpr, cap, n):
+    """With a structure move at every step (struct_every = 1) the MALA kernel makes no
+    constant move, and its structural proposals are evaluated by the stack-free forward
+    sweep with a plan built per proposal: the trajectories follow the oracle's as the
+    random-walk kernel's do (tapes over 64 tokens at cap 128)."""
+    from gpu_util import kappa, ll_close
+    rng = np.random.default_rng(cap + n)
+    if n == 32:
+        from gpuropolis_amd import workloads as W
+        x, y, sd = W.dataset_S(0)
+    else:
+        x = rng.uniform(-5, 5, n).astype(np.float32)
+        y = (1.5 * np.exp(-0.4 * x) + rng.normal(0, 0.05, n)).astype(np.float32)
+        sd = np.full(n, 0.05, dtype=np.float32)
+    nch, steps = 64, 200
+    cfg = oracle.make_cfg(cap=cap, seed=77, proposal=1, eps=0.01, struct_every=1)
+    ops, cs, ln = oracle.cfg_init(cfg, nch)
+    o_ops, o_cs, o_ln = ops.copy(), cs.copy(), ln.copy()
+    _, l1, cnt = oracle.cfg_run_block(cfg, o_ops, o_cs, o_ln, x, y, sd, 0, steps)
+    p1 = np.array([oracle.tape_prior(o_ops[:o_ln[i], i], o_cs[:o_ln[i], i], 10) for i in range(nch)])
+    with gpr.Context(cap=cap, seed=77, proposal=gpr.PROP_MALA, eps=0.01, struct_every=1,
+                     resample=gpr.RS_NONE) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(ops, cs, ln)
+        say(f"mala structure moves cap={cap} n={n} steps={steps}")
+        ctx.run_block(steps)
+        g_ops, g_cs, g_ln, g_pr, g_ll = ctx.read_state()
+        st = ctx.stats()
+    same = (g_ln == o_ln) & np.all(g_ops == o_ops, axis=0) & np.all(g_cs.view(np.uint32) == o_cs.view(np.uint32), axis=0)
+    say("chains on the oracle path:", int(same.sum()), "of", nch, "longest tape", int(g_ln.max()))
+    assert same.sum() >= int(0.95 * nch)
+    assert st.const_proposed == 0 and st.struct_proposed == nch * steps
+    c = np.flatnonzero(same)
+    assert np.array_equal(g_pr[c], p1[c])
+    kap = np.array([kappa(oracle, o_ops[:o_ln[i], i], o_cs[:o_ln[i], i], x, y, sd) for i in c])
+    ok, diff, tol = ll_close(g_ll[c], l1[c], n)
+    assert np.all(ok | (np.isfinite(g_ll[c]) & (diff <= tol + 4 * kap)))
+    # a chain off the shared path may differ in every later accept
+    assert abs(st.struct_accepted - cnt[1]) <= 3 + steps * (nch - same.sum())
+
+
+def test_mala_mixed_moves_keep_state_consistent(oracle, gpr):
+    """Structure moves and MALA constant moves mixed: after the block, every chain's
+    stored prior is its final program's (bit for bit) and its stored LL matches the
+    oracle's evaluation of that program."""
+    from gpu_util import kappa, ll_close
+    from gpuropolis_amd import workloads as W
+    x, y, sd = W.dataset_S(0)
+    nch = 256
+    with gpr.Context(cap=64, seed=31, proposal=gpr.PROP_MALA, eps=0.01, resample=gpr.RS_NONE) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.init_chains()
+        say("mala mixed moves: 256 chains x 300 steps")
+        ctx.run_block(300)
+        ops, cs, ln, pr, ll = ctx.read_state()
+        st = ctx.stats()
+    assert st.struct_accepted > 0 and st.const_accepted > 0
+    p_ref = np.array([oracle.tape_prior(ops[:ln[i], i], cs[:ln[i], i], 10) for i in range(nch)])
+    l_ref = np.array([oracle.tape_loglik(ops[:ln[i], i], cs[:ln[i], i], x, y, sd, mode=2) for i in range(nch)])
+    assert np.array_equal(pr, p_ref)
+    kap = np.array([kappa(oracle, ops[:ln[i], i], cs[:ln[i], i], x, y, sd) for i in range(nch)])
+    ok, diff, tol = ll_close(ll, l_ref, len(x))
+    assert np.all(ok | (np.isfinite(ll) & (diff <= tol + 4 * kap)))
