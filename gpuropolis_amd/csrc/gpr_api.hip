@@ -294,20 +294,25 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     for (GprMetric &m : c->hmetric) memset(m.slot, -1, sizeof(m.slot));
     ALLOC(c->d_cutoff, (size_t)nd);
     ALLOC(c->d_nres, 1);
-    HIP_TRY(hipMemcpy(c->d_ds, c->hds.data(), sizeof(GprDataset) * nd, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_x, x, sizeof(float) * total_pts, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_y, y, sizeof(float) * total_pts, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_w, w.data(), sizeof(float) * total_pts, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_sd, sd, sizeof(float) * total_pts, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_lnc, lnc.data(), sizeof(double) * total_pts, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_tiles, c->htiles.data(), sizeof(GprTile) * c->htiles.size(), hipMemcpyHostToDevice));
+    // everything on the context stream (non-blocking, so not ordered with the null
+    // stream's asynchronous memsets), finished before returning
+    const hipStream_t st = c->stream;
+    HIP_TRY(hipMemcpyAsync(c->d_ds, c->hds.data(), sizeof(GprDataset) * nd, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_x, x, sizeof(float) * total_pts, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_y, y, sizeof(float) * total_pts, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_w, w.data(), sizeof(float) * total_pts, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_sd, sd, sizeof(float) * total_pts, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_lnc, lnc.data(), sizeof(double) * total_pts, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_tiles, c->htiles.data(), sizeof(GprTile) * c->htiles.size(), hipMemcpyHostToDevice,
+                           st));
     c->gexec_stale = true;
-    HIP_TRY(hipMemset(c->d_counters, 0, sizeof(unsigned long long) * GPR_NCNT));
-    HIP_TRY(hipMemset(c->d_ops, 0, (size_t)c->rows * N));
-    HIP_TRY(hipMemset(c->d_cs, 0, sizeof(float) * c->rows * N));
-    HIP_TRY(hipMemset(c->d_len, 0, sizeof(int32_t) * N));
-    HIP_TRY(hipMemset(c->d_prior, 0, sizeof(double) * N));
-    HIP_TRY(hipMemset(c->d_ll, 0, sizeof(double) * N));
+    HIP_TRY(hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * GPR_NCNT, st));
+    HIP_TRY(hipMemsetAsync(c->d_ops, 0, (size_t)c->rows * N, st));
+    HIP_TRY(hipMemsetAsync(c->d_cs, 0, sizeof(float) * c->rows * N, st));
+    HIP_TRY(hipMemsetAsync(c->d_len, 0, sizeof(int32_t) * N, st));
+    HIP_TRY(hipMemsetAsync(c->d_prior, 0, sizeof(double) * N, st));
+    HIP_TRY(hipMemsetAsync(c->d_ll, 0, sizeof(double) * N, st));
+    HIP_TRY(hipStreamSynchronize(st));
     c->step = 0;
     c->block = 0;
     c->have_data = true;
