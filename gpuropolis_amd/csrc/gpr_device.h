@@ -408,12 +408,19 @@ struct Chunk {
 // Per-lane Kahan sums, wave tree reduction; -inf if any term is not finite.
 // LS: the value stack in LDS, ls = this wave's slab of GPR_MAX_DEPTH * 64 * KC floats,
 // pm = the program's token-class masks; else the VGPR stack.
+// reject_below: the caller rejects the proposal whenever the LL lies below this value.
+// Every term of the sum is >= 0, so after each chunk but the last the sum so far bounds
+// the LL from above; when that bound (with a relative margin of 1e-4 on the partial sum,
+// far above its FP32 rounding) is below reject_below, or the partial sum is not finite,
+// the evaluation stops and returns -inf: the proposal is rejected either way.  *points
+// receives the number of data points evaluated.
 template <int KC, bool HI, bool LS = false>
 __device__ double wave_loglik(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
                               const float *dw, const Chunk<KC> &c0, int nchunks, float *ls = nullptr,
-                              const OpMasks *pm = nullptr) {
+                              const OpMasks *pm = nullptr, double reject_below = -GPR_INF, int *points = nullptr) {
     float acc = 0.0f, comp = 0.0f;
     const int l = lane_id();
+    if (points) *points = d.n;
     for (int ch = 0; ch < nchunks; ch++) {
         Chunk<KC> c;
         if (nchunks == 1) c = c0; else c.load(dx, dy, dw, d.n, ch * 64 * KC);
@@ -428,6 +435,14 @@ __device__ double wave_loglik(const WaveProg &p, const GprDataset &d, const floa
             const float tv = acc + yv;
             comp = (tv - acc) - yv;
             acc = tv;
+        }
+        if (ch + 1 < nchunks && reject_below > -GPR_INF) {
+            const double lb = (double)wave_sum_f(acc - comp) * (1.0 - 1e-4);
+            const double ub = d.lconst - (d.wconst > 0.0 ? d.wconst * lb : lb);
+            if (!(ub >= reject_below)) {
+                if (points) *points = (ch + 1) * 64 * KC;
+                return -GPR_INF;
+            }
         }
     }
     // each lane's compensated partial, combined across the wave in FP64

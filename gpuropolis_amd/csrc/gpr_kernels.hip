@@ -94,8 +94,10 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         else cmask = op_masks<HI>(cur);
         // step % struct_every, carried as a counter (no integer division per step)
         // per-tile counts: structure proposed/accepted, constant proposed/accepted, and over
-        // evaluated programs the sums of tokens, of (operator tokens + 4) and of constants
-        uint32_t c_sp = 0, c_sa = 0, c_cp = 0, c_ca = 0, c_len = 0, c_ops = 0, c_nc = 0;
+        // evaluated programs the sums of tokens x points, of (operator tokens + 4) x points
+        // and of constants
+        uint32_t c_sp = 0, c_sa = 0, c_cp = 0, c_ca = 0, c_nc = 0;
+        unsigned long long c_ev = 0, c_fl = 0;
         const uint32_t se = (uint32_t)a.struct_every;
         uint32_t srem = step0 % se;
         for (int i = -1; i < a.steps; i++) {
@@ -260,20 +262,28 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
             GradPlan pplan{};
             OpMasks pmask{};
             double pll;
+            int pts = d.n;
             if (MALA) {
                 pplan = structural ? grad_plan<HI>(prop) : gplan;
                 pll = have_pll ? pll_m : wave_loglik_slab<HI>(prop, pplan, d, dx, dy, dw, lv);
             } else {
                 pmask = structural ? op_masks<HI>(prop) : cmask;
-                pll = wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &pmask);
+                // a = pprior + pll - post + extra < -17 rejects without a draw (cfg_accept);
+                // an LL below this bound gives a < -18 after any FP64 rounding of a
+                double reject_below = -GPR_INF;
+                if (i >= 0 && __builtin_isfinite(post) && __builtin_isfinite(pprior) && __builtin_isfinite(extra))
+                    reject_below = post - pprior - extra - 18.0 -
+                                   1e-12 * (__builtin_fabs(post) + __builtin_fabs(pprior) + __builtin_fabs(extra));
+                pll = wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &pmask, reject_below, &pts);
+                pts = min(pts, d.n);
             }
             {
                 const int nleaf = __popcll(ballot(l < prop.len && prop.o0 <= GPR_OP_ONE)) +
                                   (HI ? __popcll(ballot(l + 64 < prop.len && prop.o1 <= GPR_OP_ONE)) : 0);
                 const int nconst = __popcll(ballot(l < prop.len && gpr_has_const(prop.o0))) +
                                    (HI ? __popcll(ballot(l + 64 < prop.len && gpr_has_const(prop.o1))) : 0);
-                c_len += (uint32_t)prop.len;
-                c_ops += (uint32_t)(prop.len - nleaf + 4);
+                c_ev += (unsigned long long)prop.len * (unsigned long long)pts;
+                c_fl += (unsigned long long)(prop.len - nleaf + 4) * (unsigned long long)pts;
                 c_nc += (uint32_t)nconst;
             }
             if (i < 0) { prior = pprior; ll = pll; continue; }
@@ -296,15 +306,14 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
             }
         }
         {
-            const unsigned long long n = (unsigned long long)d.n;
             unsigned long long v = 0;
             v = l == GPR_CNT_SPROP ? c_sp : v;
             v = l == GPR_CNT_SACC ? c_sa : v;
             v = l == GPR_CNT_CPROP ? c_cp : v;
             v = l == GPR_CNT_CACC ? c_ca : v;
-            v = l == GPR_CNT_EVALS ? (unsigned long long)c_len * n : v;
+            v = l == GPR_CNT_EVALS ? c_ev : v;
             v = l == GPR_CNT_STEPS ? (unsigned long long)a.steps : v;
-            v = l == GPR_CNT_FLOPS ? (unsigned long long)c_ops * n + c_nc : v;
+            v = l == GPR_CNT_FLOPS ? c_fl + c_nc : v;
             cnt += v;
         }
         // Write back only a changed program, and only its rows below the longer of the old

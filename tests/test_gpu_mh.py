@@ -97,3 +97,42 @@ def test_map_resample_matches_oracle(oracle, gpr):
     assert np.array_equal(r_ln, o_ln) and np.array_equal(r_ops, o_ops)
     assert np.array_equal(r_cs.view(np.uint32), o_cs.view(np.uint32))
     assert np.array_equal(r_pr + r_ll, o_pr + o_ll)
+
+
+@pytest.mark.parametrize("kc", [1, 2])
+def test_early_rejection_is_exact(oracle, gpr, kc):
+    """n = 250 at kc = 1 or 2 (4 or 2 chunks per evaluation): a proposal whose LL bound
+    after a chunk already rejects it stops there.  Chains and accept counts are bit for
+    bit those of kc = 4 (one chunk, every point evaluated) and follow the oracle, while the
+    evaluated-point counter shows that evaluations stopped early."""
+    rng = np.random.default_rng(250)
+    x = rng.uniform(-5, 5, 250).astype(np.float32)
+    y = (1.5 * np.exp(-0.4 * x) + rng.normal(0, 0.05, 250)).astype(np.float32)
+    sd = np.full(250, 0.05, dtype=np.float32)
+    nch, steps = 256, 300
+    cfg = oracle.make_cfg(cap=64, seed=55)
+    ops, cs, ln = oracle.cfg_init(cfg, nch)
+    o_ops, o_cs, o_ln = ops.copy(), cs.copy(), ln.copy()
+    _, _, cnt = oracle.cfg_run_block(cfg, o_ops, o_cs, o_ln, x, y, sd, 0, steps)
+    out = {}
+    for k in (4, kc):
+        with gpr.Context(cap=64, seed=55, schedule=gpr.SCHED_WAVE) as ctx:
+            ctx.set_data([(x, y, sd)], [nch])
+            ctx.set_tuning([dict(n_class=c, hi=h, kc=k, waves_per_wg=1, wg_per_cu=8, ms=0.0)
+                            for c in (1, 2, 4) for h in (0, 1)])
+            ctx.write_state(ops, cs, ln)
+            say(f"early rejection n=250 kc={k}")
+            ctx.run_block(steps)
+            out[k] = (ctx.read_state(), ctx.stats())
+    (s4, st4), (sk, stk) = out[4], out[kc]
+    for u, v in zip(s4, sk):
+        assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
+    for f in ("struct_proposed", "struct_accepted", "const_proposed", "const_accepted", "steps"):
+        assert getattr(st4, f) == getattr(stk, f), f
+    say(f"evaluated node x points: kc=4 {st4.node_evals}, kc={kc} {stk.node_evals}")
+    assert stk.node_evals < 0.9 * st4.node_evals
+    g_ops, g_cs, g_ln = sk[0], sk[1], sk[2]
+    same = (g_ln == o_ln) & np.all(g_ops == o_ops, axis=0) & np.all(g_cs.view(np.uint32) == o_cs.view(np.uint32), axis=0)
+    say("chains on the oracle path:", int(same.sum()), "of", nch)
+    assert same.sum() >= int(0.95 * nch)
+    assert abs(stk.const_accepted - cnt[3]) <= 6 + 2 * (nch - same.sum())
