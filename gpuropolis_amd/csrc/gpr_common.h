@@ -209,9 +209,13 @@ GPR_HD gpr_u4 gpr_step_block(uint64_t seed, uint32_t chain, uint32_t step, uint3
 #define GPR_LOGPI_F 0x1.250d06p+0f          /* logf((float)3.141592653589) */
 #define GPR_HALF_LOG2PI_D 0x1.d67f1c864beb4p-1
 // Cauchy(0,1) log density with the reference's float arithmetic (main.cu:303-305)
+GPR_HD float gpr_lcauchy_logf(float c) {
+#pragma clang fp contract(off)
+    return gpr_logf((float)(1.0 + (double)(c * c)));
+}
 GPR_HD double gpr_lcauchy(float c) {
 #pragma clang fp contract(off)
-    return -(double)GPR_LOGPI_F - (double)gpr_logf((float)(1.0 + (double)(c * c)));
+    return -(double)GPR_LOGPI_F - (double)gpr_lcauchy_logf(c);
 }
 GPR_HD double gpr_lphi(float z) {
 #pragma clang fp contract(off)

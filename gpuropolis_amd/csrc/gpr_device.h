@@ -480,6 +480,15 @@ __device__ double wave_prior(const WaveProg &p, double lnsym) {
     return gpr_dl_prior(dl, lnsym, s);
 }
 
+// wave_prior of a program whose constant-carrying tokens are its K constant tokens, from
+// lane base + r of lf holding gpr_lcauchy_logf of the r-th constant (same terms, same order)
+__device__ __forceinline__ double prior_batched(int len, double lnsym, float lf, int base, int K) {
+#pragma clang fp contract(off)
+    double s = 0.0;
+    for (int r = 0; r < K; r++) s += -(double)GPR_LOGPI_F - (double)rl_f(lf, base + r);
+    return gpr_dl_prior(len, lnsym, s);
+}
+
 // Subtree [s0, j] ending at token j: s0-1 is the last u < j whose stack depth after u is
 // below the depth after j.  Also returns the maximum depth of the tape.
 template <bool HI>

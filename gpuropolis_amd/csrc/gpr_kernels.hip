@@ -25,8 +25,11 @@
 // of dmax_cap * 64 floats and the current structure's plan in registers.
 extern __shared__ float gpr_lds_slab[];
 
+// The plain one-point-per-lane kernel is latency bound: six waves per SIMD (80 VGPRs, a
+// few spilled values in its cold paths) run it faster than the five that fit unspilled.
 template <int KC, bool HI, bool MALA>
-__global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
+__global__ void __launch_bounds__(GPR_WG) __attribute__((amdgpu_waves_per_eu((!MALA && KC == 1) ? 6 : 1)))
+k_mh_config_wave(GprMHArgs a) {
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
     const int N = a.nchains;
@@ -83,6 +86,16 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
         // per-lane constant-proposal block, reused by four consecutive steps
         gpr_u4 wlo = {0, 0, 0, 0}, whi = {0, 0, 0, 0};
         uint32_t wblk = 0xffffffffu;
+        // Batched random-walk moves (BATCH: no MALA, tape within the low half; per step: no
+        // metric, at most 16 constant tokens and no constant-operand token).  At the first
+        // constant step of a four-step Philox block, lane j = 16 w + r takes the normal of
+        // word w of the r-th constant token's block and, from the current constants, the
+        // proposed constant of step w and the logf of its Cauchy term; each step then reads
+        // its lanes (proposal and prior), the same values as the per-step path.  Any
+        // accepted move drops the batch (the next constant step recomputes it).
+        constexpr bool BATCH = !MALA && !HI;
+        float cb = 0.0f, lfb = 0.0f;
+        uint32_t zblk = 0xffffffffu;
         // MALA: gradient of the current state (lane-distributed like the constants) and its
         // metric drift M g at the metric rows
         float cg0 = 0.0f, cg1 = 0.0f, pg0 = 0.0f, pg1 = 0.0f, cmg = 0.0f, pmg = 0.0f;
@@ -109,6 +122,8 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
             double extra = 0.0;
             bool have_pll = false;
             double pll_m = 0.0;
+            bool bprior = false;   // the prior comes from the batch: lanes bbase .. bbase+bK-1
+            int bbase = 0, bK = 0;
             const bool structural = struct_slot && !a.fixed_structure;
             if (i < 0) {
             } else if (structural) {
@@ -182,7 +197,37 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                 const uint64_t mk0 = ballot(k0), mk1 = HI ? ballot(k1) : 0;
                 if ((mk0 | mk1) == 0) continue;
                 c_cp++;
-                if ((step >> 2) != wblk) {
+                const bool batch = BATCH && mk == 0 && __popcll(mk0) <= 16 &&
+                                   ballot(l < cur.len && gpr_has_const(cur.o0)) == mk0;
+                if (batch) {
+                    const int wi = (int)(step & 3);
+                    if ((step >> 2) != zblk) {
+                        zblk = step >> 2;
+                        const gpr_u4 wb = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 0);
+                        // lane j: the token of constant rank j & 15
+                        int tr = 0;
+                        uint64_t m = mk0;
+                        for (int r = 0; m; r++) {
+                            const int b = __builtin_ctzll(m);
+                            m &= m - 1;
+                            tr = (l & 15) == r ? b : tr;
+                        }
+                        const int w = l >> 4;
+                        const uint32_t wx = (uint32_t)shfl_all((int)wb.x, tr), wy = (uint32_t)shfl_all((int)wb.y, tr);
+                        const uint32_t wz = (uint32_t)shfl_all((int)wb.z, tr), ww = (uint32_t)shfl_all((int)wb.w, tr);
+                        const float zb = gpr_normal(gpr_u01(w == 0 ? wx : (w == 1 ? wy : (w == 2 ? wz : ww))));
+                        cb = gpr_axpy(a.sigma, zb, shfl_all(cur.c0, tr));
+                        lfb = gpr_lcauchy_logf(cb);
+                    }
+                    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk0 >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mk0, 0u));
+                    const float cv = shfl_all(cb, 16 * wi + rank);
+                    if (k0) prop.c0 = cv;
+                    bprior = true;
+                    bbase = 16 * wi;
+                    bK = __popcll(mk0);
+                } else {
+                if (BATCH || (step >> 2) != wblk) {
                     wblk = step >> 2;
                     wlo = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 0);
                     if (HI) whi = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 1);
@@ -256,8 +301,9 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                     }
                     extra = pok ? bq - fq : -GPR_INF;
                 }
+                }
             }
-            const double pprior = wave_prior<HI>(prop, a.lnsym);
+            const double pprior = bprior ? prior_batched(prop.len, a.lnsym, lfb, bbase, bK) : wave_prior<HI>(prop, a.lnsym);
             // token classes: the current program's, or the structural proposal's
             GradPlan pplan{};
             OpMasks pmask{};
@@ -296,9 +342,11 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_config_wave(GprMHArgs a) {
                 if (structural) {
                     c_sa++;
                     grad_ok = false;
+                    zblk = 0xffffffffu;
                     if (MALA) gplan = pplan; else cmask = pmask;
                 } else {
                     c_ca++;
+                    zblk = 0xffffffffu;
                     cg0 = pg0;
                     cg1 = pg1;
                     cmg = pmg;
