@@ -227,6 +227,85 @@ def batched(G, seed=0, blocks=3, mh_steps=1000):
             "frac_valu_peak": st.alg_flops / (ms / 1e3) / VALU_LANE_OP_PEAK, "launch_shape": table}
 
 
+NIST_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "nist")
+
+
+def nist_batch(G, seed=0, blocks=2, mh_steps=1000, chains=8192, tune_steps=50):
+    """The NIST real-data batch: all 18 NIST StRD datasets (n 6 to 236, the reference's
+    data-sources/NIST data.txt files) in one persistent launch, 8192 chains each (the
+    reference's run-all.sh N), blocks of MH steps plus resampling.  Both semantics: the
+    reference's heap-31 sampler with its median re-draw rule, and the postfix sampler
+    (cap 64) with MAP-100 resampling."""
+    import numpy as np
+    names = sorted(os.listdir(NIST_DIR))
+    dsets = []
+    for nm in names:
+        a = np.loadtxt(os.path.join(NIST_DIR, nm, "data.txt"), comments="#", ndmin=2)
+        dsets.append(tuple(np.ascontiguousarray(a[:, k], dtype=np.float32) for k in range(3)))
+    out = {"workload": f"NIST batch: {len(names)} NIST StRD datasets (n {min(len(d[0]) for d in dsets)} to "
+                       f"{max(len(d[0]) for d in dsets)}), {chains} chains each, one persistent launch per "
+                       f"size class, blocks of {mh_steps} MH steps + resampling", "blocks": blocks}
+    for name, kw in (("reference", dict(semantics=G.SEM_REFERENCE, resample=G.RS_QUANTILE)),
+                     ("config", dict(cap=64, resample=G.RS_MAP))):
+        ctx = G.Context(seed=seed, schedule=G.SCHED_WAVE, **kw)
+        ctx.set_data(dsets, [chains] * len(dsets))
+        ctx.init_chains()
+        ctx.run_block(mh_steps)
+        ctx.resample()
+        if tune_steps > 0:
+            ctx.autotune(tune_steps)
+        ctx.reset_stats()
+        kms = 0.0
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(blocks):
+            ctx.run_block(mh_steps)
+            kms += ctx.stats().last_block_ms
+            ctx.resample()
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+        ctx.close()
+        out[name] = {"mh_steps_per_s": st.steps / wall, "kernel_ms_per_block": kms / blocks,
+                     "node_evals_per_s": st.node_evals / wall,
+                     "alg_tflops": st.alg_flops / (kms / 1e3) / 1e12}
+    return out
+
+
+def full_form(G, seed=0, blocks=1, mh_steps=200, chains=1 << 20):
+    """The full form: 1,048,576 chains over 256 datasets (n log-uniform in [16, 1024], x on
+    [-10, 10]), programs of at most 128 tokens, MALA constant moves and subtree structure
+    moves in the batched persistent launch, MAP-100 resampling."""
+    from gpuropolis_amd import workloads as W
+    dsets, _ = W.datasets_F(seed)
+    nd = len(dsets)
+    per = [chains // nd] * nd
+    ctx = G.Context(cap=128, seed=seed, schedule=G.SCHED_WAVE, proposal=G.PROP_MALA, eps=0.01)
+    ctx.set_data(dsets, per)
+    ctx.init_chains()
+    ctx.run_block(mh_steps)
+    ctx.resample()
+    ctx.reset_stats()
+    kms = 0.0
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(blocks):
+        ctx.run_block(mh_steps)
+        kms += ctx.stats().last_block_ms
+        ctx.resample()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.close()
+    return {"workload": f"full form: {sum(per)} chains, {nd} datasets (n log-uniform [16, 1024]), cap 128, "
+                        f"MALA constants (eps 0.01) + subtree structure moves, MAP-100 resampling, "
+                        f"blocks of {mh_steps} MH steps", "blocks": blocks,
+            "mh_steps_per_s": st.steps / wall, "kernel_ms_per_block": kms / blocks,
+            "node_evals_per_s": st.node_evals / wall, "alg_tflops": st.alg_flops / (kms / 1e3) / 1e12,
+            "acceptance": {"structure": st.struct_accepted / max(1, st.struct_proposed),
+                           "constants": st.const_accepted / max(1, st.const_proposed)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +323,8 @@ def main():
                     help="blocks timed on the other evaluator schedule for comparison (0: skip)")
     ap.add_argument("--batched-blocks", type=int, default=3, help="timed blocks of the batched workload (0: skip)")
     ap.add_argument("--no-ess", dest="ess", action="store_false", help="skip the MALA vs random-walk ESS run")
+    ap.add_argument("--nist-blocks", type=int, default=2, help="timed blocks of the NIST batch (0: skip)")
+    ap.add_argument("--full-blocks", type=int, default=1, help="timed blocks of the full form (0: skip)")
     ap.add_argument("--tune-steps", type=int, default=50,
                     help="autotune the wave launch shapes on this many MH steps after warm-up (0: defaults)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -316,6 +397,10 @@ def main():
                     "alg_tflops": o_st.alg_flops / (o_ms / 1e3) / 1e12}}
     if args.batched_blocks > 0:
         out["batched"] = batched(G, args.seed, args.batched_blocks, args.mh_steps)
+    if args.nist_blocks > 0:
+        out["nist"] = nist_batch(G, args.seed, args.nist_blocks, args.mh_steps)
+    if args.full_blocks > 0:
+        out["full"] = full_form(G, args.seed, args.full_blocks)
     if args.ess:
         out["ess"] = ess_rational(G, args.seed)
     if not args.no_cpu_baseline:

@@ -65,6 +65,8 @@ struct gpr_ctx {
     int *d_tile_counter = nullptr;  // one per bucket
     unsigned long long *d_counters = nullptr;
     int32_t *d_good = nullptr, *d_ngood = nullptr, *d_map_chain = nullptr, *d_nres = nullptr;
+    float *d_gslab = nullptr;          // MALA token slabs of programs beyond the LDS rows
+    size_t gslab_floats = 0;
     double *d_map_post = nullptr, *d_cutoff = nullptr;
     uint32_t step = 0, block = 0;
     // device copy of {step, block} read by graph replays, and its host staging
@@ -117,13 +119,15 @@ static void free_state(gpr_ctx *c) {
     drop_graph(c);
     void *ps[] = {c->d_ds, c->d_x, c->d_y, c->d_w, c->d_sd, c->d_lnc, c->d_ops, c->d_cs, c->d_len, c->d_prior, c->d_ll,
                   c->d_tiles, c->d_tile_counter, c->d_counters, c->d_good, c->d_ngood, c->d_map_chain,
-                  c->d_nres, c->d_map_post, c->d_cutoff, c->d_clock, c->d_metric};
+                  c->d_nres, c->d_map_post, c->d_cutoff, c->d_clock, c->d_metric, c->d_gslab};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     c->d_ds = nullptr; c->d_x = c->d_y = c->d_w = c->d_sd = nullptr; c->d_lnc = nullptr; c->d_ops = nullptr; c->d_cs = nullptr;
     c->d_len = nullptr; c->d_prior = c->d_ll = nullptr; c->d_tiles = nullptr;
     c->d_tile_counter = nullptr; c->d_counters = nullptr; c->d_good = c->d_ngood = c->d_map_chain = nullptr;
     c->d_nres = nullptr; c->d_map_post = c->d_cutoff = nullptr; c->d_clock = nullptr; c->d_metric = nullptr;
+    c->d_gslab = nullptr;
+    c->gslab_floats = 0;
     c->hmetric.clear();
     c->any_metric = false;
     c->have_data = c->have_state = false;
@@ -204,11 +208,33 @@ static void build_tiles(gpr_ctx *c) {
         }
 }
 
+// MALA token slabs of the two-register kernels (cap > 64) live in global memory, one per
+// resident wave (workgroups of the largest launch x waves per workgroup).  Sized whenever
+// the tiles change, before any launch, so never inside a graph capture.
+static int ensure_gslab(gpr_ctx *c) {
+    if (c->cfg.semantics != GPR_SEM_CONFIG || c->cfg.proposal != GPR_PROP_MALA || c->rows <= 64) return GPR_OK;
+    size_t waves = 0;
+    for (const KernelBucket &kb : c->buckets)
+        waves = std::max(waves, (size_t)std::max(1, std::min(kb.ntiles, c->cus * kb.wgpc)) * (size_t)kb.wpg);
+    const size_t need = waves * (size_t)c->rows * 64;
+    if (need <= c->gslab_floats) return GPR_OK;
+    if (c->d_gslab) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipFree(c->d_gslab));
+        c->d_gslab = nullptr;
+        c->gslab_floats = 0;
+    }
+    ALLOC(c->d_gslab, need);
+    c->gslab_floats = need;
+    c->gexec_stale = true;
+    return GPR_OK;
+}
+
 static int upload_tiles(gpr_ctx *c) {
     HIP_TRY(hipMemcpyAsync(c->d_tiles, c->htiles.data(), sizeof(GprTile) * c->htiles.size(), hipMemcpyHostToDevice,
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    return GPR_OK;
+    return ensure_gslab(c);
 }
 
 extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x, const float *y, const float *sd,
@@ -265,8 +291,12 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     c->nd = nd;
     c->N = (int)total_chains;
     c->rows = ref ? GPR_REF_SLOTS : c->cfg.cap;
+    // default shapes: one chain per workgroup, 32 workgroups per CU; the postfix sampler at
+    // one point per lane per chunk (it stops rejected proposals after any 64-point chunk and
+    // its one-point kernel runs six waves per SIMD), the reference sampler with the chunk
+    // widened to the class's points
     for (int k = 0; k < 3; k++)
-        for (int h = 0; h < 2; h++) c->ct[k][h] = ClassTune{kClassN[k], 1, 16, 0.0};
+        for (int h = 0; h < 2; h++) c->ct[k][h] = ClassTune{ref ? kClassN[k] : 1, 1, 32, 0.0};
     c->tuned_sched = -1;
     build_tiles(c);
     const size_t N = (size_t)c->N;
@@ -313,6 +343,7 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     HIP_TRY(hipMemsetAsync(c->d_prior, 0, sizeof(double) * N, st));
     HIP_TRY(hipMemsetAsync(c->d_ll, 0, sizeof(double) * N, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (int r = ensure_gslab(c)) return r;
     c->step = 0;
     c->block = 0;
     c->have_data = true;
@@ -419,6 +450,7 @@ static hipError_t launch_bucket(gpr_ctx *c, const GprMHArgs &a, int b) {
     ab.tile_counter = c->d_tile_counter + b;
     ab.dmax_cap = c->rows;
     ab.wpg = kb.wpg;
+    ab.mala_gslab = kb.hi ? c->d_gslab : nullptr;
     const int grid = std::max(1, std::min(kb.ntiles, c->cus * kb.wgpc));
     if (c->cfg.semantics == GPR_SEM_REFERENCE) return gpr_launch_mh_ref_wave(ab, kb.kc, grid, c->stream);
     return gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, c->cfg.proposal == GPR_PROP_MALA, grid, c->stream);

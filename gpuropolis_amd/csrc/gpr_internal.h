@@ -79,6 +79,9 @@ struct GprMHArgs {
     int rng;
     int dmax_cap;           // largest per-dataset cap (LDS sizing: lane kernel tapes, MALA slabs)
     int wpg;                // waves (= chains of a tile) per workgroup of the wave kernels: 1, 2 or 4
+    // MALA token slabs of tapes over 64 tokens (the two-register kernels): dmax_cap rows
+    // per resident wave (workgroup * wpg + wave) in global memory; null without such tapes
+    float *mala_gslab;
 };
 
 struct GprEvalArgs {

@@ -21,8 +21,10 @@
 // MALA: constant moves are Langevin proposals c' = c + h g(c) + eps z (h = eps^2/2, g the
 // gradient of the log posterior from the adjoint sweep) with the Hastings ratio of the
 // two Gaussian proposal densities.  Under MALA every evaluation uses the stack-free
-// sweep at one point per lane (KC = 1): each wave keeps its token values in an LDS slab
-// of dmax_cap * 64 floats and the current structure's plan in registers.
+// sweep at one point per lane (KC = 1): each wave keeps its token values in a slab of
+// dmax_cap * 64 floats, in LDS for tapes of up to 64 tokens and in global memory for the
+// two-register kernels (an LDS slab of 128 rows, 32 KB, would leave room for five waves
+// per CU), and the current structure's plan in registers.
 extern __shared__ float gpr_lds_slab[];
 
 // The plain one-point-per-lane kernel is latency bound: six waves per SIMD (80 VGPRs, a
@@ -35,7 +37,9 @@ k_mh_config_wave(GprMHArgs a) {
     const int N = a.nchains;
     // dynamic LDS: the value stacks of the wpg waves, or under MALA their token-value slabs
     float *ls = MALA ? nullptr : gpr_lds_slab + (size_t)wave * (GPR_MAX_DEPTH * 64 * KC);
-    float *lv = MALA ? gpr_lds_slab + (size_t)wave * a.dmax_cap * 64 : nullptr;
+    float *lv = !MALA ? nullptr
+                : HI  ? a.mala_gslab + ((size_t)blockIdx.x * a.wpg + wave) * (size_t)a.dmax_cap * 64
+                      : gpr_lds_slab + (size_t)wave * a.dmax_cap * 64;
     const uint32_t step0 = a.clock ? (uint32_t)uni((int)a.clock[0]) : a.step0;
     // launch counters, lane k holds counter k (folded in once per tile)
     unsigned long long cnt = 0;
@@ -383,7 +387,7 @@ k_mh_config_wave(GprMHArgs a) {
 
 template <int KC, bool HI, bool MALA>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
-    const size_t sh = MALA ? sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg
+    const size_t sh = MALA ? (HI ? 0 : sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg)
                            : sizeof(float) * (size_t)a.wpg * GPR_MAX_DEPTH * 64 * KC;
     if (sh > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA>,

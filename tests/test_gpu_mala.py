@@ -178,3 +178,49 @@ def test_mala_mixed_moves_keep_state_consistent(oracle, gpr):
     kap = np.array([kappa(oracle, ops[:ln[i], i], cs[:ln[i], i], x, y, sd) for i in range(nch)])
     ok, diff, tol = ll_close(ll, l_ref, len(x))
     assert np.all(ok | (np.isfinite(ll) & (diff <= tol + 4 * kap)))
+
+
+def test_mala_long_programs_shadow_oracle(oracle, gpr):
+    """At cap 128 the MALA kernel keeps each wave's token slab in global memory: sums of
+    c_j * x^(j mod 3 + 1) terms, 17 to 125 tokens in one launch, shadow the oracle's MALA
+    chains."""
+    rng = np.random.default_rng(17)
+    x = rng.uniform(-1.0, 1.0, 90).astype(np.float32)
+    y = (0.7 * x - 0.4 * x ** 2 + 0.2 * x ** 3 + rng.normal(0.0, 0.05, 90)).astype(np.float32)
+    sd = np.full(90, 0.05, dtype=np.float32)
+    nch, steps, eps, cap = 48, 40, 2e-3, 128
+    tapes = []
+    for c in range(nch):
+        k = (3, 8, 12, 17, 21)[c % 5]
+        toks = []
+        for j in range(k):
+            toks += ["X"] * (j % 3 + 1) + ["MUL"] * (j % 3) + [("CONST", float(rng.normal(0, 0.1))), "MUL"]
+            toks += ["ADD"] if j else []
+        tapes.append(oracle.tape_arrays(toks))
+    O = np.zeros((cap, nch), dtype=np.uint8)
+    V = np.zeros((cap, nch), dtype=np.float32)
+    L = np.zeros(nch, dtype=np.int32)
+    for c, (o, v) in enumerate(tapes):
+        O[:len(o), c] = o
+        V[:len(o), c] = v
+        L[c] = len(o)
+    assert L.max() > 100 and (L > 64).sum() >= nch // 2 and (L <= 64).sum() > 0
+    cfg = oracle.make_cfg(cap=cap, proposal=1, eps=eps, fixed_structure=1, seed=23)
+    o_ops, o_cs, o_ln = O.copy(), V.copy(), L.copy()
+    _, _, cnt = oracle.cfg_run_block(cfg, o_ops, o_cs, o_ln, x, y, sd, 0, steps)
+    with gpr.Context(cap=cap, seed=23, proposal=gpr.PROP_MALA, eps=eps, fixed_structure=1,
+                     resample=gpr.RS_NONE) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(O, V, L)
+        say(f"mala long programs: {nch} chains, lengths {L.min()}..{L.max()}, {steps} steps")
+        ctx.run_block(steps)
+        g_cs = ctx.read_state()[1]
+        st = ctx.stats()
+    rel = np.max(np.abs(g_cs.astype(np.float64) - o_cs) / np.maximum(np.abs(o_cs.astype(np.float64)), 1e-3), axis=0)
+    close = rel < 1e-3
+    say("chains within 1e-3 of the oracle's:", int(close.sum()), "of", nch, "(over 64 tokens:",
+        int(close[L > 64].sum()), "of", int((L > 64).sum()), ") acceptance gpu",
+        st.const_accepted / st.const_proposed, "oracle", cnt[3] / max(cnt[2], 1))
+    assert close[L > 64].sum() >= int(0.9 * (L > 64).sum())
+    assert close.sum() >= int(0.9 * nch)
+    assert cnt[3] > 0.2 * cnt[2]
