@@ -145,6 +145,9 @@ extern "C" int gpr_create(const gpr_config *cfg, gpr_ctx **out) {
     if (cfg->nsym < 2) return fail(GPR_E_ARG, "nsym must be >= 2");
     if (cfg->rng == GPR_RNG_LCG && cfg->semantics != GPR_SEM_REFERENCE)
         return fail(GPR_E_ARG, "the LCG stream exists only with reference semantics");
+    if (cfg->ops < 0 || cfg->ops >= GPR_NUM_OPSETS) return fail(GPR_E_ARG, "ops must be a GPR_OPS_* op set");
+    if (cfg->ops != GPR_OPS_DEFAULT && cfg->semantics != GPR_SEM_REFERENCE)
+        return fail(GPR_E_ARG, "op sets choose the reference semantics' heap ops: reference semantics only");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (cfg->device < 0 || cfg->device >= ndev)
@@ -367,6 +370,8 @@ static GprMHArgs mh_args(gpr_ctx *c, int steps, int32_t block_seed) {
     a.seed = c->cfg.seed;
     a.dmax = c->cfg.dmax;
     a.lnsym = log((double)c->cfg.nsym);
+    a.opset = c->cfg.ops;
+    a.ref_lnops = log((double)gpr_opset_nops(c->cfg.ops) + 1.0);   // log((bayes_t)NOPS+1), main.cu:536
     a.sigma = c->cfg.sigma;
     a.eps = c->cfg.eps;
     a.struct_every = c->cfg.struct_every;
@@ -417,7 +422,7 @@ extern "C" int gpr_write_state(gpr_ctx *c, const uint8_t *ops, const float *cs, 
         }
     } else {
         for (size_t i = 0; i < N * GPR_REF_SLOTS; i++)
-            if (ops[i] >= GPR_REF_NOPS) return fail(GPR_E_ARG, "bad heap op code");
+            if (ops[i] >= GPR_REF_NCODES) return fail(GPR_E_ARG, "bad heap op code");
     }
     HIP_TRY(hipSetDevice(c->cfg.device));
     HIP_TRY(hipMemcpyAsync(c->d_ops, ops, (size_t)c->rows * N, hipMemcpyHostToDevice, c->stream));
@@ -530,6 +535,7 @@ static GprResampleArgs rs_args(gpr_ctx *c) {
     r.nats = c->cfg.resample_nats;
     r.quantile = c->cfg.resample_quantile;
     r.semantics = c->cfg.semantics;
+    r.opset = c->cfg.ops;
     r.nresampled = c->d_nres;
     return r;
 }
@@ -885,15 +891,19 @@ extern "C" int gpr_grad_logpost(gpr_ctx *c, int dataset, const uint8_t *d_ops, c
 }
 
 // heap -> postfix of the reachable tree (host), main.cu:371-393 semantics
-static const int8_t kRefToTape[GPR_REF_NOPS] = {
+// (log 2 and the physical constants become constant tokens of their value, halving a
+// c* token with c = 0.5)
+static const int8_t kRefToTape[GPR_REF_NCODES] = {
     GPR_OP_ONE, GPR_OP_CONST, GPR_OP_INV, -1, -1, GPR_OP_ADD, GPR_OP_SUB, GPR_OP_RSUB, GPR_OP_CADD, GPR_OP_MUL,
     GPR_OP_CMUL, GPR_OP_DIV, GPR_OP_RDIV, GPR_OP_LOG, GPR_OP_EXP, GPR_OP_POW, GPR_OP_CPOW, GPR_OP_RPOW, GPR_OP_CRPOW,
-    GPR_OP_SQRT, GPR_OP_SIN, GPR_OP_ASIN, GPR_OP_COS, GPR_OP_ACOS, GPR_OP_TAN, GPR_OP_ATAN, GPR_OP_ABS};
+    GPR_OP_SQRT, GPR_OP_SIN, GPR_OP_ASIN, GPR_OP_COS, GPR_OP_ACOS, GPR_OP_TAN, GPR_OP_ATAN, GPR_OP_ABS,
+    GPR_OP_CONST, GPR_OP_CMUL, GPR_OP_CONST, GPR_OP_CONST, GPR_OP_CONST, GPR_OP_CONST, GPR_OP_CONST, GPR_OP_CONST,
+    GPR_OP_CONST, GPR_OP_CONST, GPR_OP_CONST, GPR_OP_CONST, GPR_OP_CONST};
 
 static int decode_heap(const uint8_t *hop, const float *hc, size_t stride, int slot, int pos, uint8_t *ops, float *cs) {
     if (slot > GPR_REF_SLOTS) { ops[pos] = GPR_OP_X; cs[pos] = 0.0f; return pos + 1; }
     const int o = hop[(size_t)(slot - 1) * stride];
-    if (o >= GPR_REF_NOPS) return -1;
+    if (o >= GPR_REF_NCODES) return -1;
     if (o == GPR_REF_A) return decode_heap(hop, hc, stride, 2 * slot, pos, ops, cs);
     if (o == GPR_REF_B) return decode_heap(hop, hc, stride, 2 * slot + 1, pos, ops, cs);
     const int t = kRefToTape[o];
@@ -904,6 +914,8 @@ static int decode_heap(const uint8_t *hop, const float *hc, size_t stride, int s
     if (pos < 0) return pos;
     ops[pos] = (uint8_t)t;
     cs[pos] = gpr_has_const(t) ? hc[(size_t)(slot - 1) * stride] : 0.0f;
+    if (gpr_ref_is_physical(o)) cs[pos] = gpr_ref_physical_value(o);
+    if (o == GPR_REF_HALF) cs[pos] = 0.5f;
     return pos + 1;
 }
 
@@ -911,6 +923,19 @@ extern "C" int gpr_ref_decode(const uint8_t *hop, const float *hc, size_t stride
     if (!hop || !hc || !ops || !cs || stride < 1) return fail(GPR_E_ARG, "null argument");
     const int n = decode_heap(hop, hc, stride, 1, 0, ops, cs);
     if (n < 0) return fail(GPR_E_ARG, "bad heap op code");
+    return n;
+}
+
+extern "C" int gpr_ref_op_set(int set, uint8_t *codes, int max_codes, const char **program_code) {
+    if (set < 0 || set >= GPR_NUM_OPSETS) return fail(GPR_E_ARG, "unknown op set");
+    const int n = gpr_opset_nops(set);
+    if (codes) {
+        if (max_codes < n) return fail(GPR_E_ARG, "codes holds fewer than NOPS entries");
+        for (int i = 0; i < n; i++) codes[i] = (uint8_t)gpr_opset_code(set, i);
+    }
+    static const char *const kCode[GPR_NUM_OPSETS] = {GPR_REF_PROGRAM_CODE, "1CIab+-_#*@/|", "1CAB+-_#*@^",
+                                                      "1CAB+-_#@", "B2epcghm:o.~Kab+-_*"};
+    if (program_code) *program_code = kCode[set];
     return n;
 }
 

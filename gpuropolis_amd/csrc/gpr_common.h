@@ -232,6 +232,51 @@ GPR_HD double gpr_dl_prior(int dl, double lnsym, double s) {
     return -(double)dl * lnsym + s;
 }
 
+// ---------------------------------------------------------------- reference op sets
+// The reference chooses an op set at compile time (main.cu:51-86); here it is a run-time
+// table: the heap code of each enum value in enum order and the size NOPS.  SIMPLIFIED,
+// POLYNOMIAL and LINEAR are the intended sets (their #if blocks do not compile as
+// written: duplicate enumerators, a missing comma, a second enum OPS).
+GPR_HD int gpr_opset_nops(int set) {
+    return set == GPR_OPS_SIMPLIFIED ? 13 : set == GPR_OPS_POLYNOMIAL ? 11 : set == GPR_OPS_LINEAR ? 9
+         : set == GPR_OPS_PHYSICAL ? 19 : GPR_REF_NOPS;
+}
+GPR_HD int gpr_opset_code(int set, int i) {
+    // one 40-byte row per op set, in the order of GPR_OPS_* (a load, no selects: on the
+    // device i is per lane)
+    const char *t =
+        // default
+        "\x00\x01\x02\x03\x04\x05\x06\x07\x08\x09\x0a\x0b\x0c\x0d\x0e\x0f\x10\x11\x12\x13\x14\x15\x16\x17\x18\x19\x1a\x1b\x1c\x1d\x1e\x1f\x20\x21\x22\x23\x24\x25\x26\x27"
+        // SIMPLIFIED
+        "\x00\x01\x02\x03\x04\x05\x06\x07\x08\x09\x0a\x0b\x0c\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00"
+        // POLYNOMIAL
+        "\x00\x01\x03\x04\x05\x06\x07\x08\x09\x0a\x10\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00"
+        // LINEAR
+        "\x00\x01\x03\x04\x05\x06\x07\x08\x0a\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00"
+        // PHYSICAL
+        "\x1b\x1c\x1d\x1e\x1f\x20\x21\x22\x23\x24\x25\x26\x27\x03\x04\x05\x06\x07\x09\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00\x00";
+    return (unsigned char)t[set * 40 + i];
+}
+// PHYSICAL_CONSTANTS values (main.cu:429-441): the data_t value of each double literal;
+// log 2 is log(2.0) in double, then float
+GPR_HD bool gpr_ref_is_physical(int o) { return o == GPR_REF_LOGOF2 || (o >= GPR_REF_E && o <= GPR_REF_KB); }
+GPR_HD float gpr_ref_physical_value(int o) {
+    switch (o) {
+    case GPR_REF_E: return (float)1.6021766208e-19;
+    case GPR_REF_PI: return (float)1.14472988584915;
+    case GPR_REF_CLIGHT: return (float)19.5186009865452;
+    case GPR_REF_G: return (float)-23.4302046558627;
+    case GPR_REF_HBAR: return (float)-78.2347583540371;
+    case GPR_REF_MU0: return (float)-13.5870714043359;
+    case GPR_REF_EL: return (float)-43.2777536741305;
+    case GPR_REF_MP: return (float)-61.6554051167679;
+    case GPR_REF_ME: return (float)-69.1708328401338;
+    case GPR_REF_E0: return (float)-25.4501305691713;
+    case GPR_REF_KB: return (float)-52.6369038080531;
+    default: return (float)0.69314718055994530942;   // GPR_REF_LOGOF2: log(2.0)
+    }
+}
+
 // ---------------------------------------------------------------- config generation table
 // 64 buckets of the top 6 bits of a Philox word: X 16, CONST 16, ADD SUB MUL DIV POW
 // EXP LOG NEG 4 each.

@@ -82,6 +82,9 @@ struct GprMHArgs {
     // MALA token slabs of tapes over 64 tokens (the two-register kernels): dmax_cap rows
     // per resident wave (workgroup * wpg + wave) in global memory; null without such tapes
     float *mala_gslab;
+    // reference semantics: op set (GPR_OPS_*) and log(NOPS + 1) of its prior
+    int opset;
+    double ref_lnops;
 };
 
 struct GprEvalArgs {
@@ -119,6 +122,7 @@ struct GprResampleArgs {
     float quantile;
     int semantics;
     int32_t *nresampled;    // [1]
+    int opset;              // reference semantics: op set of the re-drawn programs
 };
 
 // launchers (gpr_kernels.hip)

@@ -20,7 +20,7 @@
 #define R_LOG1M01_D (-0x1.af8e82826b325p-4) /* log(1.0 - 0.1f) */
 #define R_LOG1M015_D (-0x1.4cd6ba6a55088p-3)/* log(1.0 - 0.15f) */
 #define R_LOG2_D 0x1.62e42fefa39efp-1
-#define R_LOG28_D 0x1.aa85ad6964a23p+1    /* log(NOPS + 1) */
+#define R_LOG28_D 0x1.aa85ad6964a23p+1    /* log(NOPS + 1) of the default op set */
 #define R_LOG2PI_F 0x1.d67f1ep+0f
 
 __device__ __forceinline__ float r_pow10(int order) {  // powf(10, order), order in [-4, 3]
@@ -168,8 +168,14 @@ __device__ __forceinline__ int r_to_tape(int o) {
     case GPR_REF_TAN: return GPR_OP_TAN;
     case GPR_REF_ATAN: return GPR_OP_ATAN;
     case GPR_REF_ABS: return GPR_OP_ABS;
-    default: return -1;
+    case GPR_REF_HALF: return GPR_OP_CMUL;     // a/2.0 in double, then float == 0.5f * a
+    default: return gpr_ref_is_physical(o) ? GPR_OP_CONST : -1;
     }
+}
+// the constant a token of heap op o carries: its heap constant, a physical constant's
+// value, or the factor of halving
+__device__ __forceinline__ float r_tape_const(int o, float hc) {
+    return gpr_ref_is_physical(o) ? gpr_ref_physical_value(o) : (o == GPR_REF_HALF ? 0.5f : hc);
 }
 
 // ---------------------------------------------------------------- lane schedule
@@ -206,7 +212,8 @@ __device__ __forceinline__ float r_dispatch_eval(int o, float a, float b, float 
     case GPR_REF_RPOW: return powf(b, a);
     case GPR_REF_CRPOW: return powf(C, a);
     case GPR_REF_ABS: return fabsf(a);
-    default: return __builtin_nanf("");
+    case GPR_REF_HALF: return (float)((double)a / 2.0);
+    default: return gpr_ref_is_physical(o) ? gpr_ref_physical_value(o) : __builtin_nanf("");
     }
 }
 
@@ -224,6 +231,7 @@ __device__ float r_fold_eval(const gpr_i32v &P, const gpr_f32v &C, float x) {
     return buf[1];
 }
 __device__ __forceinline__ int r_len_kind(int o) {  // 0 leaf, 1 A, 2 B, 3 unary, 4 C-op, 5 binary
+    if (gpr_ref_is_physical(o)) return 0;              // PHYSICAL_CONSTANT_LENGTH_PRIOR = 1
     switch (o) {
     case GPR_REF_ONE: case GPR_REF_CONST: return 0;
     case GPR_REF_A: return 1;
@@ -262,11 +270,12 @@ __device__ void r_fold_len_count(const gpr_i32v &P, float &length, int &count) {
 
 // compute_prior (main.cu:529-544) and compute_likelihood (main.cu:454-468), one lane.
 // lnc[i] = 0.5*(logf(2 pi) + 2 logf(sd_i)) is precomputed on the host with the host libm.
-__device__ double r_prior(const gpr_i32v &P, const gpr_f32v &C) {
+__device__ double r_prior(const gpr_i32v &P, const gpr_f32v &C, double lnops) {
+#pragma clang fp contract(off)
     float length;
     int count;
     r_fold_len_count(P, length, count);
-    double prior = -(double)(1.0f * length) * R_LOG28_D;
+    double prior = -(double)(1.0f * length) * lnops;
     for (int c = 0; c < GPR_REF_SLOTS; c++) prior += gpr_lcauchy(C[c]);
     return prior;
 }
@@ -294,6 +303,7 @@ __device__ __forceinline__ bool r_accept(S &s, double ppost, double cur, double 
 template <int RNG>
 __global__ void __launch_bounds__(128) k_mh_ref_lane(GprMHArgs a) {
     const uint32_t step0 = a.clock ? a.clock[0] : a.step0;
+    const int nops = gpr_opset_nops(a.opset);
     const int chain = blockIdx.x * blockDim.x + threadIdx.x;
     if (chain >= a.nchains) return;
     int di = 0;
@@ -308,7 +318,7 @@ __global__ void __launch_bounds__(128) k_mh_ref_lane(GprMHArgs a) {
         P[i] = a.ops[(size_t)i * N + chain];
         C[i] = a.cs[(size_t)i * N + chain];
     }
-    double cur_prior = r_prior(P, C);
+    double cur_prior = r_prior(P, C, a.ref_lnops);
     double cur_ll = r_likelihood(P, C, x, y, sdv, lc, d.n);
     double cur_post = cur_prior + cur_ll;
     LcgSrc lcg{(uint32_t)a.block_seed + (uint32_t)chain};
@@ -332,12 +342,12 @@ __global__ void __launch_bounds__(128) k_mh_ref_lane(GprMHArgs a) {
             for (int i = 0; i < GPR_REF_SLOTS; i++) {
                 oldP[i] = P[i];
                 if (i >= nn && r_is_descendant(nn, i)) {
-                    P[i] = RNG == GPR_RNG_LCG ? r_random_int(GPR_REF_NOPS, lcg) : r_random_int(GPR_REF_NOPS, ph);
+                    P[i] = gpr_opset_code(a.opset, RNG == GPR_RNG_LCG ? r_random_int(nops, lcg) : r_random_int(nops, ph));
                 }
             }
             r_fold_len_count(P, length, count);
             backward = -(double)gpr_logf((float)count);
-            const double pp = r_prior(P, C);
+            const double pp = r_prior(P, C, a.ref_lnops);
             const double pl = r_likelihood(P, C, x, y, sdv, lc, d.n);
             const double ppost = pp + pl;
             const double acc = ppost - cur_post + backward - forward;
@@ -358,7 +368,7 @@ __global__ void __launch_bounds__(128) k_mh_ref_lane(GprMHArgs a) {
                 forward += ft;
                 backward += bt;
             }
-            const double pp = r_prior(P, C);
+            const double pp = r_prior(P, C, a.ref_lnops);
             const double pl = r_likelihood(P, C, x, y, sdv, lc, d.n);
             const double ppost = pp + pl;
             const double acc = ppost - cur_post + backward - forward;
@@ -466,23 +476,27 @@ __device__ RefTape r_decode_wave(int hop, float hc) {
     rt.slot = valid ? (tok & 255) : 255;
     rt.p.o1 = 0;
     rt.p.c1 = 0.0f;
-    const float cs = shfl_all(hc, rt.slot & 31);
+    const float cs = shfl_all(r_tape_const(hop, hc), rt.slot & 31);
     rt.p.c0 = (valid && gpr_has_const(rt.p.o0)) ? cs : 0.0f;
     rt.count = len - __popcll(ballot(valid && rt.p.o0 == GPR_OP_X));
-    rt.dl = len + __popcll(ballot(valid && rt.p.o0 >= GPR_OP_CADD));
+    // description length: tokens plus the C-op tokens (dispatch_length kind 4; halving is
+    // a c* token of length 1 + a)
+    const int tk = shfl_all(k, rt.slot & 31);
+    rt.dl = len + __popcll(ballot(valid && tk == 4));
     return rt;
 }
 
 // refresh the tape's constants from the heap constants
-__device__ __forceinline__ void r_tape_consts(RefTape &rt, float hc) {
-    const float cs = shfl_all(hc, rt.slot & 31);
+__device__ __forceinline__ void r_tape_consts(RefTape &rt, int hop, float hc) {
+    const float cs = shfl_all(r_tape_const(hop, hc), rt.slot & 31);
     rt.p.c0 = (lane_id() < rt.p.len && gpr_has_const(rt.p.o0)) ? cs : 0.0f;
 }
 
 // prior: -length*log(NOPS+1) + sum over all 31 constants in slot order (main.cu:529-544)
-__device__ __forceinline__ double r_prior_wave(int dl, float hc) {
+__device__ __forceinline__ double r_prior_wave(int dl, float hc, double lnops) {
+#pragma clang fp contract(off)
     const double lc = gpr_lcauchy(hc);
-    double prior = -(double)(1.0f * (float)dl) * R_LOG28_D;
+    double prior = -(double)(1.0f * (float)dl) * lnops;
     for (int c = 0; c < GPR_REF_SLOTS; c++) prior += rl_d(lc, c);
     return prior;
 }
@@ -490,6 +504,7 @@ __device__ __forceinline__ double r_prior_wave(int dl, float hc) {
 template <int KC>
 __global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
     const uint32_t step0 = a.clock ? a.clock[0] : a.step0;
+    const int nops = gpr_opset_nops(a.opset);
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
     const int N = a.nchains;
@@ -513,7 +528,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
         int hop = live ? a.ops[(size_t)l * N + chain] : 0;
         float hc = live ? a.cs[(size_t)l * N + chain] : 0.0f;
         RefTape cur = r_decode_wave(hop, hc);
-        double cur_prior = r_prior_wave(cur.dl, hc);
+        double cur_prior = r_prior_wave(cur.dl, hc, a.ref_lnops);
         double cur_ll = wave_loglik<KC, false>(cur.p, d, dx, dy, dw, c0, nchunks);
         double cur_post = cur_prior + cur_ll;
         double forward = 0.0, backward = 0.0;
@@ -530,7 +545,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
                 ph.st.init(a.seed, chain, step, GPR_P_STRUCT, 0);
                 const int nn = !r_valid(cur_post) ? 0 : r_random_int(GPR_REF_SLOTS, ph);
                 for (int i = nn; i < GPR_REF_SLOTS; i++) {
-                    if (r_is_descendant(nn, i)) php = wl_i(php, r_random_int(GPR_REF_NOPS, ph), i);
+                    if (r_is_descendant(nn, i)) php = wl_i(php, gpr_opset_code(a.opset, r_random_int(nops, ph)), i);
                 }
                 prop = r_decode_wave(php, phc);
                 backward = -(double)gpr_logf((float)prop.count);
@@ -546,9 +561,9 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
                     forward += rl_d(ft, c);
                     backward += rl_d(bt, c);
                 }
-                r_tape_consts(prop, phc);
+                r_tape_consts(prop, php, phc);
             }
-            const double pp = r_prior_wave(prop.dl, phc);
+            const double pp = r_prior_wave(prop.dl, phc, a.ref_lnops);
             const double pl = wave_loglik<KC, false>(prop.p, d, dx, dy, dw, c0, nchunks);
             n_evals += (unsigned long long)prop.p.len * (unsigned long long)d.n;
             {
@@ -592,26 +607,40 @@ hipError_t gpr_launch_mh_ref_wave(const GprMHArgs &a, int kc, int grid, hipStrea
 // ---------------------------------------------------------------- init and median resampling
 // initial draw (main.cu:1133-1137): 31 ops ~ U{0..NOPS-2}, then 31 constants
 // 10^U{-4..3} N(0,1), host transforms, stream (chain, step, purpose, 0)
+// The op set is a template parameter of the draw kernels: each one-lane-per-chain draw
+// loop sees a constant table row.
+template <int SET>
 __device__ void r_draw_chain(uint64_t seed, uint32_t chain, uint32_t step, uint32_t purpose, uint8_t *ops, float *cs,
                              int N) {
 #pragma clang fp contract(off)
     PhiloxSrc s;
     s.st.init(seed, chain, step, purpose, 0);
-    for (int i = 0; i < GPR_REF_SLOTS; i++) ops[(size_t)i * N + chain] = (uint8_t)r_host_random_int(GPR_REF_NOPS - 1, s);
+    for (int i = 0; i < GPR_REF_SLOTS; i++)
+        ops[(size_t)i * N + chain] = (uint8_t)gpr_opset_code(SET, r_host_random_int(gpr_opset_nops(SET) - 1, s));
     for (int i = 0; i < GPR_REF_SLOTS; i++) {
         const int order = r_host_random_int(8, s) - 4;
         cs[(size_t)i * N + chain] = 1.0f * r_pow10(order) * r_host_random_normal(s);
     }
 }
 
+template <int SET>
 __global__ void k_init_ref(GprMHArgs a) {
     const int chain = blockIdx.x * blockDim.x + threadIdx.x;
     if (chain >= a.nchains) return;
-    r_draw_chain(a.seed, (uint32_t)chain, 0, GPR_P_INIT, a.ops, a.cs, a.nchains);
+    r_draw_chain<SET>(a.seed, (uint32_t)chain, 0, GPR_P_INIT, a.ops, a.cs, a.nchains);
 }
 
+#define GPR_OPSET_LAUNCH(KERNEL, SET, GRID, BLOCK, S, ARGS)                                                        \
+    switch (SET) {                                                                                                \
+    case GPR_OPS_SIMPLIFIED: hipLaunchKernelGGL(KERNEL<GPR_OPS_SIMPLIFIED>, GRID, BLOCK, 0, S, ARGS); break;      \
+    case GPR_OPS_POLYNOMIAL: hipLaunchKernelGGL(KERNEL<GPR_OPS_POLYNOMIAL>, GRID, BLOCK, 0, S, ARGS); break;      \
+    case GPR_OPS_LINEAR: hipLaunchKernelGGL(KERNEL<GPR_OPS_LINEAR>, GRID, BLOCK, 0, S, ARGS); break;              \
+    case GPR_OPS_PHYSICAL: hipLaunchKernelGGL(KERNEL<GPR_OPS_PHYSICAL>, GRID, BLOCK, 0, S, ARGS); break;          \
+    default: hipLaunchKernelGGL(KERNEL<GPR_OPS_DEFAULT>, GRID, BLOCK, 0, S, ARGS); break;                         \
+    }
+
 hipError_t gpr_launch_init_ref(const GprMHArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_init_ref, dim3((a.nchains + 255) / 256), dim3(256), 0, s, a);
+    GPR_OPSET_LAUNCH(k_init_ref, a.opset, dim3((a.nchains + 255) / 256), dim3(256), s, a);
     return hipGetLastError();
 }
 
@@ -666,13 +695,14 @@ __global__ void __launch_bounds__(R_SEL_WG) k_rs_quantile_select(GprResampleArgs
     if (tid == 0) a.cutoff[blockIdx.x] = r_unkey(s_prefix);
 }
 
+template <int SET>
 __global__ void k_rs_quantile_redraw(GprResampleArgs a) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.nchains) return;
     int di = 0;
     while (di + 1 < a.ndatasets && a.ds[di + 1].chain0 <= c) di++;
     if (!(a.prior[c] + a.ll[c] < a.cutoff[di])) return;   // NaN posteriors never resampled
-    r_draw_chain(a.seed, (uint32_t)c, a.clock ? a.clock[1] : a.block, GPR_P_RESAMPLE, a.ops, a.cs, a.nchains);
+    r_draw_chain<SET>(a.seed, (uint32_t)c, a.clock ? a.clock[1] : a.block, GPR_P_RESAMPLE, a.ops, a.cs, a.nchains);
     atomicAdd(a.nresampled, 1);
 }
 
@@ -680,7 +710,7 @@ hipError_t gpr_launch_resample_quantile(const GprResampleArgs &a, hipStream_t s)
     hipLaunchKernelGGL(k_rs_quantile_select, dim3(a.ndatasets), dim3(R_SEL_WG), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_rs_quantile_redraw, dim3((a.nchains + 255) / 256), dim3(256), 0, s, a);
+    GPR_OPSET_LAUNCH(k_rs_quantile_redraw, a.opset, dim3((a.nchains + 255) / 256), dim3(256), s, a);
     return hipGetLastError();
 }
 

@@ -8,6 +8,22 @@
 
 #include "gpuropolis.h"
 
+// Op sets (main.cu:51-86): the heap code of each enum value, in enum order (the
+// library's gpr_ref_op_set returns the same table).
+static const int kOpsetN[GPR_NUM_OPSETS] = {27, 13, 11, 9, 19};
+static const uint8_t kOpset[GPR_NUM_OPSETS][27] = {
+    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26},
+    {GPR_REF_ONE, GPR_REF_CONST, GPR_REF_INV, GPR_REF_A, GPR_REF_B, GPR_REF_PLUS, GPR_REF_MINUS, GPR_REF_RMINUS,
+     GPR_REF_CPLUS, GPR_REF_TIMES, GPR_REF_CTIMES, GPR_REF_DIV, GPR_REF_RDIV},
+    {GPR_REF_ONE, GPR_REF_CONST, GPR_REF_A, GPR_REF_B, GPR_REF_PLUS, GPR_REF_MINUS, GPR_REF_RMINUS, GPR_REF_CPLUS,
+     GPR_REF_TIMES, GPR_REF_CTIMES, GPR_REF_CPOW},
+    {GPR_REF_ONE, GPR_REF_CONST, GPR_REF_A, GPR_REF_B, GPR_REF_PLUS, GPR_REF_MINUS, GPR_REF_RMINUS, GPR_REF_CPLUS,
+     GPR_REF_CTIMES},
+    {GPR_REF_LOGOF2, GPR_REF_HALF, GPR_REF_E, GPR_REF_PI, GPR_REF_CLIGHT, GPR_REF_G, GPR_REF_HBAR, GPR_REF_MU0,
+     GPR_REF_EL, GPR_REF_MP, GPR_REF_ME, GPR_REF_E0, GPR_REF_KB, GPR_REF_A, GPR_REF_B, GPR_REF_PLUS, GPR_REF_MINUS,
+     GPR_REF_RMINUS, GPR_REF_TIMES}};
+static bool is_physical(int o) { return o == GPR_REF_LOGOF2 || (o >= GPR_REF_E && o <= GPR_REF_KB); }
+
 namespace gpr_host {
 
 bool load_data_file(const char *path, float sdscale, bool first_half, bool even_half, std::vector<Datum> &out,
@@ -89,6 +105,19 @@ static std::string heap_str_op(int o, const std::string &a, const std::string &b
     case GPR_REF_RPOW: return "(" + b + "^" + a + ")";
     case GPR_REF_CRPOW: return "(" + C + "^" + a + ")";
     case GPR_REF_ABS: return "abs(" + a + ")";
+    case GPR_REF_E: return "E";
+    case GPR_REF_PI: return "PI";
+    case GPR_REF_CLIGHT: return "CLIGHT";
+    case GPR_REF_G: return "G";
+    case GPR_REF_HBAR: return "HBAR";
+    case GPR_REF_MU0: return "MU0";
+    case GPR_REF_EL: return "EL";
+    case GPR_REF_MP: return "MP";
+    case GPR_REF_ME: return "ME";
+    case GPR_REF_E0: return "E0";
+    case GPR_REF_KB: return "KB";
+    case GPR_REF_LOGOF2: return "log(2.0)";
+    case GPR_REF_HALF: return "(" + a + "/2.0)";
     default: return "NAN";
     }
 }
@@ -122,6 +151,19 @@ static std::string heap_tex_op(int o, const std::string &a, const std::string &b
     case GPR_REF_RPOW: return b + "^{" + a + "}";
     case GPR_REF_CRPOW: return C + "^{" + a + "}";
     case GPR_REF_ABS: return "\\mid " + a + "\\mid ";
+    case GPR_REF_E: return "e";
+    case GPR_REF_PI: return "\\pi";
+    case GPR_REF_CLIGHT: return "c";
+    case GPR_REF_G: return "G";
+    case GPR_REF_HBAR: return "\\bar{h}";
+    case GPR_REF_MU0: return "\\mu_0";
+    case GPR_REF_EL: return "e_l";
+    case GPR_REF_MP: return "m_p";
+    case GPR_REF_ME: return "m_e";
+    case GPR_REF_E0: return "\\epsilon_0";
+    case GPR_REF_KB: return "k_B";
+    case GPR_REF_LOGOF2: return "\\log 2)";               // unbalanced as in main.cu:947
+    case GPR_REF_HALF: return "(" + a + "/2.0)";
     default: return "NAN";
     }
 }
@@ -156,8 +198,9 @@ float heap_length(const uint8_t *P) {  // dispatch_length fold, main.cu:477-526
         case GPR_REF_POW: v = 1.0f + a + b; break;
         case GPR_REF_RMINUS: case GPR_REF_RPOW: v = 1.0f + b + a; break;
         case GPR_REF_INV: case GPR_REF_SQRT: case GPR_REF_LOG: case GPR_REF_SIN: case GPR_REF_ASIN: case GPR_REF_COS:
-        case GPR_REF_ACOS: case GPR_REF_TAN: case GPR_REF_ATAN: case GPR_REF_EXP: case GPR_REF_ABS: v = 1.0f + a; break;
-        default: v = NAN; break;
+        case GPR_REF_ACOS: case GPR_REF_TAN: case GPR_REF_ATAN: case GPR_REF_EXP: case GPR_REF_ABS:
+        case GPR_REF_HALF: v = 1.0f + a; break;
+        default: v = is_physical(P[i - 1]) ? 1.0f : NAN; break;   // PHYSICAL_CONSTANT_LENGTH_PRIOR
         }
         buf[i] = v;
     }
@@ -182,17 +225,34 @@ float heap_degree(const uint8_t *P, const float *C) {  // dispatch_degree fold, 
         case GPR_REF_CPOW: v = a * c; break;
         case GPR_REF_POW: v = b == 0.0f ? a * c : INFINITY; break;
         case GPR_REF_DIV: v = b == 0.0f ? a : INFINITY; break;
-        default: v = INFINITY; break;
+        case GPR_REF_HALF: v = a; break;
+        default: v = is_physical(P[i - 1]) ? 0.0f : INFINITY; break;
         }
         buf[i] = v;
     }
     return buf[1];
 }
 
-std::string heap_code(const uint8_t *P) {
-    static const char *code = GPR_REF_PROGRAM_CODE;
+// PROGRAM_CODE of each op set (main.cu:55, 61, 67, 74, 85), indexed by enum value
+const char *opset_program_code(int set) {
+    switch (set) {
+    case GPR_OPS_SIMPLIFIED: return "1CIab+-_#*@/|";
+    case GPR_OPS_POLYNOMIAL: return "1CAB+-_#*@^";
+    case GPR_OPS_LINEAR: return "1CAB+-_#@";
+    case GPR_OPS_PHYSICAL: return "B2epcghm:o.~Kab+-_*";
+    default: return GPR_REF_PROGRAM_CODE;
+    }
+}
+std::string heap_code(const uint8_t *P, int set) {
+    if (set < 0 || set >= GPR_NUM_OPSETS) set = GPR_OPS_DEFAULT;
+    const char *code = opset_program_code(set);
     std::string s;
-    for (int i = 0; i < GPR_REF_SLOTS; i++) s += P[i] < GPR_REF_NOPS ? code[P[i]] : '?';
+    for (int i = 0; i < GPR_REF_SLOTS; i++) {
+        char ch = '?';                                  // a code outside the op set
+        for (int k = 0; k < kOpsetN[set]; k++)
+            if (kOpset[set][k] == P[i]) ch = code[k];
+        s += ch;
+    }
     return s;
 }
 

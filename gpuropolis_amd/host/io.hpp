@@ -25,7 +25,8 @@ std::string heap_string(const uint8_t *P, const float *C, const char *const_form
 std::string heap_latex(const uint8_t *P, const float *C, const char *const_format);    // display_latex
 float heap_length(const uint8_t *P);                                                   // dispatch_length fold
 float heap_degree(const uint8_t *P, const float *C);                                   // polynomial_degree
-std::string heap_code(const uint8_t *P);                                               // PROGRAM_CODE chars
+std::string heap_code(const uint8_t *P, int opset = 0);                               // PROGRAM_CODE chars (op set)
+const char *opset_program_code(int opset);                                            // main.cu:55-85
 
 // Postfix program display (config semantics).
 std::string tape_string(const uint8_t *ops, const float *cs, int len, const char *const_format);

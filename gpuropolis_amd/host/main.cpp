@@ -52,6 +52,7 @@ struct Options {
     int struct_every = 5;
     int fixed_structure = 0;
     int rng = GPR_RNG_PHILOX;
+    int ops = GPR_OPS_DEFAULT;
     int resample = -1;
     double resample_nats = -1.0;
     float resample_quantile = -1.0f;
@@ -62,7 +63,7 @@ struct Options {
 enum {
     OPT_SEMANTICS = 1000, OPT_CAP, OPT_DMAX, OPT_NSYM, OPT_SCHEDULE, OPT_PROPOSAL, OPT_SIGMA, OPT_EPS,
     OPT_STRUCT_EVERY, OPT_FIXED, OPT_RNG, OPT_RESAMPLE, OPT_NATS, OPT_QUANTILE, OPT_NOLOG, OPT_STATE_IN,
-    OPT_STATE_OUT, OPT_HELP
+    OPT_STATE_OUT, OPT_OPS, OPT_HELP
 };
 
 const struct option kLongOptions[] = {
@@ -94,6 +95,7 @@ const struct option kLongOptions[] = {
     {"struct-every", required_argument, nullptr, OPT_STRUCT_EVERY},
     {"fixed-structure", no_argument, nullptr, OPT_FIXED},
     {"rng", required_argument, nullptr, OPT_RNG},
+    {"ops", required_argument, nullptr, OPT_OPS},
     {"resample", required_argument, nullptr, OPT_RESAMPLE},
     {"resample-nats", required_argument, nullptr, OPT_NATS},
     {"resample-quantile", required_argument, nullptr, OPT_QUANTILE},
@@ -110,6 +112,7 @@ const char *kUsage =
     "  additions:      [--semantics reference|config] [--cap 64] [--dmax 16] [--nsym 10]\n"
     "                  [--schedule auto|lane|wave] [--proposal rw|mala] [--sigma S] [--eps E]\n"
     "                  [--struct-every 5] [--fixed-structure] [--rng philox|lcg]\n"
+    "                  [--ops default|simplified|polynomial|linear|physical]\n"
     "                  [--resample none|map|quantile] [--resample-nats 100] [--resample-quantile 0.5]\n"
     "                  [--no-log] [--state-in FILE] [--state-out FILE]\n";
 
@@ -163,6 +166,12 @@ int parse(int argc, char **argv, Options &o) {
         case OPT_FIXED: o.fixed_structure = 1; break;
         case OPT_RNG:
             if (!pick(optarg, {{"philox", GPR_RNG_PHILOX}, {"lcg", GPR_RNG_LCG}}, o.rng)) return 1;
+            break;
+        case OPT_OPS:
+            if (!pick(optarg, {{"default", GPR_OPS_DEFAULT}, {"simplified", GPR_OPS_SIMPLIFIED},
+                               {"polynomial", GPR_OPS_POLYNOMIAL}, {"linear", GPR_OPS_LINEAR},
+                               {"physical", GPR_OPS_PHYSICAL}}, o.ops))
+                return 1;
             break;
         case OPT_RESAMPLE:
             if (!pick(optarg, {{"none", GPR_RS_NONE}, {"map", GPR_RS_MAP}, {"quantile", GPR_RS_QUANTILE}}, o.resample))
@@ -238,7 +247,7 @@ void write_rows(FILE *fp, const Options &o, int block, int rows, const std::vect
         if (ref) {
             fprintf(fp, "\"%s\"\t\"%s\"\t\"%s\"\t\"%s\"\t", heap_string(P.data(), C.data(), "C").c_str(),
                     heap_string(P.data(), C.data(), "%.10f").c_str(), heap_latex(P.data(), C.data(), "%.2f").c_str(),
-                    heap_code(P.data()).c_str());
+                    heap_code(P.data(), o.ops).c_str());
         } else {
             fprintf(fp, "\"%s\"\t\"%s\"\t\"%s\"\t\"%s\"\t", tape_string(P.data(), C.data(), L, "C").c_str(),
                     tape_string(P.data(), C.data(), L, "%.10f").c_str(),
@@ -318,6 +327,7 @@ int main(int argc, char **argv) {
     cfg.struct_every = o.struct_every;
     cfg.fixed_structure = o.fixed_structure;
     cfg.rng = o.rng;
+    cfg.ops = o.ops;
     cfg.resample = o.resample >= 0 ? o.resample : (ref ? GPR_RS_QUANTILE : GPR_RS_MAP);
     if (o.resample_nats >= 0) cfg.resample_nats = o.resample_nats;
     if (o.resample_quantile >= 0) cfg.resample_quantile = o.resample_quantile;

@@ -17,7 +17,8 @@ OPS = ["X", "CONST", "ONE", "ADD", "SUB", "MUL", "DIV", "POW", "RSUB", "RDIV", "
 OP = {n: i for i, n in enumerate(OPS)}
 REF_OPS = ["ONE", "CONST", "INV", "A", "B", "PLUS", "MINUS", "RMINUS", "CPLUS", "TIMES", "CTIMES", "DIV",
            "RDIV", "LOG", "EXP", "POW", "CPOW", "RPOW", "CRPOW", "SQRT", "SIN", "ASIN", "COS", "ACOS",
-           "TAN", "ATAN", "ABS"]
+           "TAN", "ATAN", "ABS", "LOGOF2", "HALF", "E", "PI", "CLIGHT", "G", "HBAR", "MU0", "EL", "MP", "ME",
+           "E0", "KB"]
 ROP = {n: i for i, n in enumerate(REF_OPS)}
 
 SEM_CONFIG, SEM_REFERENCE = 0, 1
@@ -25,13 +26,14 @@ SCHED_AUTO, SCHED_LANE, SCHED_WAVE = 0, 1, 2
 PROP_RW, PROP_MALA = 0, 1
 RNG_PHILOX, RNG_LCG = 0, 1
 RS_NONE, RS_MAP, RS_QUANTILE = 0, 1, 2
+OPS_DEFAULT, OPS_SIMPLIFIED, OPS_POLYNOMIAL, OPS_LINEAR, OPS_PHYSICAL = 0, 1, 2, 3, 4
 MAX_CAP, MAX_DEPTH = 128, 16
 
 EXPORTS = ["gpr_config_default", "gpr_create", "gpr_destroy", "gpr_last_error", "gpr_version", "gpr_set_data",
            "gpr_init_chains", "gpr_write_state", "gpr_read_state", "gpr_run_block", "gpr_run_block_async",
            "gpr_resample", "gpr_run_blocks_graph", "gpr_get_stats", "gpr_reset_stats", "gpr_sync", "gpr_stream",
            "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws",
-           "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning", "gpr_set_metric"]
+           "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning", "gpr_set_metric", "gpr_ref_op_set"]
 
 
 class Config(C.Structure):
@@ -39,7 +41,7 @@ class Config(C.Structure):
                 ("nsym", C.c_int), ("proposal", C.c_int), ("sigma", C.c_float), ("eps", C.c_float),
                 ("struct_every", C.c_int), ("fixed_structure", C.c_int), ("schedule", C.c_int), ("rng", C.c_int),
                 ("resample", C.c_int), ("resample_nats", C.c_double), ("resample_quantile", C.c_float),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("ops", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -98,6 +100,7 @@ def load():
         "gpr_set_tuning": (C.c_int, [vp, C.POINTER(Tune), C.c_int, C.c_int]),
         "gpr_get_tuning": (C.c_int, [vp, C.POINTER(Tune), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "gpr_ref_decode": (C.c_int, [vp, vp, C.c_size_t, vp, vp]),
+        "gpr_ref_op_set": (C.c_int, [C.c_int, vp, C.c_int, C.POINTER(C.c_char_p)]),
         "gpr_test_ref_decode": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, vp]),
         "gpr_test_draws": (C.c_int, [vp, vp, C.c_int, C.c_uint64, vp]),
     }
@@ -258,6 +261,14 @@ class Context:
 
     def grad_logpost_dev(self, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp):
         check(load().gpr_grad_logpost(self.h, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp))
+
+
+def ref_op_set(ops):
+    """(heap codes of op set `ops` in its enum order, its PROGRAM_CODE)."""
+    codes = np.zeros(32, dtype=np.uint8)
+    pc = C.c_char_p()
+    n = check(load().gpr_ref_op_set(ops, _p(codes), 32, C.byref(pc)))
+    return [int(c) for c in codes[:n]], pc.value.decode()
 
 
 def ref_decode(hop, hc):

@@ -35,16 +35,34 @@ enum {
 };
 
 /* Heap op codes of the reference's default op set, enum OPS at main.cu:53, with the
- * 27-character PROGRAM_CODE of main.cu:55. */
+ * 27-character PROGRAM_CODE of main.cu:55.  Codes 27..39 are the further ops of the
+ * PHYSICAL_CONSTANTS op set (main.cu:78-86): log 2, halving and eleven physical
+ * constants (their logs, main.cu:429-441).  Heap programs hold these codes for every op
+ * set; an op set only chooses which codes are drawn (gpr_ref_op_set). */
 enum {
     GPR_REF_ONE = 0, GPR_REF_CONST, GPR_REF_INV, GPR_REF_A, GPR_REF_B, GPR_REF_PLUS, GPR_REF_MINUS,
     GPR_REF_RMINUS, GPR_REF_CPLUS, GPR_REF_TIMES, GPR_REF_CTIMES, GPR_REF_DIV, GPR_REF_RDIV,
     GPR_REF_LOG, GPR_REF_EXP, GPR_REF_POW, GPR_REF_CPOW, GPR_REF_RPOW, GPR_REF_CRPOW, GPR_REF_SQRT,
     GPR_REF_SIN, GPR_REF_ASIN, GPR_REF_COS, GPR_REF_ACOS, GPR_REF_TAN, GPR_REF_ATAN, GPR_REF_ABS,
-    GPR_REF_NOPS
+    GPR_REF_NOPS,                   /* 27: size of the default op set */
+    GPR_REF_LOGOF2 = GPR_REF_NOPS, GPR_REF_HALF, GPR_REF_E, GPR_REF_PI, GPR_REF_CLIGHT, GPR_REF_G,
+    GPR_REF_HBAR, GPR_REF_MU0, GPR_REF_EL, GPR_REF_MP, GPR_REF_ME, GPR_REF_E0, GPR_REF_KB,
+    GPR_REF_NCODES
 };
 #define GPR_REF_SLOTS 31            /* PROGRAM_LENGTH = NCONSTANTS, main.cu:26-27 */
 #define GPR_REF_PROGRAM_CODE "1CIab+-_#*@/|LE^pVPRSsCcTtA"
+
+/* Op sets of the reference, chosen at compile time there (main.cu:51-86) and at run
+ * time here (gpr_config.ops, reference semantics): which heap codes the sampler draws,
+ * NOPS for the prior, and the PROGRAM_CODE characters. */
+enum {
+    GPR_OPS_DEFAULT = 0,            /* main.cu:51-56, 27 ops */
+    GPR_OPS_SIMPLIFIED = 1,         /* SIMPLIFIED_OPS, main.cu:58-62, 13 ops */
+    GPR_OPS_POLYNOMIAL = 2,         /* POLYNOMIAL_OPS, main.cu:64-68, 11 ops */
+    GPR_OPS_LINEAR = 3,             /* LINEAR_OPS, main.cu:71-75, 9 ops */
+    GPR_OPS_PHYSICAL = 4,           /* PHYSICAL_CONSTANTS, main.cu:78-86, 19 ops */
+    GPR_NUM_OPSETS
+};
 
 /* ---------------------------------------------------------------- configuration */
 enum { GPR_SEM_CONFIG = 0, GPR_SEM_REFERENCE = 1 };
@@ -77,6 +95,7 @@ typedef struct {
     double resample_nats;       /* MAP rule: resample chains below MAP - resample_nats */
     float resample_quantile;    /* quantile rule: RESAMPLE_QUANTILE, main.cu:36 */
     uint64_t seed;              /* Philox key */
+    int ops;                    /* GPR_OPS_*: op set of reference semantics (main.cu:51-86) */
 } gpr_config;
 
 typedef struct {
@@ -196,6 +215,11 @@ int gpr_grad_logpost(gpr_ctx *ctx, int dataset, const uint8_t *d_ops, const floa
 /* Heap-31 -> postfix decode of the reachable tree (program_fold, main.cu:371-393):
  * ops/cs receive up to 63 tokens; returns the token count or a negative error. */
 int gpr_ref_decode(const uint8_t *hop, const float *hc, size_t stride, uint8_t *ops, float *cs);
+
+/* Op set `set` (GPR_OPS_*): its NOPS heap codes in the op set's enum order (main.cu:53,
+ * 59, 65, 72, 82), and in program_code (may be NULL) its PROGRAM_CODE string, one
+ * character per enum value.  Returns NOPS or a negative error. */
+int gpr_ref_op_set(int set, uint8_t *codes, int max_codes, const char **program_code);
 
 /* ---------------------------------------------------------------- test hooks */
 /* No reference counterpart: device-side pieces exposed so tests can compare them with

@@ -216,7 +216,62 @@ static float ref_dispatch_eval(int o, float a, float b, float C) {
     case GPO_R_RPOW: return powf(b, a);
     case GPO_R_CRPOW: return powf(C, a);
     case GPO_R_ABS: return fabsf(a);
+    /* PHYSICAL_CONSTANTS, main.cu:429-441: the data_t value of each double literal */
+    case GPO_R_E: return (float)1.6021766208e-19;
+    case GPO_R_PI: return (float)1.14472988584915;
+    case GPO_R_CLIGHT: return (float)19.5186009865452;
+    case GPO_R_G: return (float)-23.4302046558627;
+    case GPO_R_HBAR: return (float)-78.2347583540371;
+    case GPO_R_MU0: return (float)-13.5870714043359;
+    case GPO_R_EL: return (float)-43.2777536741305;
+    case GPO_R_MP: return (float)-61.6554051167679;
+    case GPO_R_ME: return (float)-69.1708328401338;
+    case GPO_R_E0: return (float)-25.4501305691713;
+    case GPO_R_KB: return (float)-52.6369038080531;
+    case GPO_R_LOGOF2: return (float)log(2.0);
+    case GPO_R_HALF: return (float)((double)a / 2.0);
     default: return NAN;
+    }
+}
+static int ref_is_physical(int o) { return o == GPO_R_LOGOF2 || (o >= GPO_R_E && o <= GPO_R_KB); }
+
+/* op sets, main.cu:51-86: the heap code of each enum value in enum order; the
+ * SIMPLIFIED, POLYNOMIAL and LINEAR sets are the intended ones (their #if blocks do not
+ * compile as written: duplicate enumerators, a missing comma, a second enum OPS). */
+static const uint8_t OPSET_SIMPLIFIED[13] = { GPO_R_ONE, GPO_R_CONST, GPO_R_INV, GPO_R_A, GPO_R_B, GPO_R_PLUS,
+    GPO_R_MINUS, GPO_R_RMINUS, GPO_R_CPLUS, GPO_R_TIMES, GPO_R_CTIMES, GPO_R_DIV, GPO_R_RDIV };
+static const uint8_t OPSET_POLYNOMIAL[11] = { GPO_R_ONE, GPO_R_CONST, GPO_R_A, GPO_R_B, GPO_R_PLUS, GPO_R_MINUS,
+    GPO_R_RMINUS, GPO_R_CPLUS, GPO_R_TIMES, GPO_R_CTIMES, GPO_R_CPOW };
+static const uint8_t OPSET_LINEAR[9] = { GPO_R_ONE, GPO_R_CONST, GPO_R_A, GPO_R_B, GPO_R_PLUS, GPO_R_MINUS,
+    GPO_R_RMINUS, GPO_R_CPLUS, GPO_R_CTIMES };
+static const uint8_t OPSET_PHYSICAL[19] = { GPO_R_LOGOF2, GPO_R_HALF, GPO_R_E, GPO_R_PI, GPO_R_CLIGHT, GPO_R_G,
+    GPO_R_HBAR, GPO_R_MU0, GPO_R_EL, GPO_R_MP, GPO_R_ME, GPO_R_E0, GPO_R_KB, GPO_R_A, GPO_R_B, GPO_R_PLUS,
+    GPO_R_MINUS, GPO_R_RMINUS, GPO_R_TIMES };
+int gpo_opset_nops(int set) {
+    switch (set) {
+    case 1: return 13;
+    case 2: return 11;
+    case 3: return 9;
+    case 4: return 19;
+    default: return GPO_R_NOPS;
+    }
+}
+int gpo_opset_code(int set, int i) {
+    switch (set) {
+    case 1: return OPSET_SIMPLIFIED[i];
+    case 2: return OPSET_POLYNOMIAL[i];
+    case 3: return OPSET_LINEAR[i];
+    case 4: return OPSET_PHYSICAL[i];
+    default: return i;
+    }
+}
+const char *gpo_opset_program_code(int set) {     /* main.cu:55, 61, 67, 74, 85 */
+    switch (set) {
+    case 1: return "1CIab+-_#*@/|";
+    case 2: return "1CAB+-_#*@^";
+    case 3: return "1CAB+-_#@";
+    case 4: return "B2epcghm:o.~Kab+-_*";
+    default: return "1CIab+-_#*@/|LE^pVPRSsCcTtA";
     }
 }
 
@@ -232,13 +287,14 @@ static float ref_dispatch_length(int o, float a, float b) {
     case GPO_R_RMINUS: case GPO_R_RPOW: return 1.0f + b + a;
     case GPO_R_INV: case GPO_R_SQRT: case GPO_R_LOG: case GPO_R_SIN: case GPO_R_ASIN:
     case GPO_R_COS: case GPO_R_ACOS: case GPO_R_TAN: case GPO_R_ATAN: case GPO_R_EXP:
-    case GPO_R_ABS: return 1.0f + a;
-    default: return NAN;
+    case GPO_R_ABS: case GPO_R_HALF: return 1.0f + a;
+    default: return ref_is_physical(o) ? 1.0f : NAN;       /* PHYSICAL_CONSTANT_LENGTH_PRIOR */
     }
 }
 
 /* dispatch_countnodes, main.cu:553-601 */
 static int ref_dispatch_count(int o, int a, int b) {
+    if (ref_is_physical(o)) return 1;
     switch (o) {
     case GPO_R_ONE: case GPO_R_CONST: return 1;
     case GPO_R_A: return a;
@@ -264,7 +320,8 @@ static float ref_dispatch_degree(int o, float a, float b, float c) {
     case GPO_R_CPOW: return a * c;
     case GPO_R_POW: return (b == 0.0f) ? a * c : INFINITY;
     case GPO_R_DIV: return (b == 0.0f) ? a : INFINITY;
-    default: return INFINITY;
+    case GPO_R_HALF: return a;
+    default: return ref_is_physical(o) ? 0.0f : INFINITY;
     }
 }
 
@@ -320,10 +377,14 @@ int gpo_ref_is_descendant(int n, int k) {
 
 /* heap -> postfix of the reachable tree.  Value-identical to the fold because the
  * fold's root depends only on reachable slots. */
-static const int8_t REF_TO_TAPE[GPO_R_NOPS] = {
+static const int8_t REF_TO_TAPE[GPO_R_NCODES] = {
     GPO_ONE, GPO_CONST, GPO_INV, -1, -1, GPO_ADD, GPO_SUB, GPO_RSUB, GPO_CADD, GPO_MUL, GPO_CMUL,
     GPO_DIV, GPO_RDIV, GPO_LOG, GPO_EXP, GPO_POW, GPO_CPOW, GPO_RPOW, GPO_CRPOW, GPO_SQRT,
-    GPO_SIN, GPO_ASIN, GPO_COS, GPO_ACOS, GPO_TAN, GPO_ATAN, GPO_ABS };
+    GPO_SIN, GPO_ASIN, GPO_COS, GPO_ACOS, GPO_TAN, GPO_ATAN, GPO_ABS,
+    /* log 2 and the physical constants are constant tokens of their value; halving is
+     * c*a with c = 0.5 (a/2.0 in double, then float, equals 0.5f*a) */
+    GPO_CONST, GPO_CMUL, GPO_CONST, GPO_CONST, GPO_CONST, GPO_CONST, GPO_CONST, GPO_CONST, GPO_CONST,
+    GPO_CONST, GPO_CONST, GPO_CONST, GPO_CONST };
 
 static int decode_rec(const uint8_t *hop, const float *hc, size_t stride, int i, int pos,
                       uint8_t *ops, float *cs, uint8_t *slot) {
@@ -338,6 +399,8 @@ static int decode_rec(const uint8_t *hop, const float *hc, size_t stride, int i,
     if (ar == 2) pos = decode_rec(hop, hc, stride, 2 * i + 1, pos, ops, cs, slot);
     ops[pos] = (uint8_t)t;
     cs[pos] = (t == GPO_CONST || t >= GPO_CADD) ? hc[(i - 1) * stride] : 0.0f;
+    if (ref_is_physical(o)) cs[pos] = ref_dispatch_eval(o, 0.0f, 0.0f, 0.0f);
+    if (o == GPO_R_HALF) cs[pos] = 0.5f;
     slot[pos] = (uint8_t)(i - 1);
     return pos + 1;
 }
@@ -374,11 +437,16 @@ double gpo_ref_likelihood(const uint8_t *hop, const float *hc, size_t stride,
 }
 
 /* compute_prior, main.cu:529-544 (PRIOR_MULTIPLIER = CONSTANT_SCALE = 1) */
-double gpo_ref_prior(const uint8_t *hop, const float *hc, size_t stride) {
+double gpo_ref_prior_ops(const uint8_t *hop, const float *hc, size_t stride, int set) {
     float length = gpo_ref_fold_length(hop, stride);
-    double prior = -(double)(1.0f * length) * LOG28_D;
+    /* log((bayes_t)NOPS+1), main.cu:536; the default set's is the literal LOG28_D */
+    const double lnops = set == 0 ? LOG28_D : log((double)gpo_opset_nops(set) + 1.0);
+    double prior = -(double)(1.0f * length) * lnops;
     for (int c = 0; c < GPO_HEAP; c++) prior += gpo_ref_lcauchypdf(hc[c * stride], 1.0f);
     return prior;
+}
+double gpo_ref_prior(const uint8_t *hop, const float *hc, size_t stride) {
+    return gpo_ref_prior_ops(hop, hc, stride, 0);
 }
 
 /* ------------------------------------------------------------------ postfix tapes */
@@ -1056,7 +1124,7 @@ static void ref_run_chain(const gpo_refcfg *cfg, uint32_t chain, uint64_t step0,
                           int32_t random_seed, uint8_t *P, float *C,
                           const float *x, const float *y, const float *sd, int n,
                           double *prior_out, double *ll_out) {
-    double cur_prior = gpo_ref_prior(P, C, 1);
+    double cur_prior = gpo_ref_prior_ops(P, C, 1, cfg->ops);
     double cur_ll = gpo_ref_likelihood(P, C, 1, x, y, sd, n);
     double cur_post = cur_prior + cur_ll;
     uint32_t rx = (uint32_t)random_seed + chain;         /* main.cu:691 */
@@ -1071,10 +1139,11 @@ static void ref_run_chain(const gpo_refcfg *cfg, uint32_t chain, uint64_t step0,
             int nn = !ref_valid(cur_post) ? 0 : dev_random_int(GPO_HEAP, &s);
             for (int i = nn; i < GPO_HEAP; i++) {
                 old_P[i] = P[i];
-                if (gpo_ref_is_descendant(nn, i)) P[i] = (uint8_t)dev_random_int(GPO_R_NOPS, &s);
+                if (gpo_ref_is_descendant(nn, i))
+                    P[i] = (uint8_t)gpo_opset_code(cfg->ops, dev_random_int(gpo_opset_nops(cfg->ops), &s));
             }
             backward = -(double)gpo_logf((float)gpo_ref_fold_count(P, 1));
-            double pp = gpo_ref_prior(P, C, 1);
+            double pp = gpo_ref_prior_ops(P, C, 1, cfg->ops);
             double pl = gpo_ref_likelihood(P, C, 1, x, y, sd, n);
             double ppost = pp + pl;
             double acc = ppost - cur_post + backward - forward;
@@ -1108,7 +1177,7 @@ static void ref_run_chain(const gpo_refcfg *cfg, uint32_t chain, uint64_t step0,
                     C[c] = C[c] + 1.0f * POW10_F[order + 4] * dev_random_normal(&s);
                 }
             }
-            double pp = gpo_ref_prior(P, C, 1);
+            double pp = gpo_ref_prior_ops(P, C, 1, cfg->ops);
             double pl = gpo_ref_likelihood(P, C, 1, x, y, sd, n);
             double ppost = pp + pl;
             double acc = ppost - cur_post + backward - forward;
@@ -1141,19 +1210,20 @@ void gpo_ref_run_block(const gpo_refcfg *cfg, int nchains, uint32_t chain0, uint
 }
 
 /* initial draw, main.cu:1133-1137, with Philox in place of rand() */
-static void ref_draw_chain(src_t *s, uint8_t *hop, float *hc, size_t stride) {
-    for (int i = 0; i < GPO_HEAP; i++) hop[i * stride] = (uint8_t)host_random_int(GPO_R_NOPS - 1, s);
+static void ref_draw_chain(src_t *s, uint8_t *hop, float *hc, size_t stride, int set) {
+    for (int i = 0; i < GPO_HEAP; i++)
+        hop[i * stride] = (uint8_t)gpo_opset_code(set, host_random_int(gpo_opset_nops(set) - 1, s));
     for (int i = 0; i < GPO_HEAP; i++) {
         int order = host_random_int(8, s) - 4;
         hc[i * stride] = 1.0f * POW10_F[order + 4] * host_random_normal(s);
     }
 }
-void gpo_ref_init(uint64_t seed, int nchains, uint32_t chain0, uint8_t *hop, float *hc) {
-    gpo_refcfg cfg = { 0, seed };
+void gpo_ref_init(uint64_t seed, int nchains, uint32_t chain0, uint8_t *hop, float *hc, int set) {
+    gpo_refcfg cfg = { 0, seed, set };
     for (int c = 0; c < nchains; c++) {
         src_t s;
         src_sub(&s, &cfg, NULL, chain0 + (uint32_t)c, 0, GPO_P_INIT, 0);
-        ref_draw_chain(&s, hop + c, hc + c, (size_t)nchains);
+        ref_draw_chain(&s, hop + c, hc + c, (size_t)nchains, set);
     }
 }
 
@@ -1166,19 +1236,19 @@ static int cmp_ref(const void *pa, const void *pb) {
 }
 int gpo_ref_resample(uint64_t seed, uint32_t block, int nchains, uint32_t chain0,
                      uint8_t *hop, float *hc, const double *prior, const double *ll,
-                     double *cutoff_out) {
+                     double *cutoff_out, int set) {
     double *post = (double *)malloc(sizeof(double) * (size_t)nchains);
     for (int c = 0; c < nchains; c++) post[c] = prior[c] + ll[c];
     qsort(post, (size_t)nchains, sizeof(double), cmp_ref);
     double cutoff = post[(int)(nchains * 0.5f)];         /* RESAMPLE_QUANTILE, main.cu:36,1219 */
     free(post);
-    gpo_refcfg cfg = { 0, seed };
+    gpo_refcfg cfg = { 0, seed, set };
     int nre = 0;
     for (int c = 0; c < nchains; c++) {
         if (prior[c] + ll[c] < cutoff) {
             src_t s;
             src_sub(&s, &cfg, NULL, chain0 + (uint32_t)c, block, GPO_P_RESAMPLE, 0);
-            ref_draw_chain(&s, hop + c, hc + c, (size_t)nchains);
+            ref_draw_chain(&s, hop + c, hc + c, (size_t)nchains, set);
             nre++;
         }
     }

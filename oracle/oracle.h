@@ -37,7 +37,11 @@ enum {
     GPO_R_RMINUS, GPO_R_CPLUS, GPO_R_TIMES, GPO_R_CTIMES, GPO_R_DIV, GPO_R_RDIV,
     GPO_R_LOG, GPO_R_EXP, GPO_R_POW, GPO_R_CPOW, GPO_R_RPOW, GPO_R_CRPOW, GPO_R_SQRT,
     GPO_R_SIN, GPO_R_ASIN, GPO_R_COS, GPO_R_ACOS, GPO_R_TAN, GPO_R_ATAN, GPO_R_ABS,
-    GPO_R_NOPS
+    GPO_R_NOPS,
+    /* the further ops of the PHYSICAL_CONSTANTS op set, main.cu:82 */
+    GPO_R_LOGOF2 = GPO_R_NOPS, GPO_R_HALF, GPO_R_E, GPO_R_PI, GPO_R_CLIGHT, GPO_R_G, GPO_R_HBAR,
+    GPO_R_MU0, GPO_R_EL, GPO_R_MP, GPO_R_ME, GPO_R_E0, GPO_R_KB,
+    GPO_R_NCODES
 };
 #define GPO_HEAP 31
 
@@ -155,19 +159,27 @@ int gpo_cfg_resample(int nchains, uint32_t chain0, uint32_t block, uint64_t seed
 typedef struct {
     int rng;            /* 0 = Philox substreams, 1 = reference LCG */
     uint64_t seed;
+    int ops;            /* op set: 0 default, 1 simplified, 2 polynomial, 3 linear, 4 physical */
 } gpo_refcfg;
+
+/* op sets of main.cu:51-86: size NOPS, the heap code of enum value i, PROGRAM_CODE */
+int gpo_opset_nops(int set);
+int gpo_opset_code(int set, int i);
+const char *gpo_opset_program_code(int set);
+/* compute_prior with the op set's NOPS */
+double gpo_ref_prior_ops(const uint8_t *hop, const float *hc, size_t stride, int set);
 
 void gpo_ref_run_block(const gpo_refcfg *cfg, int nchains, uint32_t chain0, uint64_t step0, int steps,
                        int32_t random_seed, uint8_t *hop, float *hc,
                        const float *x, const float *y, const float *sd, int n,
                        double *prior, double *ll, int nthreads);
-void gpo_ref_init(uint64_t seed, int nchains, uint32_t chain0, uint8_t *hop, float *hc);
+void gpo_ref_init(uint64_t seed, int nchains, uint32_t chain0, uint8_t *hop, float *hc, int set);
 /* n draws of a reference transform (random_int/float/normal/geometric, host variants) */
 void gpo_ref_draws(int kind, int rng, uint64_t seed, uint32_t lcg_seed, int n, int arg, double *out);
 /* median rule of main.cu:1215-1231; returns the number of re-drawn chains */
 int gpo_ref_resample(uint64_t seed, uint32_t block, int nchains, uint32_t chain0,
                      uint8_t *hop, float *hc, const double *prior, const double *ll,
-                     double *cutoff_out);
+                     double *cutoff_out, int set);
 
 #ifdef __cplusplus
 }

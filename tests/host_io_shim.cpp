@@ -39,6 +39,7 @@ int shim_heap(const uint8_t *P, const float *C, int which, const char *cf, char 
     if (which == 1) return put(heap_latex(P, C, cf), buf, cap);
     return put(heap_code(P), buf, cap);
 }
+int shim_heap_code(const uint8_t *P, int opset, char *buf, int cap) { return put(heap_code(P, opset), buf, cap); }
 float shim_heap_length(const uint8_t *P) { return heap_length(P); }
 float shim_heap_degree(const uint8_t *P, const float *C) { return heap_degree(P, C); }
 

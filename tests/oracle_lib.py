@@ -18,7 +18,9 @@ OP = {n: i for i, n in enumerate(OPS)}
 # reference heap op codes, main.cu:53
 REF_OPS = ["ONE", "CONST", "INV", "A", "B", "PLUS", "MINUS", "RMINUS", "CPLUS", "TIMES", "CTIMES", "DIV",
            "RDIV", "LOG", "EXP", "POW", "CPOW", "RPOW", "CRPOW", "SQRT", "SIN", "ASIN", "COS", "ACOS",
-           "TAN", "ATAN", "ABS"]
+           "TAN", "ATAN", "ABS",
+           # the further ops of the PHYSICAL_CONSTANTS op set (main.cu:82)
+           "LOGOF2", "HALF", "E", "PI", "CLIGHT", "G", "HBAR", "MU0", "EL", "MP", "ME", "E0", "KB"]
 ROP = {n: i for i, n in enumerate(REF_OPS)}
 P_STRUCT, P_CPROP, P_ACCEPT, P_INIT, P_RESAMPLE = 1, 2, 3, 4, 5
 
@@ -42,7 +44,7 @@ class GpoCfg(C.Structure):
 
 
 class GpoRefCfg(C.Structure):
-    _fields_ = [("rng", C.c_int), ("seed", C.c_uint64)]
+    _fields_ = [("rng", C.c_int), ("seed", C.c_uint64), ("ops", C.c_int)]
 
 
 class GpoChainStats(C.Structure):
@@ -107,9 +109,13 @@ def lib():
         "gpo_ref_run_block": (None, [C.POINTER(GpoRefCfg), C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_int32,
                                      _u8p, _f32p, _f32p, _f32p, _f32p, C.c_int, _f64p, _f64p, C.c_int]),
         "gpo_ref_draws": (None, [C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_int, C.c_int, _f64p]),
-        "gpo_ref_init": (None, [C.c_uint64, C.c_int, C.c_uint32, _u8p, _f32p]),
+        "gpo_ref_init": (None, [C.c_uint64, C.c_int, C.c_uint32, _u8p, _f32p, C.c_int]),
         "gpo_ref_resample": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, _u8p, _f32p, _f64p, _f64p,
-                                       _f64p]),
+                                       _f64p, C.c_int]),
+        "gpo_opset_nops": (C.c_int, [C.c_int]),
+        "gpo_opset_code": (C.c_int, [C.c_int, C.c_int]),
+        "gpo_opset_program_code": (C.c_char_p, [C.c_int]),
+        "gpo_ref_prior_ops": (C.c_double, [_u8p, _f32p, C.c_size_t, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -277,11 +283,17 @@ def ref_run_block(refcfg, hop, hc, x, y, sd, step0, steps, random_seed=0, chain0
     return prior, ll
 
 
-def ref_init(seed, nchains, chain0=0):
+def ref_init(seed, nchains, chain0=0, ops=0):
     hop = np.zeros((31, nchains), dtype=np.uint8)
     hc = np.zeros((31, nchains), dtype=np.float32)
-    lib().gpo_ref_init(seed, nchains, chain0, u8(hop), f32(hc))
+    lib().gpo_ref_init(seed, nchains, chain0, u8(hop), f32(hc), ops)
     return hop, hc
+
+
+def opset(ops):
+    """(heap codes in enum order, PROGRAM_CODE) of op set `ops`."""
+    n = lib().gpo_opset_nops(ops)
+    return [lib().gpo_opset_code(ops, i) for i in range(n)], lib().gpo_opset_program_code(ops).decode()
 
 
 def ref_draws(kind, n, rng=0, seed=0, lcg_seed=0, arg=0):

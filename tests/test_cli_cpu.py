@@ -149,12 +149,29 @@ def test_heap_length_degree_code_match_oracle_folds(io):
     code = "1CIab+-_#*@/|LE^pVPRSsCcTtA"
     for _ in range(1000):
         P = rng.integers(0, 27, 31).astype(np.uint8)
+        if _ % 2:
+            P = rng.integers(0, len(O.REF_OPS), 31).astype(np.uint8)   # with the PHYSICAL_CONSTANTS ops
         Cs = rng.choice(np.array([0.0, 1.0, 2.0, 3.0, -1.0, 0.5], dtype=np.float32), 31)
         assert io.shim_heap_length(u8(P)) == O.lib().gpo_ref_fold_length(u8(P), 1)
         a = io.shim_heap_degree(u8(P), f32(Cs))
         b = O.lib().gpo_ref_fold_degree(u8(P), f32(Cs), 1)
         assert a == b or (np.isnan(a) and np.isnan(b))
-        assert heap_text(io, P, Cs, 2) == "".join(code[p] for p in P)
+        assert heap_text(io, P, Cs, 2) == "".join(code[p] if p < 27 else "?" for p in P)
+
+
+def test_heap_code_column_per_op_set(io):
+    """the code column prints each op set's PROGRAM_CODE character of the op's enum value;
+    heap codes outside the op set print '?'"""
+    rng = np.random.default_rng(6)
+    for ops in range(5):
+        codes, pc = O.opset(ops)
+        for _ in range(50):
+            P = rng.choice(np.array(codes, dtype=np.uint8), 31)
+            P[rng.integers(0, 31)] = [c for c in range(len(O.REF_OPS)) if c not in codes][0]
+            buf = C.create_string_buffer(64)
+            n = io.shim_heap_code(u8(P), ops, buf, 64)
+            want = "".join(pc[codes.index(p)] if p in codes else "?" for p in P)
+            assert buf.value.decode()[:n] == want
 
 
 def one_op_heap(op, c_root=2.5, c_left=1.5):
@@ -180,6 +197,11 @@ KNOWN = {
     "SIN": ("sin(1.50)", "sin(1.50)"), "ASIN": ("asin(1.50)", "asin(1.50)"), "COS": ("cos(1.50)", "cos(1.50)"),
     "ACOS": ("acos(1.50)", "acos(1.50)"), "TAN": ("tan(1.50)", "tan(1.50)"), "ATAN": ("atan(1.50)", "atan(1.50)"),
     "ABS": ("abs(1.50)", "\\mid 1.50\\mid "),
+    # PHYSICAL_CONSTANTS ops (main.cu:855-868, 935-948); the LaTeX of log 2 is unbalanced there
+    "E": ("E", "e"), "PI": ("PI", "\\pi"), "CLIGHT": ("CLIGHT", "c"), "G": ("G", "G"), "HBAR": ("HBAR", "\\bar{h}"),
+    "MU0": ("MU0", "\\mu_0"), "EL": ("EL", "e_l"), "MP": ("MP", "m_p"), "ME": ("ME", "m_e"),
+    "E0": ("E0", "\\epsilon_0"), "KB": ("KB", "k_B"), "LOGOF2": ("log(2.0)", "\\log 2)"),
+    "HALF": ("(1.50/2.0)", "(1.50/2.0)"),
 }
 
 

@@ -42,11 +42,17 @@ FN = {"asin": "np.arcsin", "acos": "np.arccos", "atan": "np.arctan", "log": "np.
       "sqrt": "np.sqrt", "sin": "np.sin", "cos": "np.cos", "tan": "np.tan", "abs": "np.abs"}
 
 
+# PHYSICAL_CONSTANTS names of the string column and their values (main.cu:429-439)
+PHYS = {"CLIGHT": 19.5186009865452, "HBAR": -78.2347583540371, "MU0": -13.5870714043359,
+        "E0": -25.4501305691713, "KB": -52.6369038080531, "EL": -43.2777536741305, "MP": -61.6554051167679,
+        "ME": -69.1708328401338, "PI": 1.14472988584915, "E": 1.6021766208e-19, "G": -23.4302046558627}
+
+
 def eval_expr(expr, x):
     e = NUM.sub(lambda m: "F(" + m.group(0) + ")", expr).replace("^", "**")
     e = re.sub(r"\b(asin|acos|atan|log|exp|sqrt|sin|cos|tan|abs)\(", lambda m: FN[m.group(1)] + "(", e)
     with np.errstate(all="ignore"):
-        v = eval(e, {"np": np, "x": x, "F": np.float64, "nan": np.nan, "inf": np.inf})
+        v = eval(e, {"np": np, "x": x, "F": np.float64, "nan": np.nan, "inf": np.inf, **PHYS})
     return np.broadcast_to(np.asarray(v, dtype=np.float64), x.shape)
 
 
@@ -56,7 +62,7 @@ def loglik(f, y, sd):
         return float(np.sum(-(r * r) / (2 * sd * sd) - 0.5 * (np.log(2 * np.pi) + 2 * np.log(sd))))
 
 
-def check_rows(rows, N, blocks, nconst, d, ref):
+def check_rows(rows, N, blocks, nconst, d, ref, code=CODE):
     assert len(rows) == N * len(blocks)
     x, y, sd = (d[:, i].astype(np.float64) for i in range(3))
     agree = total = 0
@@ -71,7 +77,7 @@ def check_rows(rows, N, blocks, nconst, d, ref):
         for s in f[7:11]:
             assert s[0] == '"' and s[-1] == '"'
         if ref:
-            assert len(f[10]) == 33 and all(ch in CODE for ch in f[10][1:-1])
+            assert len(f[10]) == 33 and all(ch in code for ch in f[10][1:-1])
         for c in f[11:11 + nconst]:
             assert re.fullmatch(r"-?\d+\.\d{10}", c) or c in ("nan", "-nan", "inf", "-inf")
         if np.isfinite(ll):
@@ -130,3 +136,18 @@ def test_cli_config_semantics_and_state_files(cli, tmp_path):
     r = subprocess.run([cli, "--in", DATA, "--out", str(tmp_path / "c"), "--quiet", "--semantics", "config",
                         "--cap", "64", "--N", "256", "--state-in", st], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "does not match" in r.stderr
+
+
+def test_cli_op_sets(cli, tmp_path):
+    """--ops chooses the reference's op set (main.cu:51-86): the code column prints the op
+    set's PROGRAM_CODE, and the printed programs (physical constants by name) reproduce
+    the printed likelihoods."""
+    d = np.loadtxt(DATA).astype(np.float32)
+    for name, code in (("physical", "B2epcghm:o.~Kab+-_*"), ("polynomial", "1CAB+-_#*@^")):
+        text = run(cli, tmp_path / name, "--ops", name, "--N", "256", "--steps", "100", "--outer", "3",
+                   "--seed", "3", "--show-constants")
+        rows = rows_of(text)
+        check_rows(rows, 256, [1, 2], 31, d, ref=True, code=code)
+        used = {ch for r in rows for ch in r.split("\t")[10][1:-1]}
+        say(name, "codes printed:", "".join(sorted(used)))
+        assert len(used) >= len(code) - 2

@@ -86,7 +86,74 @@ def test_ref_quantile_resample_matches_oracle(oracle, gpr):
     o_hop, o_hc = g_hop.copy(), g_hc.copy()
     cutoff = np.zeros(1)
     nre = oracle.lib().gpo_ref_resample(8, 1, nch, 0, oracle.u8(o_hop), oracle.f32(o_hc), oracle.f64(g_pr),
-                                        oracle.f64(g_ll), oracle.f64(cutoff))
+                                        oracle.f64(g_ll), oracle.f64(cutoff), 0)
     assert nre > 0
     assert np.array_equal(r_hop, o_hop)
     assert np.array_equal(r_hc.view(np.uint32), o_hc.view(np.uint32))
+
+
+# ------------------------------------------------------------------------ op sets
+@pytest.mark.parametrize("ops", [1, 2, 3, 4])
+def test_ref_op_set_init_and_resample_match_oracle(oracle, gpr, ops):
+    """Initial draws and the median rule's re-draws of each op set (main.cu:51-86) are
+    the oracle's, bit for bit."""
+    x, y, sd = data("S")
+    nch = 1001
+    hop, hc = oracle.ref_init(31, nch, ops=ops)
+    with gpr.Context(semantics=gpr.SEM_REFERENCE, seed=31, ops=ops, resample=gpr.RS_QUANTILE) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        say(f"op set {ops}: init and quantile resample")
+        ctx.init_chains()
+        i_hop, i_hc, _, _, _ = ctx.read_state()
+        ctx.run_block(10, 1)
+        g_hop, g_hc, _, g_pr, g_ll = ctx.read_state()
+        ctx.resample()
+        r_hop, r_hc, _, _, _ = ctx.read_state()
+    assert np.array_equal(i_hop, hop)
+    assert np.array_equal(i_hc.view(np.uint32), hc.view(np.uint32))
+    o_hop, o_hc = g_hop.copy(), g_hc.copy()
+    cutoff = np.zeros(1)
+    nre = oracle.lib().gpo_ref_resample(31, 1, nch, 0, oracle.u8(o_hop), oracle.f32(o_hc), oracle.f64(g_pr),
+                                        oracle.f64(g_ll), oracle.f64(cutoff), ops)
+    assert nre > 0
+    assert np.array_equal(r_hop, o_hop)
+    assert np.array_equal(r_hc.view(np.uint32), o_hc.view(np.uint32))
+
+
+@pytest.mark.parametrize("sched,ops", [("wave", 2), ("wave", 4), ("lane", 4), ("wave", 3), ("lane", 1)])
+def test_ref_op_set_trajectories_match_oracle(oracle, gpr, sched, ops):
+    """The sampler with each op set follows the oracle's trajectories: moves draw the op
+    set's codes, the prior uses its NOPS, and the physical constants and halving
+    evaluate as main.cu:429-441 on both schedules."""
+    x, y, sd = data("Misra1a")
+    nch, steps = 64, 200
+    hop, hc = oracle.ref_init(6, nch, ops=ops)
+    o_hop, o_hc = hop.copy(), hc.copy()
+    cfg = oracle.GpoRefCfg(0, 77, ops)
+    _, _ = oracle.ref_run_block(cfg, o_hop, o_hc, x, y, sd, 0, steps)
+    p1, l1 = oracle.ref_run_block(cfg, o_hop, o_hc, x, y, sd, steps, steps)
+    sch = gpr.SCHED_LANE if sched == "lane" else gpr.SCHED_WAVE
+    with gpr.Context(semantics=gpr.SEM_REFERENCE, seed=77, ops=ops, schedule=sch) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(hop, hc)
+        say(f"op set {ops} trajectories sched={sched}")
+        ctx.run_block(steps)
+        ctx.run_block(steps)
+        g_hop, g_hc, _, g_pr, g_ll = ctx.read_state()
+    same = np.all(g_hop == o_hop, axis=0) & np.all(g_hc.view(np.uint32) == o_hc.view(np.uint32), axis=0)
+    say("chains on the oracle path:", int(same.sum()), "of", nch)
+    assert same.sum() >= int(0.95 * nch)
+    c = np.flatnonzero(same)
+    bad = c[g_pr[c] != p1[c]]
+    for i in bad[:4]:
+        h = o_hop[:, i].copy()
+        say("prior mismatch chain", i, g_pr[i], p1[i], g_pr[i] - p1[i], "heap", list(h),
+            "length", oracle.lib().gpo_ref_fold_length(oracle.u8(h), 1))
+    assert len(bad) == 0
+    ok, diff, tol = ll_close(g_ll[c], l1[c], len(x))
+    kap = np.array([kappa(oracle, *oracle.ref_decode(o_hop[:, i].copy(), o_hc[:, i].copy())[:2], x, y, sd)
+                    for i in c])
+    assert np.all(ok | (np.isfinite(g_ll[c]) & (diff <= tol + 4 * kap)))
+    codes, _ = oracle.opset(ops)
+    assert set(np.unique(g_hop)) <= set(codes)
+

@@ -6,6 +6,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import oracle_lib as O
 
@@ -68,3 +69,11 @@ def test_errors_are_reported_not_fatal(gpr):
     bad = np.full(31, 200, dtype=np.uint8)
     assert L.gpr_ref_decode(bad.ctypes.data, np.zeros(31, np.float32).ctypes.data, 1,
                             np.zeros(64, np.uint8).ctypes.data, np.zeros(64, np.float32).ctypes.data) == -1
+
+
+def test_op_sets_need_reference_semantics(gpr):
+    """argument checks before any device call"""
+    with pytest.raises(gpr.GprError, match="reference semantics only"):
+        gpr.Context(semantics=gpr.SEM_CONFIG, ops=2)
+    with pytest.raises(gpr.GprError, match="op set"):
+        gpr.Context(semantics=gpr.SEM_REFERENCE, ops=7)

@@ -1,7 +1,7 @@
 """Pins of the oracle's program layer (CPU only).
 
 heap fold (program_fold, main.cu:371-393)  vs an independent recursive evaluation
-decode heap -> postfix                      vs the fold, bit-exact, all 27 ops
+decode heap -> postfix                      vs the fold, bit-exact, all 40 heap codes
 length / count (main.cu:477-608)            vs closed forms on the decoded tape
 is_descendant (main.cu:660-677)             vs brute-force subtree enumeration
 NIST StRD certified models                  residual sums of squares of the certified fits
@@ -72,11 +72,14 @@ def test_decode_matches_fold_all_ops():
 def test_length_and_count_closed_forms():
     rng = np.random.default_rng(2)
     L = O.lib()
-    cops = {O.OP[n] for n in ("CADD", "CMUL", "CPOW", "CRPOW")}
+    # constant-operand heap ops cost 2 (CONST_LENGTH_PRIOR + 1); halving decodes to a c* token
+    # but costs 1 + a, so C-op tokens are counted by their heap op
+    cops = {O.ROP[n] for n in ("CPLUS", "CTIMES", "CPOW", "CRPOW")}
     for _ in range(2000):
         hop, hc = random_heap(rng, O.REF_OPS)
-        ops, _, _ = O.ref_decode(hop, hc)
-        n_tok, n_x, n_c = len(ops), int(np.sum(ops == O.OP["X"])), int(sum(int(o) in cops for o in ops))
+        ops, _, slot = O.ref_decode(hop, hc)
+        n_tok, n_x = len(ops), int(np.sum(ops == O.OP["X"]))
+        n_c = int(sum(int(hop[s]) in cops for s in slot if s != 255))
         assert L.gpo_ref_fold_length(O.u8(hop), 1) == n_tok + n_c
         assert L.gpo_ref_fold_count(O.u8(hop), 1) == n_tok - n_x
 

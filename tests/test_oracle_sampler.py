@@ -260,7 +260,7 @@ def test_reference_median_resample_rule():
     ll[[3, 5, 8]] = np.nan
     cutoff = np.zeros(1)
     h2, c2 = hop.copy(), hc.copy()
-    nre = O.lib().gpo_ref_resample(9, 2, nch, 0, O.u8(h2), O.f32(c2), O.f64(prior), O.f64(ll), O.f64(cutoff))
+    nre = O.lib().gpo_ref_resample(9, 2, nch, 0, O.u8(h2), O.f32(c2), O.f64(prior), O.f64(ll), O.f64(cutoff), 0)
     post = prior + ll
     srt = np.concatenate([np.full(np.isnan(post).sum(), -np.inf), np.sort(post[~np.isnan(post)])])
     assert cutoff[0] == srt[int(nch * np.float32(0.5))]  # qsort with NaN first, main.cu:987-995,1219
