@@ -27,8 +27,9 @@
 // per CU), and the current structure's plan in registers.
 extern __shared__ float gpr_lds_slab[];
 
-// The plain one-point-per-lane kernel is latency bound: six waves per SIMD (80 VGPRs, a
-// few spilled values in its cold paths) run it faster than the five that fit unspilled.
+// The plain one-point-per-lane kernel is latency bound: six waves per SIMD (80 VGPRs) run
+// it faster than the four that fit unspilled, although the compiler then spills per-tile
+// values to scratch at every tile start and reloads some of them in the step loop.
 template <int KC, bool HI, bool MALA>
 __global__ void __launch_bounds__(GPR_WG) __attribute__((amdgpu_waves_per_eu((!MALA && KC == 1) ? 6 : 1)))
 k_mh_config_wave(GprMHArgs a) {
