@@ -325,7 +325,12 @@ k_mh_config_wave(GprMHArgs a) {
                 if (i >= 0 && __builtin_isfinite(post) && __builtin_isfinite(pprior) && __builtin_isfinite(extra))
                     reject_below = post - pprior - extra - 18.0 -
                                    1e-12 * (__builtin_fabs(post) + __builtin_fabs(pprior) + __builtin_fabs(extra));
-                pll = wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &pmask, reject_below, &pts);
+                // tapes within 64 tokens evaluate with the one-register interpreter (the high
+                // words of the tape and of its masks are zero)
+                if (HI && prop.len <= 64)
+                    pll = wave_loglik<KC, false, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &pmask, reject_below, &pts);
+                else
+                    pll = wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &pmask, reject_below, &pts);
                 pts = min(pts, d.n);
             }
             {
