@@ -1101,20 +1101,33 @@ static int ref_valid(double x) { return isfinite(x); }
 /* n sequential draws of one reference transform from one source (tests):
  * kind 0 random_int(arg), 1 random_float, 2 random_normal, 3 random_geometric(0.1),
  * 4 host random_int(arg), 5 host random_float, 6 host random_normal */
+static void ref_draws_from(src_t *s, int kind, int n, int arg, double *out);
 void gpo_ref_draws(int kind, int rng, uint64_t seed, uint32_t lcg_seed, int n, int arg, double *out) {
-    gpo_refcfg cfg = { rng, seed };
+    gpo_refcfg cfg = { rng, seed, 0 };
     uint32_t rx = lcg_seed;
     src_t s;
     src_sub(&s, &cfg, &rx, 0, 0, 0, 0);
+    ref_draws_from(&s, kind, n, arg, out);
+}
+/* the same draws from the Philox substream (chain, step, purpose, slot) that the sampler
+ * uses at that step, e.g. the accept draw or a constant's proposal draws */
+void gpo_ref_stream_draws(int kind, uint64_t seed, uint32_t chain, uint32_t step, uint32_t purpose,
+                          uint32_t slot, int n, int arg, double *out) {
+    gpo_refcfg cfg = { 0, seed, 0 };
+    src_t s;
+    src_sub(&s, &cfg, NULL, chain, step, purpose, slot);
+    ref_draws_from(&s, kind, n, arg, out);
+}
+static void ref_draws_from(src_t *s, int kind, int n, int arg, double *out) {
     for (int i = 0; i < n; i++) {
         switch (kind) {
-        case 0: out[i] = dev_random_int(arg, &s); break;
-        case 1: out[i] = dev_random_float(&s); break;
-        case 2: out[i] = dev_random_normal(&s); break;
-        case 3: out[i] = dev_random_geometric(&s); break;
-        case 4: out[i] = host_random_int(arg, &s); break;
-        case 5: out[i] = host_random_float(&s); break;
-        default: out[i] = host_random_normal(&s); break;
+        case 0: out[i] = dev_random_int(arg, s); break;
+        case 1: out[i] = dev_random_float(s); break;
+        case 2: out[i] = dev_random_normal(s); break;
+        case 3: out[i] = dev_random_geometric(s); break;
+        case 4: out[i] = host_random_int(arg, s); break;
+        case 5: out[i] = host_random_float(s); break;
+        default: out[i] = host_random_normal(s); break;
         }
     }
 }

@@ -151,11 +151,14 @@ int gpo_cfg_resample(int nchains, uint32_t chain0, uint32_t block, uint64_t seed
                      double *prior, double *ll, int32_t *src_out);
 
 /* ---------------- reference-mode sampler (main.cu:679-813)
- * Parity unpinned at the level of the reference's quirks (stale forward/backward sums,
- * the integer constant proposal, all-slot structure choice, main.cu:695-806): the
- * reference needs nvcc and cuda_runtime.h, so no golden trajectories of it exist here.
- * Pinned: its draw transforms (known answers, moments), its folds and densities
- * (closed forms), its init distribution and its median resampling rule. */
+ * The reference needs nvcc and cuda_runtime.h, so no golden trajectories of it exist
+ * here.  Pinned instead: its draw transforms (known answers, moments), its folds and
+ * densities (closed forms), its init distribution and its median resampling rule; its
+ * moves through closed forms (the forced-accept root structure proposal is uniform over
+ * the op set, the constant proposal is the 0.15 integer / 0.85 normal mixture, single
+ * leaf-slot changes follow q * P(u < expf(a)) with the -log count terms, a pure-x
+ * program never moves); and whole chains (stale forward/backward sums included) against
+ * a second, independent transcription of main.cu:683-806 in the tests. */
 typedef struct {
     int rng;            /* 0 = Philox substreams, 1 = reference LCG */
     uint64_t seed;
@@ -176,6 +179,9 @@ void gpo_ref_run_block(const gpo_refcfg *cfg, int nchains, uint32_t chain0, uint
 void gpo_ref_init(uint64_t seed, int nchains, uint32_t chain0, uint8_t *hop, float *hc, int set);
 /* n draws of a reference transform (random_int/float/normal/geometric, host variants) */
 void gpo_ref_draws(int kind, int rng, uint64_t seed, uint32_t lcg_seed, int n, int arg, double *out);
+/* the same transforms from the Philox substream (chain, step, purpose, slot) */
+void gpo_ref_stream_draws(int kind, uint64_t seed, uint32_t chain, uint32_t step, uint32_t purpose,
+                          uint32_t slot, int n, int arg, double *out);
 /* median rule of main.cu:1215-1231; returns the number of re-drawn chains */
 int gpo_ref_resample(uint64_t seed, uint32_t block, int nchains, uint32_t chain0,
                      uint8_t *hop, float *hc, const double *prior, const double *ll,

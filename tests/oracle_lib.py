@@ -109,6 +109,8 @@ def lib():
         "gpo_ref_run_block": (None, [C.POINTER(GpoRefCfg), C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_int32,
                                      _u8p, _f32p, _f32p, _f32p, _f32p, C.c_int, _f64p, _f64p, C.c_int]),
         "gpo_ref_draws": (None, [C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_int, C.c_int, _f64p]),
+        "gpo_ref_stream_draws": (None, [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_int, C.c_int, _f64p]),
         "gpo_ref_init": (None, [C.c_uint64, C.c_int, C.c_uint32, _u8p, _f32p, C.c_int]),
         "gpo_ref_resample": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, _u8p, _f32p, _f64p, _f64p,
                                        _f64p, C.c_int]),
@@ -299,4 +301,11 @@ def opset(ops):
 def ref_draws(kind, n, rng=0, seed=0, lcg_seed=0, arg=0):
     out = np.zeros(n, dtype=np.float64)
     lib().gpo_ref_draws(kind, rng, seed, lcg_seed, n, arg, f64(out))
+    return out
+
+
+def ref_stream_draws(kind, n, seed, chain, step, purpose, slot=0, arg=0):
+    """n draws of a reference transform from the Philox substream (chain, step, purpose, slot)."""
+    out = np.zeros(n, dtype=np.float64)
+    lib().gpo_ref_stream_draws(kind, seed, chain, step, purpose, slot, n, arg, f64(out))
     return out
