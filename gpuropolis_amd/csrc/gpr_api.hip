@@ -39,12 +39,18 @@ struct KernelBucket {
     bool hi;
     int kc, wpg, wgpc;
     int tile0, ntiles;
+    size_t gslab_off;                // this launch's MALA token slabs in d_gslab (floats)
 };
+// Size-class launches of one block run concurrently on these streams (at most 3 classes
+// x 2 register forms), forked from and joined back to the context stream.
+#define GPR_MAX_BUCKETS 6
 
 struct gpr_ctx {
     gpr_config cfg;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t side[GPR_MAX_BUCKETS] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[GPR_MAX_BUCKETS] = {};
     int nd = 0, N = 0, rows = 0;
     std::vector<GprDataset> hds;
     GprDataset *d_ds = nullptr;
@@ -158,9 +164,14 @@ extern "C" int gpr_create(const gpr_config *cfg, gpr_ctx **out) {
     c->cfg = *cfg;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, cfg->device) == hipSuccess) c->cus = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
+              hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int b = 0; ok && b < GPR_MAX_BUCKETS; b++)
+        ok = hipStreamCreateWithFlags(&c->side[b], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_join[b], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        (void)gpr_destroy(c);
         return fail(GPR_E_HIP, "stream/event creation failed");
     }
     *out = c;
@@ -172,6 +183,12 @@ extern "C" int gpr_destroy(gpr_ctx *c) {
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_state(c);
+    for (int b = 0; b < GPR_MAX_BUCKETS; b++) {
+        if (c->side[b]) (void)hipStreamSynchronize(c->side[b]);
+        if (c->side[b]) (void)hipStreamDestroy(c->side[b]);
+        if (c->ev_join[b]) (void)hipEventDestroy(c->ev_join[b]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -198,7 +215,7 @@ static void build_tiles(gpr_ctx *c) {
     for (int k = 2; k >= 0; k--)
         for (int hi = 1; hi >= 0; hi--) {
             const ClassTune &t = c->ct[k][hi];
-            KernelBucket b{k, hi != 0, t.kc, t.wpg, t.wgpc, (int)c->htiles.size(), 0};
+            KernelBucket b{k, hi != 0, t.kc, t.wpg, t.wgpc, (int)c->htiles.size(), 0, 0};
             std::vector<int> order;
             for (int d = 0; d < c->nd; d++)
                 if (cls_of(c->hds[d].n) == k && ((c->hds[d].cap > 64) == (hi != 0))) order.push_back(d);
@@ -212,14 +229,18 @@ static void build_tiles(gpr_ctx *c) {
 }
 
 // MALA token slabs of the two-register kernels (cap > 64) live in global memory, one per
-// resident wave (workgroups of the largest launch x waves per workgroup).  Sized whenever
-// the tiles change, before any launch, so never inside a graph capture.
+// resident wave (workgroups x waves per workgroup) of each two-register launch; the
+// launches of a block run concurrently, so each has its own region.  Sized whenever the
+// tiles change, before any launch, so never inside a graph capture.
 static int ensure_gslab(gpr_ctx *c) {
     if (c->cfg.semantics != GPR_SEM_CONFIG || c->cfg.proposal != GPR_PROP_MALA || c->rows <= 64) return GPR_OK;
-    size_t waves = 0;
-    for (const KernelBucket &kb : c->buckets)
-        waves = std::max(waves, (size_t)std::max(1, std::min(kb.ntiles, c->cus * kb.wgpc)) * (size_t)kb.wpg);
-    const size_t need = waves * (size_t)c->rows * 64;
+    size_t need = 0;
+    for (KernelBucket &kb : c->buckets) {
+        if (!kb.hi) continue;
+        if (kb.gslab_off != need) c->gexec_stale = true;
+        kb.gslab_off = need;
+        need += (size_t)std::max(1, std::min(kb.ntiles, c->cus * kb.wgpc)) * (size_t)kb.wpg * (size_t)c->rows * 64;
+    }
     if (need <= c->gslab_floats) return GPR_OK;
     if (c->d_gslab) {
         HIP_TRY(hipStreamSynchronize(c->stream));
@@ -315,7 +336,7 @@ extern "C" int gpr_set_data(gpr_ctx *c, int nd, const int32_t *n, const float *x
     ALLOC(c->d_prior, N);
     ALLOC(c->d_ll, N);
     ALLOC(c->d_tiles, N);                  // enough for one chain per tile
-    ALLOC(c->d_tile_counter, 6);
+    ALLOC(c->d_tile_counter, GPR_MAX_BUCKETS);
     ALLOC(c->d_counters, (size_t)GPR_NCNT);
     ALLOC(c->d_good, N);
     ALLOC(c->d_ngood, (size_t)nd);
@@ -447,7 +468,7 @@ extern "C" int gpr_read_state(gpr_ctx *c, uint8_t *ops, float *cs, int32_t *len,
 }
 
 // The persistent wave-schedule launch of one size class (its tile counter must be 0).
-static hipError_t launch_bucket(gpr_ctx *c, const GprMHArgs &a, int b) {
+static hipError_t launch_bucket(gpr_ctx *c, const GprMHArgs &a, int b, hipStream_t s) {
     const KernelBucket &kb = c->buckets[b];
     GprMHArgs ab = a;
     ab.tiles = c->d_tiles + kb.tile0;
@@ -455,10 +476,28 @@ static hipError_t launch_bucket(gpr_ctx *c, const GprMHArgs &a, int b) {
     ab.tile_counter = c->d_tile_counter + b;
     ab.dmax_cap = c->rows;
     ab.wpg = kb.wpg;
-    ab.mala_gslab = kb.hi ? c->d_gslab : nullptr;
+    ab.mala_gslab = kb.hi && c->d_gslab ? c->d_gslab + kb.gslab_off : nullptr;
     const int grid = std::max(1, std::min(kb.ntiles, c->cus * kb.wgpc));
-    if (c->cfg.semantics == GPR_SEM_REFERENCE) return gpr_launch_mh_ref_wave(ab, kb.kc, grid, c->stream);
-    return gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, c->cfg.proposal == GPR_PROP_MALA, grid, c->stream);
+    if (c->cfg.semantics == GPR_SEM_REFERENCE) return gpr_launch_mh_ref_wave(ab, kb.kc, grid, s);
+    return gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, c->cfg.proposal == GPR_PROP_MALA, grid, s);
+}
+
+// All size-class launches of one block.  They touch disjoint chains and have their own
+// tile counters and slabs, so they run concurrently: each persistent launch fills the
+// GPU, and the next one's workgroups take the CUs that the previous one's tail leaves
+// idle.  Forked from the context stream after the counter reset and joined back to it
+// (also the form a graph capture records).
+static hipError_t launch_buckets(gpr_ctx *c, const GprMHArgs &a) {
+    const int nb = (int)c->buckets.size();
+    if (nb == 1) return launch_bucket(c, a, 0, c->stream);
+    hipError_t e = hipEventRecord(c->ev_fork, c->stream);
+    for (int b = 0; b < nb && e == hipSuccess; b++) {
+        e = hipStreamWaitEvent(c->side[b], c->ev_fork, 0);
+        if (e == hipSuccess) e = launch_bucket(c, a, b, c->side[b]);
+        if (e == hipSuccess) e = hipEventRecord(c->ev_join[b], c->side[b]);
+    }
+    for (int b = 0; b < nb && e == hipSuccess; b++) e = hipStreamWaitEvent(c->stream, c->ev_join[b], 0);
+    return e;
 }
 
 // One MCMC block on the context stream.  on_device: step and block come from the
@@ -485,7 +524,7 @@ static int launch_block(gpr_ctx *c, int steps, int32_t block_seed, bool on_devic
         }
     } else {
         HIP_TRY(gpr_launch_fill_i32(c->d_tile_counter, (int)c->buckets.size(), 0, c->stream));
-        for (size_t b = 0; b < c->buckets.size(); b++) HIP_TRY(launch_bucket(c, a, (int)b));
+        HIP_TRY(launch_buckets(c, a));
     }
     c->sched_used = sched;
     if (on_device) {
@@ -771,7 +810,7 @@ extern "C" int gpr_autotune(gpr_ctx *c, int steps, gpr_tune *out, int max_out, i
                     TUNE_TRY(copy_all(false));
                     TUNE_TRY(gpr_launch_fill_i32(c->d_tile_counter, 6, 0, c->stream));
                     TUNE_TRY(hipEventRecord(c->ev0, c->stream));
-                    TUNE_TRY(launch_bucket(c, a, bi));
+                    TUNE_TRY(launch_bucket(c, a, bi, c->stream));
                     TUNE_TRY(hipEventRecord(c->ev1, c->stream));
                     TUNE_TRY(hipEventSynchronize(c->ev1));
                     float ms = 0.0f;

@@ -959,14 +959,29 @@ __device__ __forceinline__ double metric_logq(const double *Li, int k, double r,
 
 // config accept rule: invalid current always moves; log u < a otherwise.  u lies on the
 // grid ((k + 1/2) 2^-23), so log u >= -16.64: any a < -17 rejects without drawing.
+// The accept draw of step s is word s & 3 of the ACCEPT block s >> 2, so one Philox
+// block serves four consecutive steps: a chain loop keeps the last block it computed.
+struct AcceptDraws {
+    gpr_u4 w;
+    uint32_t blk;
+    __device__ __forceinline__ AcceptDraws() : w{0, 0, 0, 0}, blk(0xffffffffu) {}
+};
 __device__ __forceinline__ bool cfg_accept(uint64_t seed, uint32_t chain, uint32_t step, double cur, double prop,
-                                           double a) {
+                                           double a, AcceptDraws &ad) {
     if (!__builtin_isfinite(cur)) return true;
     if (!__builtin_isfinite(prop)) return false;
     if (a >= 0.0) return true;
     if (a < -17.0) return false;
-    const gpr_u4 w = gpr_step_block(seed, chain, step, GPR_P_ACCEPT, 0, 0);
-    const float u = gpr_u01(gpr_u4_word(w, (int)(step & 3)));
+    if ((step >> 2) != ad.blk) {
+        ad.w = gpr_step_block(seed, chain, step, GPR_P_ACCEPT, 0, 0);
+        ad.blk = step >> 2;
+    }
+    const float u = gpr_u01(gpr_u4_word(ad.w, (int)(step & 3)));
     return (double)gpr_logf(u) < a;
+}
+__device__ __forceinline__ bool cfg_accept(uint64_t seed, uint32_t chain, uint32_t step, double cur, double prop,
+                                           double a) {
+    AcceptDraws ad;
+    return cfg_accept(seed, chain, step, cur, prop, a, ad);
 }
 

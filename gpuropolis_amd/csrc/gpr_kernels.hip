@@ -30,7 +30,10 @@ extern __shared__ float gpr_lds_slab[];
 // The plain one-point-per-lane kernel is latency bound: six waves per SIMD (80 VGPRs) run
 // it faster than the four that fit unspilled, although the compiler then spills per-tile
 // values to scratch at every tile start and reloads some of them in the step loop.
-template <int KC, bool HI, bool MALA>
+// MET: some dataset of the launch has a proposal metric.  The metric path holds many
+// wave-uniform values (matrix rows, pointers), so launches without a metric use a
+// build without it, whose scalar registers then fit with far fewer spills.
+template <int KC, bool HI, bool MALA, bool MET>
 __global__ void __launch_bounds__(GPR_WG) __attribute__((amdgpu_waves_per_eu((!MALA && KC == 1) ? 6 : 1)))
 k_mh_config_wave(GprMHArgs a) {
     const int wave = uni((int)(threadIdx.x >> 6));
@@ -73,7 +76,7 @@ k_mh_config_wave(GprMHArgs a) {
 
         // proposal metric of this dataset; a chain whose metric rows are not all constant
         // tokens moves every constant with the identity
-        const GprMetric *mt = a.metric ? a.metric + tile.dataset : nullptr;
+        const GprMetric *mt = (MET && a.metric) ? a.metric + tile.dataset : nullptr;
         int mk = mt ? uni(mt->k) : 0;
         const int mrow = (l < mk) ? mt->rows[l] : 0;
         if (mk) {
@@ -105,6 +108,7 @@ k_mh_config_wave(GprMHArgs a) {
         // metric drift M g at the metric rows
         float cg0 = 0.0f, cg1 = 0.0f, pg0 = 0.0f, pg1 = 0.0f, cmg = 0.0f, pmg = 0.0f;
         bool grad_ok = false;
+        AcceptDraws adraw;
         // token classes of the current program (MALA: with the sweeps' plan)
         GradPlan gplan{};
         OpMasks cmask{};
@@ -344,7 +348,7 @@ k_mh_config_wave(GprMHArgs a) {
             }
             if (i < 0) { prior = pprior; ll = pll; continue; }
             const double ppost = pprior + pll;
-            if (cfg_accept(a.seed, chain, step, post, ppost, ppost - post + extra)) {
+            if (cfg_accept(a.seed, chain, step, post, ppost, ppost - post + extra, adraw)) {
                 cur = prop;
                 prior = pprior;
                 ll = pll;
@@ -391,27 +395,33 @@ k_mh_config_wave(GprMHArgs a) {
     if (l < GPR_NCNT && a.counters && cnt) atomicAdd(&a.counters[l], cnt);
 }
 
-template <int KC, bool HI, bool MALA>
+template <int KC, bool HI, bool MALA, bool MET>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
     const size_t sh = MALA ? (HI ? 0 : sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg)
                            : sizeof(float) * (size_t)a.wpg * GPR_MAX_DEPTH * 64 * KC;
     if (sh > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA, MET>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_mh_config_wave<KC, HI, MALA>), dim3(grid), dim3(64 * a.wpg), sh, s, a);
+    hipLaunchKernelGGL((k_mh_config_wave<KC, HI, MALA, MET>), dim3(grid), dim3(64 * a.wpg), sh, s, a);
     return hipGetLastError();
 }
 
-hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, bool mala, int grid, hipStream_t s) {
+template <bool MET>
+static hipError_t launch_mh_wave_met(const GprMHArgs &a, int kc, bool hi, bool mala, int grid, hipStream_t s) {
     // MALA evaluates one point per lane whatever the shape's kc
-    if (mala) return hi ? launch_mh_wave<1, true, true>(a, grid, s) : launch_mh_wave<1, false, true>(a, grid, s);
-#define GPR_L(K, H) return launch_mh_wave<K, H, false>(a, grid, s)
+    if (mala) return hi ? launch_mh_wave<1, true, true, MET>(a, grid, s) : launch_mh_wave<1, false, true, MET>(a, grid, s);
+#define GPR_L(K, H) return launch_mh_wave<K, H, false, MET>(a, grid, s)
     if (kc == 1) { if (hi) GPR_L(1, true); else GPR_L(1, false); }
     else if (kc == 2) { if (hi) GPR_L(2, true); else GPR_L(2, false); }
     else { if (hi) GPR_L(4, true); else GPR_L(4, false); }
 #undef GPR_L
+}
+
+hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, bool mala, int grid, hipStream_t s) {
+    return a.metric ? launch_mh_wave_met<true>(a, kc, hi, mala, grid, s)
+                    : launch_mh_wave_met<false>(a, kc, hi, mala, grid, s);
 }
 
 // ---------------------------------------------------------------- config MH, lane schedule
