@@ -414,6 +414,29 @@ struct Chunk {
 // far above its FP32 rounding) is below reject_below, or the partial sum is not finite,
 // the evaluation stops and returns -inf: the proposal is rejected either way.  *points
 // receives the number of data points evaluated.
+// Work-counter values of a program: operator tokens + 4 (the per-point FP32 ops of its
+// evaluation and Gaussian term) and its constant-carrying tokens.
+template <bool HI>
+__device__ __forceinline__ void prog_counts(const WaveProg &p, int &opq, int &nc) {
+    const int l = lane_id();
+    const int nleaf = __popcll(ballot(l < p.len && p.o0 <= GPR_OP_ONE)) +
+                      (HI ? __popcll(ballot(l + 64 < p.len && p.o1 <= GPR_OP_ONE)) : 0);
+    nc = __popcll(ballot(l < p.len && gpr_has_const(p.o0))) +
+         (HI ? __popcll(ballot(l + 64 < p.len && gpr_has_const(p.o1))) : 0);
+    opq = p.len - nleaf + 4;
+}
+
+// Random-walk moves of a program can be batched (four steps per Philox block) when
+// BATCH holds, the chain has no proposal metric (mk = 0), the tape fits the low half, it
+// has at most 16 constant tokens and no constant-operand token.
+template <bool BATCH, bool HI>
+__device__ __forceinline__ bool batch_ok(const WaveProg &p, int mk) {
+    if (!BATCH || mk != 0) return false;
+    const int l = lane_id();
+    const uint64_t m = ballot(l < p.len && p.o0 == GPR_OP_CONST);
+    return __popcll(m) <= 16 && ballot(l < p.len && gpr_has_const(p.o0)) == m;
+}
+
 template <int KC, bool HI, bool LS = false>
 __device__ double wave_loglik(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
                               const float *dw, const Chunk<KC> &c0, int nchunks, float *ls = nullptr,
@@ -423,7 +446,7 @@ __device__ double wave_loglik(const WaveProg &p, const GprDataset &d, const floa
     if (points) *points = d.n;
     for (int ch = 0; ch < nchunks; ch++) {
         Chunk<KC> c;
-        if (nchunks == 1) c = c0; else c.load(dx, dy, dw, d.n, ch * 64 * KC);
+        if (ch == 0) c = c0; else c.load(dx, dy, dw, d.n, ch * 64 * KC);
         float T[KC];
         if (LS) wave_eval_ls<KC, HI>(p, c.x, T, ls, *pm);
         else wave_eval<KC, HI>(p, c.x, T);

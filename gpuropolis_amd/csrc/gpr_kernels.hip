@@ -114,6 +114,12 @@ k_mh_config_wave(GprMHArgs a) {
         OpMasks cmask{};
         if (MALA) gplan = grad_plan<HI>(cur);
         else cmask = op_masks<HI>(cur);
+        // per-program values that change only with the structure: (operator tokens + 4)
+        // and constants of the current program for the work counters, and whether its
+        // random-walk moves can be batched (see BATCH below)
+        int cur_opq, cur_nc;
+        prog_counts<HI>(cur, cur_opq, cur_nc);
+        bool cur_bok = batch_ok<BATCH, HI>(cur, mk);
         // step % struct_every, carried as a counter (no integer division per step)
         // per-tile counts: structure proposed/accepted, constant proposed/accepted, and over
         // evaluated programs the sums of tokens x points, of (operator tokens + 4) x points
@@ -206,8 +212,7 @@ k_mh_config_wave(GprMHArgs a) {
                 const uint64_t mk0 = ballot(k0), mk1 = HI ? ballot(k1) : 0;
                 if ((mk0 | mk1) == 0) continue;
                 c_cp++;
-                const bool batch = BATCH && mk == 0 && __popcll(mk0) <= 16 &&
-                                   ballot(l < cur.len && gpr_has_const(cur.o0)) == mk0;
+                const bool batch = BATCH && cur_bok;
                 if (batch) {
                     const int wi = (int)(step & 3);
                     if ((step >> 2) != zblk) {
@@ -337,15 +342,11 @@ k_mh_config_wave(GprMHArgs a) {
                     pll = wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &pmask, reject_below, &pts);
                 pts = min(pts, d.n);
             }
-            {
-                const int nleaf = __popcll(ballot(l < prop.len && prop.o0 <= GPR_OP_ONE)) +
-                                  (HI ? __popcll(ballot(l + 64 < prop.len && prop.o1 <= GPR_OP_ONE)) : 0);
-                const int nconst = __popcll(ballot(l < prop.len && gpr_has_const(prop.o0))) +
-                                   (HI ? __popcll(ballot(l + 64 < prop.len && gpr_has_const(prop.o1))) : 0);
-                c_ev += (unsigned long long)prop.len * (unsigned long long)pts;
-                c_fl += (unsigned long long)(prop.len - nleaf + 4) * (unsigned long long)pts;
-                c_nc += (uint32_t)nconst;
-            }
+            int p_opq = cur_opq, p_nc = cur_nc;
+            if (structural) prog_counts<HI>(prop, p_opq, p_nc);
+            c_ev += (unsigned long long)prop.len * (unsigned long long)pts;
+            c_fl += (unsigned long long)p_opq * (unsigned long long)pts;
+            c_nc += (uint32_t)p_nc;
             if (i < 0) { prior = pprior; ll = pll; continue; }
             const double ppost = pprior + pll;
             if (cfg_accept(a.seed, chain, step, post, ppost, ppost - post + extra, adraw)) {
@@ -358,6 +359,9 @@ k_mh_config_wave(GprMHArgs a) {
                     grad_ok = false;
                     zblk = 0xffffffffu;
                     if (MALA) gplan = pplan; else cmask = pmask;
+                    cur_opq = p_opq;
+                    cur_nc = p_nc;
+                    cur_bok = batch_ok<BATCH, HI>(cur, mk);
                 } else {
                     c_ca++;
                     zblk = 0xffffffffu;
