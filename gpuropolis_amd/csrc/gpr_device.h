@@ -426,6 +426,14 @@ __device__ __forceinline__ void prog_counts(const WaveProg &p, int &opq, int &nc
     opq = p.len - nleaf + 4;
 }
 
+// Constant tokens of a program.
+template <bool HI>
+__device__ __forceinline__ int const_count(const WaveProg &p) {
+    const int l = lane_id();
+    return __popcll(ballot(l < p.len && p.o0 == GPR_OP_CONST)) +
+           (HI ? __popcll(ballot(l + 64 < p.len && p.o1 == GPR_OP_CONST)) : 0);
+}
+
 // Random-walk moves of a program can be batched (four steps per Philox block) when
 // BATCH holds, the chain has no proposal metric (mk = 0), the tape fits the low half, it
 // has at most 16 constant tokens and no constant-operand token.

@@ -94,13 +94,16 @@ k_mh_config_wave(GprMHArgs a) {
         // per-lane constant-proposal block, reused by four consecutive steps
         gpr_u4 wlo = {0, 0, 0, 0}, whi = {0, 0, 0, 0};
         uint32_t wblk = 0xffffffffu;
-        // Batched random-walk moves (BATCH: no MALA, tape within the low half; per step: no
-        // metric, at most 16 constant tokens and no constant-operand token).  At the first
-        // constant step of a four-step Philox block, lane j = 16 w + r takes the normal of
-        // word w of the r-th constant token's block and, from the current constants, the
-        // proposed constant of step w and the logf of its Cauchy term; each step then reads
-        // its lanes (proposal and prior), the same values as the per-step path.  Any
-        // accepted move drops the batch (the next constant step recomputes it).
+        // Batched random-walk moves (BATCH: no MALA, tape within the low half; per program:
+        // no metric, at most 16 constant tokens and no constant-operand token).  The draw of
+        // step s for a constant is word s & 3 of the Philox block s >> 2 of its token, so a
+        // batch covers 4 steps (16 lanes each, up to 16 constants) or, for programs of at
+        // most 4 constants, 16 steps (4 lanes each).  At the first constant step of a batch,
+        // lane j = ks u + r takes the normal of step u's word for the r-th constant token
+        // and, from the current constants, the proposed constant and the logf of its Cauchy
+        // term; each step then reads its lanes (proposal and prior), the same values as the
+        // per-step path.  Any accepted move drops the batch (the next constant step
+        // recomputes it).
         constexpr bool BATCH = !MALA && !HI;
         float cb = 0.0f, lfb = 0.0f;
         uint32_t zblk = 0xffffffffu;
@@ -120,6 +123,7 @@ k_mh_config_wave(GprMHArgs a) {
         int cur_opq, cur_nc;
         prog_counts<HI>(cur, cur_opq, cur_nc);
         bool cur_bok = batch_ok<BATCH, HI>(cur, mk);
+        bool cur_b16 = cur_bok && const_count<HI>(cur) <= 4;
         // step % struct_every, carried as a counter (no integer division per step)
         // per-tile counts: structure proposed/accepted, constant proposed/accepted, and over
         // evaluated programs the sums of tokens x points, of (operator tokens + 4) x points
@@ -214,31 +218,34 @@ k_mh_config_wave(GprMHArgs a) {
                 c_cp++;
                 const bool batch = BATCH && cur_bok;
                 if (batch) {
-                    const int wi = (int)(step & 3);
-                    if ((step >> 2) != zblk) {
-                        zblk = step >> 2;
-                        const gpr_u4 wb = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 0);
-                        // lane j: the token of constant rank j & 15
+                    // KS lanes per step: 4 for programs of at most 4 constants (a batch of 16
+                    // steps, four Philox blocks per constant), else 16 (a batch of 4 steps)
+                    const int lg = cur_b16 ? 4 : 2, ks = cur_b16 ? 4 : 16;
+                    const int u = (int)(step & ((1u << lg) - 1));
+                    if ((step >> lg) != zblk) {
+                        zblk = step >> lg;
+                        // lane j = ks * uj + r: step base + uj, constant of rank r, whose
+                        // token is tr; the lane draws that step's word of that token's block
+                        const int r = l & (ks - 1);
                         int tr = 0;
                         uint64_t m = mk0;
-                        for (int r = 0; m; r++) {
+                        for (int q = 0; m; q++) {
                             const int b = __builtin_ctzll(m);
                             m &= m - 1;
-                            tr = (l & 15) == r ? b : tr;
+                            tr = r == q ? b : tr;
                         }
-                        const int w = l >> 4;
-                        const uint32_t wx = (uint32_t)shfl_all((int)wb.x, tr), wy = (uint32_t)shfl_all((int)wb.y, tr);
-                        const uint32_t wz = (uint32_t)shfl_all((int)wb.z, tr), ww = (uint32_t)shfl_all((int)wb.w, tr);
-                        const float zb = gpr_normal(gpr_u01(w == 0 ? wx : (w == 1 ? wy : (w == 2 ? wz : ww))));
+                        const uint32_t sj = (zblk << lg) + (uint32_t)(l >> (6 - lg));
+                        const gpr_u4 wb = gpr_step_block(a.seed, chain, sj, GPR_P_CPROP, (uint32_t)tr, 0);
+                        const float zb = gpr_normal(gpr_u01(gpr_u4_word(wb, (int)(sj & 3))));
                         cb = gpr_axpy(a.sigma, zb, shfl_all(cur.c0, tr));
                         lfb = gpr_lcauchy_logf(cb);
                     }
                     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk0 >> 32),
                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mk0, 0u));
-                    const float cv = shfl_all(cb, 16 * wi + rank);
+                    const float cv = shfl_all(cb, ks * u + rank);
                     if (k0) prop.c0 = cv;
                     bprior = true;
-                    bbase = 16 * wi;
+                    bbase = ks * u;
                     bK = __popcll(mk0);
                 } else {
                 if (BATCH || (step >> 2) != wblk) {
@@ -362,6 +369,7 @@ k_mh_config_wave(GprMHArgs a) {
                     cur_opq = p_opq;
                     cur_nc = p_nc;
                     cur_bok = batch_ok<BATCH, HI>(cur, mk);
+                    cur_b16 = cur_bok && const_count<HI>(cur) <= 4;
                 } else {
                     c_ca++;
                     zblk = 0xffffffffu;
