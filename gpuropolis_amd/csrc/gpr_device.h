@@ -426,6 +426,33 @@ __device__ __forceinline__ void prog_counts(const WaveProg &p, int &opq, int &nc
     opq = p.len - nleaf + 4;
 }
 
+// The words of a Philox substream (chain, step, purpose, slot) as gpr_rstream yields
+// them, computed on the vector unit 64 at a time: lane k holds word 64 b + k (block
+// 16 b + k / 4, word k % 4) of the current batch b, and next() reads them in order with
+// v_readlane.  One vector Philox evaluation replaces up to 16 scalar ones, which takes
+// the structure move's draws off the scalar unit.
+struct WaveStream {
+    uint64_t seed;
+    uint32_t chain, step, c2;
+    uint32_t w;     // this lane's word of the current batch
+    int idx, base;  // next word within the batch, batch index
+    __device__ __forceinline__ void fill() {
+        const int k = lane_id();
+        const gpr_u4 b = gpr_philox(chain, step, c2, (uint32_t)(16 * base + (k >> 2)), (uint32_t)seed,
+                                    (uint32_t)(seed >> 32));
+        w = gpr_u4_word(b, k & 3);
+    }
+    __device__ __forceinline__ void init(uint64_t s, uint32_t ch, uint32_t st, uint32_t purpose, uint32_t slot) {
+        seed = s; chain = ch; step = st; c2 = (purpose << 24) | (slot & 0xffffffu);
+        idx = 0; base = 0;
+        fill();
+    }
+    __device__ __forceinline__ uint32_t next() {
+        if (idx == 64) { base++; idx = 0; fill(); }
+        return (uint32_t)rl_i((int)w, idx++);
+    }
+};
+
 // Constant tokens of a program.
 template <bool HI>
 __device__ __forceinline__ int const_count(const WaveProg &p) {

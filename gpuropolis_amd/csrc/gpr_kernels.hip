@@ -147,7 +147,7 @@ k_mh_config_wave(GprMHArgs a) {
             if (i < 0) {
             } else if (structural) {
                 c_sp++;
-                gpr_rstream st;
+                WaveStream st;
                 st.init(a.seed, chain, step, GPR_P_STRUCT, 0);
                 const uint32_t r = st.next();
                 const int j = __builtin_isfinite(post) ? (int)(((uint64_t)r * (uint64_t)cur.len) >> 32) : cur.len - 1;
