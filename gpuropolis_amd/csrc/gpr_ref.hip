@@ -45,6 +45,11 @@ struct PhiloxSrc {
     gpr_rstream st;
     __device__ __forceinline__ uint32_t rand31() { return st.next() >> 1; }
 };
+// the same words, drawn 64 at a time on the vector unit (wave-uniform draws)
+struct WavePhiloxSrc {
+    WaveStream st;
+    __device__ __forceinline__ uint32_t rand31() { return st.next() >> 1; }
+};
 
 template <class S> __device__ __forceinline__ int r_random_int(int n, S &s) {
     const uint32_t div = (uint32_t)(0x7fffffff / (n + 1));
@@ -502,7 +507,8 @@ __device__ __forceinline__ double r_prior_wave(int dl, float hc, double lnops) {
 }
 
 template <int KC>
-__global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
+__global__ void __launch_bounds__(GPR_WG) __attribute__((amdgpu_waves_per_eu(KC == 1 ? 4 : 1)))
+k_mh_ref_wave(GprMHArgs a) {
     const uint32_t step0 = a.clock ? a.clock[0] : a.step0;
     const int nops = gpr_opset_nops(a.opset);
     const int wave = uni((int)(threadIdx.x >> 6));
@@ -541,7 +547,7 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
             if (structural) {
                 n_sprop++;
                 forward = -(double)gpr_logf((float)cur.count);
-                PhiloxSrc ph;
+                WavePhiloxSrc ph;
                 ph.st.init(a.seed, chain, step, GPR_P_STRUCT, 0);
                 const int nn = !r_valid(cur_post) ? 0 : r_random_int(GPR_REF_SLOTS, ph);
                 for (int i = nn; i < GPR_REF_SLOTS; i++) {
@@ -564,18 +570,38 @@ __global__ void __launch_bounds__(GPR_WG) k_mh_ref_wave(GprMHArgs a) {
                 r_tape_consts(prop, php, phc);
             }
             const double pp = r_prior_wave(prop.dl, phc, a.ref_lnops);
-            const double pl = wave_loglik<KC, false>(prop.p, d, dx, dy, dw, c0, nchunks);
-            n_evals += (unsigned long long)prop.p.len * (unsigned long long)d.n;
+            // The accept draw u is the first random_float of the step's ACCEPT substream,
+            // the same value whenever it is taken, so it is drawn before the evaluation.  A
+            // valid current state accepts only if acc > 0 or u < expf(acc), and either needs
+            // acc >= log u - 1 (u > 0) or acc >= -105 (u = 0: expf(acc) underflows to 0
+            // below -103.98).  That bounds the proposal's LL from below: the evaluation
+            // stops once its partial sum proves the LL under the bound (wave_loglik), and
+            // the proposal is rejected as it would be after the full sum.
+            float u = 0.0f;
+            double reject_below = -GPR_INF;
+            if (r_valid(cur_post)) {
+                WavePhiloxSrc pa;
+                pa.st.init(a.seed, chain, step, GPR_P_ACCEPT, 0);
+                u = r_random_float(pa);
+                const double amin = u > 0.0f ? (double)gpr_logf(u) - 1.0 : -105.0;
+                if (r_valid(pp) && r_valid(forward) && r_valid(backward))
+                    reject_below = cur_post - backward + forward - pp + amin -
+                                   1e-12 * (__builtin_fabs(cur_post) + __builtin_fabs(backward) +
+                                            __builtin_fabs(forward) + __builtin_fabs(pp));
+            }
+            int pts = d.n;
+            const double pl = wave_loglik<KC, false>(prop.p, d, dx, dy, dw, c0, nchunks, nullptr, nullptr,
+                                                     reject_below, &pts);
+            pts = min(pts, d.n);
+            n_evals += (unsigned long long)prop.p.len * (unsigned long long)pts;
             {
                 const int nleaf = __popcll(ballot(l < prop.p.len && prop.p.o0 <= GPR_OP_ONE));
                 const int nconst = __popcll(ballot(l < prop.p.len && gpr_has_const(prop.p.o0)));
-                n_flops += (unsigned long long)(prop.p.len - nleaf + 4) * (unsigned long long)d.n + (unsigned long long)nconst;
+                n_flops += (unsigned long long)(prop.p.len - nleaf + 4) * (unsigned long long)pts + (unsigned long long)nconst;
             }
             const double ppost = pp + pl;
             const double acc = ppost - cur_post + backward - forward;
-            PhiloxSrc pa;
-            pa.st.init(a.seed, chain, step, GPR_P_ACCEPT, 0);
-            if (r_accept(pa, ppost, cur_post, acc)) {
+            if ((r_valid(ppost) && (acc > 0.0 || u < gpr_expf((float)acc))) || !r_valid(cur_post)) {
                 cur_post = ppost; cur_ll = pl; cur_prior = pp;
                 hop = php; hc = phc; cur = prop;
                 if (structural) n_sacc++; else n_cacc++;

@@ -36,8 +36,11 @@ def test_ref_init_matches_oracle(oracle, gpr):
     assert np.array_equal(g_hc.view(np.uint32), hc.view(np.uint32))
 
 
+# Kirby2 and Hahn1 (n = 151, 236) span several 64-point chunks, so the wave kernel's
+# early rejection (the LL bound from the pre-drawn accept u) takes part
 CASES = [("lane", 0, "S", 200), ("lane", 1, "S", 200), ("wave", 0, "S", 200), ("wave", 0, "BoxBOD", 300),
-         ("wave", 0, "Thurber", 150), ("lane", 1, "Misra1a", 100)]
+         ("wave", 0, "Thurber", 150), ("lane", 1, "Misra1a", 100), ("wave", 0, "Kirby2", 120),
+         ("wave", 0, "Hahn1", 80)]
 
 
 @pytest.mark.parametrize("sched,rng,dname,steps", CASES)
@@ -57,6 +60,18 @@ def test_ref_trajectories_match_oracle(oracle, gpr, sched, rng, dname, steps):
         ctx.run_block(steps, 4321)
         ctx.run_block(steps, 8765)
         g_hop, g_hc, _, g_pr, g_ll = ctx.read_state()
+        st = ctx.stats()
+    if sched == "wave" and len(x) > 64:
+        # the lane kernel evaluates every point of every proposal; on the same chains the
+        # wave kernel stops most proposals after their first chunk
+        with gpr.Context(semantics=gpr.SEM_REFERENCE, seed=99, rng=rng, schedule=gpr.SCHED_LANE) as lctx:
+            lctx.set_data([(x, y, sd)], [nch])
+            lctx.write_state(hop, hc)
+            lctx.run_block(steps, 4321)
+            lctx.run_block(steps, 8765)
+            lst = lctx.stats()
+        say("evaluated token x points, wave / lane:", st.node_evals, lst.node_evals)
+        assert st.node_evals < 0.8 * lst.node_evals
     same = np.all(g_hop == o_hop, axis=0) & np.all(g_hc.view(np.uint32) == o_hc.view(np.uint32), axis=0)
     say("chains on the oracle path:", int(same.sum()), "of", nch)
     assert same.sum() >= int(0.95 * nch)
