@@ -32,7 +32,7 @@ HBM_PEAK = 8.0e12
 # HBM bytes per MH launch (FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc, scaled by the
 # same counters on a plain copy of known size) for the default configuration, keyed by
 # (schedule, chains, cap, MH steps per block, seed); the profiles/ summaries hold the runs.
-MEASURED_TRAFFIC = {("wave", 131072, 64, 1000, 0): 558603328}
+MEASURED_TRAFFIC = {("wave", 131072, 64, 1000, 0): 281613280}
 
 METRIC = "MH steps/sec + node×datapoint evals/sec (% VALU peak); ESS/sec MALA vs RW"
 
