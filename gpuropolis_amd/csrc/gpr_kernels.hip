@@ -130,6 +130,11 @@ k_mh_config_wave(GprMHArgs a) {
         // and of constants
         uint32_t c_sp = 0, c_sa = 0, c_cp = 0, c_ca = 0, c_nc = 0;
         unsigned long long c_ev = 0, c_fl = 0;
+        // evaluations of the current program add only their points and their number; the
+        // sums take them with the program's values when its structure changes and at the
+        // end of the tile (a product per structure, not per step)
+        unsigned long long cur_pts = 0;
+        uint32_t cur_nev = 0;
         const uint32_t se = (uint32_t)a.struct_every;
         uint32_t srem = step0 % se;
         for (int i = -1; i < a.steps; i++) {
@@ -350,13 +355,25 @@ k_mh_config_wave(GprMHArgs a) {
                 pts = min(pts, d.n);
             }
             int p_opq = cur_opq, p_nc = cur_nc;
-            if (structural) prog_counts<HI>(prop, p_opq, p_nc);
-            c_ev += (unsigned long long)prop.len * (unsigned long long)pts;
-            c_fl += (unsigned long long)p_opq * (unsigned long long)pts;
-            c_nc += (uint32_t)p_nc;
+            if (structural) {
+                prog_counts<HI>(prop, p_opq, p_nc);
+                c_ev += (unsigned long long)prop.len * (unsigned long long)pts;
+                c_fl += (unsigned long long)p_opq * (unsigned long long)pts;
+                c_nc += (uint32_t)p_nc;
+            } else {
+                cur_pts += (unsigned long long)pts;
+                cur_nev++;
+            }
             if (i < 0) { prior = pprior; ll = pll; continue; }
             const double ppost = pprior + pll;
             if (cfg_accept(a.seed, chain, step, post, ppost, ppost - post + extra, adraw)) {
+                if (structural) {
+                    c_ev += (unsigned long long)cur.len * cur_pts;
+                    c_fl += (unsigned long long)cur_opq * cur_pts;
+                    c_nc += (uint32_t)cur_nc * cur_nev;
+                    cur_pts = 0;
+                    cur_nev = 0;
+                }
                 cur = prop;
                 prior = pprior;
                 ll = pll;
@@ -379,6 +396,9 @@ k_mh_config_wave(GprMHArgs a) {
                 }
             }
         }
+        c_ev += (unsigned long long)cur.len * cur_pts;
+        c_fl += (unsigned long long)cur_opq * cur_pts;
+        c_nc += (uint32_t)cur_nc * cur_nev;
         {
             unsigned long long v = 0;
             v = l == GPR_CNT_SPROP ? c_sp : v;
