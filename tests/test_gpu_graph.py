@@ -1,7 +1,9 @@
 """MCMC blocks captured as hipGraphs: every replay (with eager reads of the state in
 between) leaves exactly the state and counters the same eager gpr_run_block +
 gpr_resample calls leave, for the config sampler (random walk and MALA) and the
-reference sampler on the wave schedule."""
+reference sampler on the wave schedule.  The batched cases hold datasets of every size
+class and both register forms, so a block is several launches forked onto their own
+streams and joined (MALA: each two-register launch with its own token slabs)."""
 import numpy as np
 import pytest
 
@@ -11,12 +13,18 @@ pytestmark = pytest.mark.gpu
 
 CASES = [("config-rw", dict(cap=64, seed=4, resample_nats=5.0)),
          ("config-mala", dict(cap=32, seed=6, proposal=1, eps=0.01, resample_nats=5.0)),
-         ("reference", dict(semantics=1, seed=8, schedule=2, resample=2))]
+         ("reference", dict(semantics=1, seed=8, schedule=2, resample=2)),
+         ("config-rw-batched", dict(cap=128, seed=5, resample_nats=5.0)),
+         ("config-mala-batched", dict(cap=128, seed=7, proposal=1, eps=0.01, resample_nats=5.0))]
 
 
-def data():
+def set_data(ctx, name, nch):
     from gpuropolis_amd import workloads as W
-    return W.dataset_S(0)
+    if name.endswith("batched"):
+        dsets, caps, _ = W.datasets_B(0, nd=9)
+        ctx.set_data(dsets, [nch // 9] * len(dsets), caps)
+    else:
+        ctx.set_data([W.dataset_S(0)], [nch])
 
 
 def same_state(a, b):
@@ -26,10 +34,9 @@ def same_state(a, b):
 @pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
 def test_graph_replays_match_eager(gpr, name, kw):
     nch, steps, reps = 1000, 40, 3
-    x, y, sd = data()
     eager = []
     with gpr.Context(**kw) as ctx:
-        ctx.set_data([(x, y, sd)], [nch])
+        set_data(ctx, name, nch)
         ctx.init_chains()
         for _ in range(reps):
             ctx.run_block(steps)
@@ -37,7 +44,7 @@ def test_graph_replays_match_eager(gpr, name, kw):
             eager.append(ctx.read_state())
         st_e = ctx.stats()
     with gpr.Context(**kw) as ctx:
-        ctx.set_data([(x, y, sd)], [nch])
+        set_data(ctx, name, nch)
         ctx.init_chains()
         for r in range(reps):
             say(f"{name}: graph replay {r}")
@@ -47,7 +54,7 @@ def test_graph_replays_match_eager(gpr, name, kw):
     for f in ("struct_proposed", "struct_accepted", "const_proposed", "const_accepted", "steps", "node_evals"):
         assert getattr(st_e, f) == getattr(st_g, f), f
     with gpr.Context(**kw) as ctx:
-        ctx.set_data([(x, y, sd)], [nch])
+        set_data(ctx, name, nch)
         ctx.init_chains()
         say(f"{name}: one graph of {reps} blocks")
         ctx.run_blocks_graph(reps, steps)
@@ -56,7 +63,7 @@ def test_graph_replays_match_eager(gpr, name, kw):
         ctx.resample()
         s1 = ctx.read_state()
     with gpr.Context(**kw) as ctx:
-        ctx.set_data([(x, y, sd)], [nch])
+        set_data(ctx, name, nch)
         ctx.init_chains()
         for _ in range(reps + 1):
             ctx.run_block(steps)
