@@ -462,11 +462,11 @@ __device__ __forceinline__ int const_count(const WaveProg &p) {
 }
 
 // Random-walk moves of a program can be batched (four steps per Philox block) when
-// BATCH holds, the chain has no proposal metric (mk = 0), the tape fits the low half, it
-// has at most 16 constant tokens and no constant-operand token.
+// BATCH holds, the chain has no proposal metric (mk = 0), the tape fits the low half (at
+// most 64 tokens), it has at most 16 constant tokens and no constant-operand token.
 template <bool BATCH, bool HI>
 __device__ __forceinline__ bool batch_ok(const WaveProg &p, int mk) {
-    if (!BATCH || mk != 0) return false;
+    if (!BATCH || mk != 0 || (HI && p.len > 64)) return false;
     const int l = lane_id();
     const uint64_t m = ballot(l < p.len && p.o0 == GPR_OP_CONST);
     return __popcll(m) <= 16 && ballot(l < p.len && gpr_has_const(p.o0)) == m;

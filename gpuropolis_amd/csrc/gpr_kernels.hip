@@ -94,7 +94,7 @@ k_mh_config_wave(GprMHArgs a) {
         // per-lane constant-proposal block, reused by four consecutive steps
         gpr_u4 wlo = {0, 0, 0, 0}, whi = {0, 0, 0, 0};
         uint32_t wblk = 0xffffffffu;
-        // Batched random-walk moves (BATCH: no MALA, tape within the low half; per program:
+        // Batched random-walk moves (BATCH: no MALA; per program: tape within the low half,
         // no metric, at most 16 constant tokens and no constant-operand token).  The draw of
         // step s for a constant is word s & 3 of the Philox block s >> 2 of its token, so a
         // batch covers 4 steps (16 lanes each, up to 16 constants) or, for programs of at
@@ -104,7 +104,7 @@ k_mh_config_wave(GprMHArgs a) {
         // term; each step then reads its lanes (proposal and prior), the same values as the
         // per-step path.  Any accepted move drops the batch (the next constant step
         // recomputes it).
-        constexpr bool BATCH = !MALA && !HI;
+        constexpr bool BATCH = !MALA;
         float cb = 0.0f, lfb = 0.0f;
         uint32_t zblk = 0xffffffffu;
         // MALA: gradient of the current state (lane-distributed like the constants) and its
