@@ -235,11 +235,14 @@ __device__ __forceinline__ void wave_eval(const WaveProg &p, const float (&xs)[K
 }
 
 // The same interpreter with the value stack in LDS: the top two entries stay in VGPRs
-// (T, U) and the rest live in this wave's slab ls[level][lane][KC] (GPR_MAX_DEPTH
-// levels).  A push spills U to the slab, a binary op refills U from it; both are one
-// 16-byte LDS access per lane at KC = 4.  Saves the 16*KC VGPRs of the indexed stack
-// (occupancy) and the indexing-mode switches.  Same operations on the same operands
-// as wave_eval, so the values are bit-identical.
+// (T, U) and the rest live in this wave's slab ls[level][lane][KC] (GPR_MAX_DEPTH + 2
+// levels; levels 0 and 1 take the stores and loads of an empty U, so the pushes and
+// pops need no test).  A push spills U to the slab, a binary op refills U from it; both
+// are one 16-byte LDS access per lane at KC = 4.  Saves the 16*KC VGPRs of the indexed
+// stack (occupancy) and the indexing-mode switches.  Same operations on the same
+// operands as wave_eval, so the values are bit-identical.  The tape must be well formed
+// with stack depth at most GPR_MAX_DEPTH: the sampler's programs are (init, proposals
+// and gpr_write_state check it), and the stack pointer is not clamped.
 template <int KC>
 __device__ __forceinline__ void ls_store(float *q, const float (&v)[KC]) {
     if (KC == 4) *(float4 *)q = make_float4(v[0], v[1], v[2], v[3]);
@@ -281,7 +284,7 @@ template <int KC, bool HI>
 __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs)[KC], float (&T)[KC], float *ls,
                                              const OpMasks &m) {
     float U[KC];
-    float *lane = ls + lane_id() * KC;
+    float *lane = ls + 2 * 64 * KC + lane_id() * KC;   // level -2 of the stack below T, U
     int sp = 0;   // entries on the stack, T and U included
 #pragma unroll
     for (int k = 0; k < KC; k++) { T[k] = 0.0f; U[k] = 0.0f; }
@@ -307,11 +310,11 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
             continue;
         }
         if (m.leaf[h] & bit) {
-            // level sp-2 receives U when U holds an entry (bounded for malformed tapes)
-            if (sp >= 2) ls_store<KC>(lane + min(sp - 2, GPR_MAX_DEPTH - 1) * 64 * KC, U);
+            // level sp-2 receives U (an unused level while U is empty)
+            ls_store<KC>(lane + (sp - 2) * 64 * KC, U);
 #pragma unroll
             for (int k = 0; k < KC; k++) U[k] = T[k];
-            sp = min(sp + 1, GPR_MAX_DEPTH + 1);
+            sp++;
             const float cv = h ? rl_f(p.c1, l) : rl_f(p.c0, l);
             const bool isx = (m.x[h] & bit) != 0;
             GPR_KLOOP(isx ? xs[k] : cv);
@@ -324,17 +327,16 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
                 const float pr = U[k] * T[k];
                 T[k] = mul ? pr : pm;
             }
-            if (sp >= 3) ls_load<KC>(lane + min(sp - 3, GPR_MAX_DEPTH - 1) * 64 * KC, U);
-            sp = max(sp - 1, 0);
+            ls_load<KC>(lane + (sp - 3) * 64 * KC, U);   // the new second entry (level sp-3)
+            sp--;
             continue;
         } else {
         const int op = h ? rl_i(p.o1, l) : rl_i(p.o0, l);
         if (op <= GPR_OP_ONE) {
-            // level sp-2 receives U when U holds an entry (bounded for malformed tapes)
-            if (sp >= 2) ls_store<KC>(lane + min(sp - 2, GPR_MAX_DEPTH - 1) * 64 * KC, U);
+            ls_store<KC>(lane + (sp - 2) * 64 * KC, U);
 #pragma unroll
             for (int k = 0; k < KC; k++) U[k] = T[k];
-            sp = min(sp + 1, GPR_MAX_DEPTH + 1);
+            sp++;
             const float cv = h ? rl_f(p.c1, l) : rl_f(p.c0, l);
             const float sv = op == GPR_OP_CONST ? cv : 1.0f;
             const bool isx = op == GPR_OP_X;
@@ -358,9 +360,8 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
                 default: GPR_KLOOP(powf(T[k], U[k])); break;
                 }
             }
-            // the new second entry is level sp-3, when there is one
-            if (sp >= 3) ls_load<KC>(lane + min(sp - 3, GPR_MAX_DEPTH - 1) * 64 * KC, U);
-            sp = max(sp - 1, 0);
+            ls_load<KC>(lane + (sp - 3) * 64 * KC, U);   // the new second entry (level sp-3)
+            sp--;
         } else {
             const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 0.0f;
             switch (op) {
@@ -406,7 +407,7 @@ struct Chunk {
 
 // Gaussian log-likelihood of the wave's program: lconst - sum_i (f_i - y_i)^2 w_i.
 // Per-lane Kahan sums, wave tree reduction; -inf if any term is not finite.
-// LS: the value stack in LDS, ls = this wave's slab of GPR_MAX_DEPTH * 64 * KC floats,
+// LS: the value stack in LDS, ls = this wave's slab of (GPR_MAX_DEPTH + 2) * 64 * KC floats,
 // pm = the program's token-class masks; else the VGPR stack.
 // reject_below: the caller rejects the proposal whenever the LL lies below this value.
 // Every term of the sum is >= 0, so after each chunk but the last the sum so far bounds

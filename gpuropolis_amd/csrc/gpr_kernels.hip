@@ -40,7 +40,7 @@ k_mh_config_wave(GprMHArgs a) {
     const int l = lane_id();
     const int N = a.nchains;
     // dynamic LDS: the value stacks of the wpg waves, or under MALA their token-value slabs
-    float *ls = MALA ? nullptr : gpr_lds_slab + (size_t)wave * (GPR_MAX_DEPTH * 64 * KC);
+    float *ls = MALA ? nullptr : gpr_lds_slab + (size_t)wave * ((GPR_MAX_DEPTH + 2) * 64 * KC);
     float *lv = !MALA ? nullptr
                 : HI  ? a.mala_gslab + ((size_t)blockIdx.x * a.wpg + wave) * (size_t)a.dmax_cap * 64
                       : gpr_lds_slab + (size_t)wave * a.dmax_cap * 64;
@@ -430,7 +430,7 @@ k_mh_config_wave(GprMHArgs a) {
 template <int KC, bool HI, bool MALA, bool MET>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
     const size_t sh = MALA ? (HI ? 0 : sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg)
-                           : sizeof(float) * (size_t)a.wpg * GPR_MAX_DEPTH * 64 * KC;
+                           : sizeof(float) * (size_t)a.wpg * (GPR_MAX_DEPTH + 2) * 64 * KC;
     if (sh > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA, MET>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
