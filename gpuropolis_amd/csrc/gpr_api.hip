@@ -174,6 +174,12 @@ extern "C" int gpr_create(const gpr_config *cfg, gpr_ctx **out) {
         (void)gpr_destroy(c);
         return fail(GPR_E_HIP, "stream/event creation failed");
     }
+    // the kernels' table of integer logs (the same values on every call)
+    if (gpr_launch_fill_logint(c->stream) != hipSuccess || gpr_launch_fill_ref_logint(c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        (void)gpr_destroy(c);
+        return fail(GPR_E_HIP, "log table fill failed");
+    }
     *out = c;
     return GPR_OK;
 }

@@ -138,6 +138,8 @@ hipError_t gpr_launch_init_ref(const GprMHArgs &a, hipStream_t s);
 hipError_t gpr_launch_resample(const GprResampleArgs &a, hipStream_t s);
 hipError_t gpr_launch_resample_quantile(const GprResampleArgs &a, hipStream_t s);
 hipError_t gpr_launch_fill_i32(int *p, int n, int v, hipStream_t s);
+hipError_t gpr_launch_fill_logint(hipStream_t s);
+hipError_t gpr_launch_fill_ref_logint(hipStream_t s);
 hipError_t gpr_launch_clock_advance(uint32_t *clock, uint32_t steps, hipStream_t s);
 hipError_t gpr_launch_test_ref_decode(const uint8_t *hop, const float *hc, int nheaps, uint8_t *ops, float *cs,
                                       int32_t *meta, hipStream_t s);

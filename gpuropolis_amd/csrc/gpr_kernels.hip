@@ -27,6 +27,14 @@
 // per CU), and the current structure's plan in registers.
 extern __shared__ float gpr_lds_slab[];
 
+// gpr_logf of the integers 0 .. GPR_MAX_CAP (program lengths), filled once per device by
+// gpr_launch_fill_logint: the structure move's -log len terms become scalar loads.
+static __device__ float gpr_logint[GPR_MAX_CAP + 1];
+__global__ void k_fill_logint() {
+    const int i = (int)threadIdx.x;
+    if (i <= GPR_MAX_CAP) gpr_logint[i] = gpr_logf((float)i);
+}
+
 // The plain one-point-per-lane kernel is latency bound: six waves per SIMD (80 VGPRs) run
 // it faster than the four that fit unspilled, although the compiler then spills per-tile
 // values to scratch at every tile start and reloads some of them in the step loop.
@@ -212,8 +220,8 @@ k_mh_config_wave(GprMHArgs a) {
                     lq_old += gpr_gen_logp(o);
                     if (o == GPR_OP_CONST) lq_old += gpr_lphi((HI && t >= 64) ? rl_f(cur.c1, t & 63) : rl_f(cur.c0, t & 63));
                 }
-                const double fwd = -(double)gpr_logf((float)cur.len) + lq_new;
-                const double bwd = -(double)gpr_logf((float)plen) + lq_old;
+                const double fwd = -(double)gpr_logint[cur.len] + lq_new;
+                const double bwd = -(double)gpr_logint[plen] + lq_old;
                 extra = bwd - fwd;
             } else {
                 const bool k0 = l < cur.len && cur.o0 == GPR_OP_CONST;
@@ -968,6 +976,11 @@ __global__ void k_fill_i32(int *p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+hipError_t gpr_launch_fill_logint(hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_logint, dim3(1), dim3(GPR_MAX_CAP + 1), 0, s);
+    return hipGetLastError();
+}
+
 hipError_t gpr_launch_fill_i32(int *p, int n, int v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
     return hipGetLastError();

@@ -15,6 +15,18 @@
 //                  evaluator, per-constant proposals in parallel lanes.
 #include "gpr_device.h"
 
+// gpr_logf of the node counts 0 .. 31 (-inf at 0), filled once per device by
+// gpr_launch_fill_ref_logint: the wave kernel's -log count terms become scalar loads.
+static __device__ float gpr_ref_logint[GPR_REF_SLOTS + 1];
+__global__ void k_fill_ref_logint() {
+    const int i = (int)threadIdx.x;
+    if (i <= GPR_REF_SLOTS) gpr_ref_logint[i] = gpr_logf((float)i);
+}
+hipError_t gpr_launch_fill_ref_logint(hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_ref_logint, dim3(1), dim3(64), 0, s);
+    return hipGetLastError();
+}
+
 #define R_LOG01_F (-0x1.26bb1cp+1f)        /* logf(PROPOSE_GEOM_R = 0.1f) */
 #define R_LOG015_F (-0x1.e5a9a8p+0f)       /* logf(PROPOSE_INT = 0.15f) */
 #define R_LOG1M01_D (-0x1.af8e82826b325p-4) /* log(1.0 - 0.1f) */
@@ -546,7 +558,7 @@ k_mh_ref_wave(GprMHArgs a) {
             bool structural = (mcmci % 5) == 0;
             if (structural) {
                 n_sprop++;
-                forward = -(double)gpr_logf((float)cur.count);
+                forward = -(double)gpr_ref_logint[cur.count];
                 WavePhiloxSrc ph;
                 ph.st.init(a.seed, chain, step, GPR_P_STRUCT, 0);
                 const int nn = !r_valid(cur_post) ? 0 : r_random_int(GPR_REF_SLOTS, ph);
@@ -554,7 +566,7 @@ k_mh_ref_wave(GprMHArgs a) {
                     if (r_is_descendant(nn, i)) php = wl_i(php, gpr_opset_code(a.opset, r_random_int(nops, ph)), i);
                 }
                 prop = r_decode_wave(php, phc);
-                backward = -(double)gpr_logf((float)prop.count);
+                backward = -(double)gpr_ref_logint[prop.count];
             } else {
                 n_cprop++;
                 double ft = 0.0, bt = 0.0;
