@@ -12,7 +12,10 @@ roofline    the MH kernel: algorithmic FP32 ops per launch (n * (non-leaf tokens
             + constants, summed over every evaluated program, counted by the kernel)
             divided by the kernel's average launch time (HIP events on its stream),
             against the VALU lane-op issue peak (256 CU x 64 lanes x 2.4 GHz); the FP32
-            datasheet peak (157.3 TFLOP/s, packed FMA) is reported beside it.
+            datasheet peak (157.3 TFLOP/s, packed FMA) is reported beside it.  For the
+            default configuration it also carries the HBM bytes per launch and the share
+            of VALU and SALU issue slots in use, from rocprofv3 counters of the same
+            launch (MEASURED_TRAFFIC, MEASURED_ISSUE) over the live launch time.
 cpu_baseline the oracle's serial restatement of the same sampler (OpenMP over chains)
             on this host's cores, on a bounded sample of the same workload.
 """
@@ -32,7 +35,12 @@ HBM_PEAK = 8.0e12
 # HBM bytes per MH launch (FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc, scaled by the
 # same counters on a plain copy of known size) for the default configuration, keyed by
 # (schedule, chains, cap, MH steps per block, seed); the profiles/ summaries hold the runs.
-MEASURED_TRAFFIC = {("wave", 131072, 64, 1000, 0): 281613280}
+MEASURED_TRAFFIC = {("wave", 131072, 64, 1000, 0): 268564224}
+# VALU and SALU instructions per chain-step of the same launches (rocprofv3 --pmc
+# SQ_INSTS_VALU, SQ_INSTS_SALU over chains x steps; the profiles/ summary holds the run).
+# With the live launch time they give the share of issue slots each unit is busy: one
+# wave64 instruction per SIMD every 4 cycles, 1024 SIMDs at 2.4 GHz.
+MEASURED_ISSUE = {("wave", 131072, 64, 1000, 0): (330.9, 332.2)}
 
 METRIC = "MH steps/sec + node×datapoint evals/sec (% VALU peak); ESS/sec MALA vs RW"
 
@@ -52,6 +60,33 @@ def cpu_baseline(ds, cap, seed, sample_chains, sample_steps):
     return {"value": sample_chains * sample_steps / dt, "unit": "MH steps/s", "cores": threads, "kind": "port",
             "sample": f"{sample_chains} chains x {sample_steps} MH steps of the same workload "
                       f"(oracle/oracle.c, OpenMP, {dt:.1f} s)"}
+
+
+def step_split(args):
+    """Cost per chain-step of the two step types: programs from two warm-up blocks of the
+    large workload, then one block with structure moves off (constant steps only) and
+    one with a structure move at every step, each on the default launch shape."""
+    from gpuropolis_amd import lib as G
+    from gpuropolis_amd import workloads as W
+    ds, _ = W.dataset_L(args.seed)
+    ctx = G.Context(cap=args.cap, seed=args.seed, schedule=G.SCHED_WAVE)
+    ctx.set_data([ds], [args.chains])
+    ctx.init_chains()
+    for _ in range(2):
+        ctx.run_block(args.mh_steps)
+        ctx.resample()
+    ops, cs, ln, _, _ = ctx.read_state()
+    ctx.close()
+    out = {"mean_tokens": float(ln.mean())}
+    for name, kw in (("constant_steps", dict(fixed_structure=1)), ("structure_steps", dict(struct_every=1))):
+        ctx = G.Context(cap=args.cap, seed=args.seed, schedule=G.SCHED_WAVE, **kw)
+        ctx.set_data([ds], [args.chains])
+        ctx.write_state(ops, cs, ln)
+        ctx.run_block(args.mh_steps)
+        ms = ctx.stats().last_block_ms
+        out[name] = {"kernel_ms": ms, "ns_per_chain_step": ms * 1e6 / (args.chains * args.mh_steps)}
+        ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def copy_calibration(mib):
@@ -327,11 +362,17 @@ def main():
     ap.add_argument("--full-blocks", type=int, default=1, help="timed blocks of the full form (0: skip)")
     ap.add_argument("--tune-steps", type=int, default=50,
                     help="autotune the wave launch shapes on this many MH steps after warm-up (0: defaults)")
+    ap.add_argument("--step-split", action="store_true",
+                    help="only time one block of constant steps and one of structure steps on the large "
+                         "workload's programs (after two warm-up blocks) and print their cost per chain-step")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.step_split:
+        step_split(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -381,6 +422,11 @@ def main():
                      "peak_fp32_datasheet": FP32_DATASHEET_PEAK / 1e12,
                      "frac_fp32_datasheet": achieved / FP32_DATASHEET_PEAK},
     }
+    issue = MEASURED_ISSUE.get((args.schedule, args.chains, args.cap, args.mh_steps, args.seed))
+    if issue is not None:
+        slots = 1024 * 2.4e9 / 4 * avg_launch_s
+        chain_steps = args.chains * args.mh_steps
+        out["roofline"]["issue_frac"] = {"valu": issue[0] * chain_steps / slots, "salu": issue[1] * chain_steps / slots}
     traffic = MEASURED_TRAFFIC.get((args.schedule, args.chains, args.cap, args.mh_steps, args.seed))
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
