@@ -136,3 +136,34 @@ def test_early_rejection_is_exact(oracle, gpr, kc):
     say("chains on the oracle path:", int(same.sum()), "of", nch)
     assert same.sum() >= int(0.95 * nch)
     assert abs(stk.const_accepted - cnt[3]) <= 6 + 2 * (nch - same.sum())
+
+
+def test_work_counters_exact_on_fixed_structures(oracle, gpr):
+    """With structure moves off and one chunk of points (n <= 64, so no early rejection)
+    every block evaluates each chain's program once at its start and once per step when
+    it has a constant to move: node_evals = sum len * n * (1 + steps * has_const) and the
+    FP32 work count = sum ((len - leaves + 4) * n + constants) over the same evaluations."""
+    x, y, sd = s_data()
+    n, nch, steps, blocks = len(x), 256, 60, 2
+    assert n <= 64
+    cfg = oracle.make_cfg(cap=32, seed=5)
+    ops, cs, ln = oracle.cfg_init(cfg, nch)
+    X, C, ONE = gpr.OP["X"], gpr.OP["CONST"], gpr.OP["ONE"]
+    CADD = gpr.OP["CADD"]
+    with gpr.Context(cap=32, seed=5, fixed_structure=1, resample=gpr.RS_NONE, schedule=gpr.SCHED_WAVE) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(ops, cs, ln)
+        say("work counters on fixed structures")
+        for _ in range(blocks):
+            ctx.run_block(steps)
+        st = ctx.stats()
+    evals = flops = 0
+    for c in range(nch):
+        o = ops[:ln[c], c]
+        nconst = int(np.sum((o == C) | (o >= CADD)))
+        leaves = int(np.sum(o <= ONE))
+        k = blocks * (1 + steps * (nconst > 0))
+        evals += k * int(ln[c]) * n
+        flops += k * ((int(ln[c]) - leaves + 4) * n + nconst)
+    assert (X, C, ONE) == (0, 1, 2)
+    assert st.node_evals == evals and st.alg_flops == flops, (st.node_evals, evals, st.alg_flops, flops)
