@@ -135,12 +135,14 @@ def test_ref_op_set_init_and_resample_match_oracle(oracle, gpr, ops):
     assert np.array_equal(r_hc.view(np.uint32), o_hc.view(np.uint32))
 
 
-@pytest.mark.parametrize("sched,ops", [("wave", 2), ("wave", 4), ("lane", 4), ("wave", 3), ("lane", 1)])
-def test_ref_op_set_trajectories_match_oracle(oracle, gpr, sched, ops):
+@pytest.mark.parametrize("sched,ops,dname", [("wave", 2, "Misra1a"), ("wave", 4, "Misra1a"), ("lane", 4, "Misra1a"),
+                                             ("wave", 3, "Misra1a"), ("lane", 1, "Misra1a"), ("wave", 4, "Kirby2")])
+def test_ref_op_set_trajectories_match_oracle(oracle, gpr, sched, ops, dname):
     """The sampler with each op set follows the oracle's trajectories: moves draw the op
     set's codes, the prior uses its NOPS, and the physical constants and halving
-    evaluate as main.cu:429-441 on both schedules."""
-    x, y, sd = data("Misra1a")
+    evaluate as main.cu:429-441 on both schedules (Kirby2, n = 151: with the wave
+    kernel's early rejection)."""
+    x, y, sd = data(dname)
     nch, steps = 64, 200
     hop, hc = oracle.ref_init(6, nch, ops=ops)
     o_hop, o_hc = hop.copy(), hc.copy()
@@ -151,7 +153,7 @@ def test_ref_op_set_trajectories_match_oracle(oracle, gpr, sched, ops):
     with gpr.Context(semantics=gpr.SEM_REFERENCE, seed=77, ops=ops, schedule=sch) as ctx:
         ctx.set_data([(x, y, sd)], [nch])
         ctx.write_state(hop, hc)
-        say(f"op set {ops} trajectories sched={sched}")
+        say(f"op set {ops} trajectories sched={sched} data={dname}")
         ctx.run_block(steps)
         ctx.run_block(steps)
         g_hop, g_hc, _, g_pr, g_ll = ctx.read_state()
