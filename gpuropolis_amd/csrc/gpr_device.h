@@ -23,18 +23,18 @@ __device__ __forceinline__ int wl_i(int old, int v, int l) { return lane_id() ==
 __device__ __forceinline__ float wl_f(float old, float v, int l) { return lane_id() == l ? v : old; }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Lane gathers that stay in wave-uniform control flow.  ds_bpermute only sees the
-// values of lanes active when it executes; the empty asm pins the result at this point
-// so the compiler cannot sink the gather into a branch that deactivates source lanes.
+// Lane gathers that stay in wave-uniform control flow: lane l receives v of lane
+// src mod 64 (ds_bpermute reads the byte address' lane bits, so no index masking is
+// needed).  ds_bpermute only sees the values of lanes active when it executes; the
+// empty asm pins the result at this point so the compiler cannot sink the gather into a
+// branch that deactivates source lanes.
 __device__ __forceinline__ int shfl_all(int v, int src) {
-    int r = __shfl(v, src);
+    int r = __builtin_amdgcn_ds_bpermute(src << 2, v);
     __asm__ volatile("" : "+v"(r));
     return r;
 }
 __device__ __forceinline__ float shfl_all(float v, int src) {
-    float r = __shfl(v, src);
-    __asm__ volatile("" : "+v"(r));
-    return r;
+    return __builtin_bit_cast(float, shfl_all(__builtin_bit_cast(int, v), src));
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
