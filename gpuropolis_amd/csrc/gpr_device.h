@@ -36,6 +36,11 @@ __device__ __forceinline__ int shfl_all(int v, int src) {
 __device__ __forceinline__ float shfl_all(float v, int src) {
     return __builtin_bit_cast(float, shfl_all(__builtin_bit_cast(int, v), src));
 }
+__device__ __forceinline__ double shfl_all_d(double v, int src) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)shfl_all((int)(uint32_t)b, src), hi = (uint32_t)shfl_all((int)(uint32_t)(b >> 32), src);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 // DPP controls (gfx9 encoding): quad_perm, row_shr:n, row_mirror, row_half_mirror,
@@ -537,15 +542,6 @@ __device__ double wave_prior(const WaveProg &p, double lnsym) {
         }
     }
     return gpr_dl_prior(dl, lnsym, s);
-}
-
-// wave_prior of a program whose constant-carrying tokens are its K constant tokens, from
-// lane base + r of lf holding gpr_lcauchy_logf of the r-th constant (same terms, same order)
-__device__ __forceinline__ double prior_batched(int len, double lnsym, float lf, int base, int K) {
-#pragma clang fp contract(off)
-    double s = 0.0;
-    for (int r = 0; r < K; r++) s += -(double)GPR_LOGPI_F - (double)rl_f(lf, base + r);
-    return gpr_dl_prior(len, lnsym, s);
 }
 
 // Subtree [s0, j] ending at token j: s0-1 is the last u < j whose stack depth after u is

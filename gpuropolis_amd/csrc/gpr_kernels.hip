@@ -49,6 +49,9 @@ k_mh_config_wave(GprMHArgs a) {
     const int N = a.nchains;
     // dynamic LDS: the value stacks of the wpg waves, or under MALA their token-value slabs
     float *ls = MALA ? nullptr : gpr_lds_slab + (size_t)wave * ((GPR_MAX_DEPTH + 2) * 64 * KC);
+    // after the wpg value stacks: each wave's priors of the (up to 16) steps of its draw batch
+    double *bpri = MALA ? nullptr
+                        : (double *)(gpr_lds_slab + (size_t)a.wpg * ((GPR_MAX_DEPTH + 2) * 64 * KC)) + wave * 16;
     float *lv = !MALA ? nullptr
                 : HI  ? a.mala_gslab + ((size_t)blockIdx.x * a.wpg + wave) * (size_t)a.dmax_cap * 64
                       : gpr_lds_slab + (size_t)wave * a.dmax_cap * 64;
@@ -108,12 +111,12 @@ k_mh_config_wave(GprMHArgs a) {
         // batch covers 4 steps (16 lanes each, up to 16 constants) or, for programs of at
         // most 4 constants, 16 steps (4 lanes each).  At the first constant step of a batch,
         // lane j = ks u + r takes the normal of step u's word for the r-th constant token
-        // and, from the current constants, the proposed constant and the logf of its Cauchy
-        // term; each step then reads its lanes (proposal and prior), the same values as the
-        // per-step path.  Any accepted move drops the batch (the next constant step
+        // and, from the current constants, the proposed constant and its Cauchy term; the
+        // batch also sums each step's prior into bpri.  Each step then reads its proposal
+        // lanes and its prior, the same values as the per-step path.  Any accepted move drops the batch (the next constant step
         // recomputes it).
         constexpr bool BATCH = !MALA;
-        float cb = 0.0f, lfb = 0.0f;
+        float cb = 0.0f;
         uint32_t zblk = 0xffffffffu;
         // MALA: gradient of the current state (lane-distributed like the constants) and its
         // metric drift M g at the metric rows
@@ -154,8 +157,8 @@ k_mh_config_wave(GprMHArgs a) {
             double extra = 0.0;
             bool have_pll = false;
             double pll_m = 0.0;
-            bool bprior = false;   // the prior comes from the batch: lanes bbase .. bbase+bK-1
-            int bbase = 0, bK = 0;
+            bool bprior = false;   // the prior comes from the batch: bpri[bu]
+            int bu = 0;
             const bool structural = struct_slot && !a.fixed_structure;
             if (i < 0) {
             } else if (structural) {
@@ -251,15 +254,20 @@ k_mh_config_wave(GprMHArgs a) {
                         const gpr_u4 wb = gpr_step_block(a.seed, chain, sj, GPR_P_CPROP, (uint32_t)tr, 0);
                         const float zb = gpr_normal(gpr_u01(gpr_u4_word(wb, (int)(sj & 3))));
                         cb = gpr_axpy(a.sigma, zb, shfl_all(cur.c0, tr));
-                        lfb = gpr_lcauchy_logf(cb);
+                        // the prior of each step's proposal, summed in token order as
+                        // wave_prior: lanes ks u .. ks u + K - 1 hold its constants' terms
+                        const double t = -(double)GPR_LOGPI_F - (double)gpr_lcauchy_logf(cb);
+                        const int K = __popcll(mk0);
+                        double sacc = 0.0;
+                        for (int q = 0; q < K; q++) sacc += shfl_all_d(t, (l & ~(ks - 1)) + q);
+                        if (r == 0) bpri[l >> (6 - lg)] = gpr_dl_prior(cur.len, a.lnsym, sacc);
                     }
                     const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk0 >> 32),
                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mk0, 0u));
                     const float cv = shfl_all(cb, ks * u + rank);
                     if (k0) prop.c0 = cv;
                     bprior = true;
-                    bbase = ks * u;
-                    bK = __popcll(mk0);
+                    bu = u;
                 } else {
                 if (BATCH || (step >> 2) != wblk) {
                     wblk = step >> 2;
@@ -337,7 +345,7 @@ k_mh_config_wave(GprMHArgs a) {
                 }
                 }
             }
-            const double pprior = bprior ? prior_batched(prop.len, a.lnsym, lfb, bbase, bK) : wave_prior<HI>(prop, a.lnsym);
+            const double pprior = bprior ? bpri[bu] : wave_prior<HI>(prop, a.lnsym);
             // token classes: the current program's, or the structural proposal's
             GradPlan pplan{};
             OpMasks pmask{};
@@ -438,7 +446,7 @@ k_mh_config_wave(GprMHArgs a) {
 template <int KC, bool HI, bool MALA, bool MET>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
     const size_t sh = MALA ? (HI ? 0 : sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg)
-                           : sizeof(float) * (size_t)a.wpg * (GPR_MAX_DEPTH + 2) * 64 * KC;
+                           : (sizeof(float) * (GPR_MAX_DEPTH + 2) * 64 * KC + sizeof(double) * 16) * (size_t)a.wpg;
     if (sh > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA, MET>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
