@@ -473,6 +473,23 @@ extern "C" int gpr_read_state(gpr_ctx *c, uint8_t *ops, float *cs, int32_t *len,
     return GPR_OK;
 }
 
+// The host copies of the clock are authoritative between calls: graph replays upload
+// them before the launch and every block advances them on return.
+extern "C" int gpr_get_clock(const gpr_ctx *c, uint32_t *step, uint32_t *block) {
+    if (!c || !step || !block) return fail(GPR_E_ARG, "null argument");
+    *step = c->step;
+    *block = c->block;
+    return GPR_OK;
+}
+
+extern "C" int gpr_set_clock(gpr_ctx *c, uint32_t step, uint32_t block) {
+    if (!c) return fail(GPR_E_ARG, "null context");
+    if (!c->have_data) return fail(GPR_E_STATE, "gpr_set_data must come first (it resets the clock)");
+    c->step = step;
+    c->block = block;
+    return GPR_OK;
+}
+
 // The persistent wave-schedule launch of one size class (its tile counter must be 0).
 static hipError_t launch_bucket(gpr_ctx *c, const GprMHArgs &a, int b, hipStream_t s) {
     const KernelBucket &kb = c->buckets[b];

@@ -5,6 +5,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <queue>
 
 #include "gpuropolis.h"
 
@@ -364,6 +368,82 @@ std::string tape_code(const uint8_t *ops, int len) {
     std::string s;
     for (int t = 0; t < len; t++) s += ops[t] < GPR_NUM_OPS ? code[ops[t]] : '?';
     return s;
+}
+
+// ------------------------------------------------------------------ tops.txt (run-all.sh:45)
+namespace {
+
+struct TopLine {
+    int cls;         // 0 key does not parse, 1 NaN, 2 number
+    long double v;   // the key when cls == 2
+    std::string s;   // the line without its newline
+};
+
+TopLine top_key(std::string s) {
+    const char *p = s.c_str(), *e = p + s.size();
+    auto blank = [](char c) { return c == ' ' || c == '\t'; };
+    for (int f = 0; f < 2 && p < e; f++) {   // skip fields 1 and 2 (blanks, then non-blanks)
+        while (p < e && blank(*p)) p++;
+        while (p < e && !blank(*p)) p++;
+    }
+    TopLine t{0, 0.0L, std::move(s)};
+    char *end = nullptr;
+    const long double v = strtold(p, &end);
+    if (end != p) {
+        t.cls = std::isnan(v) ? 1 : 2;
+        t.v = v;
+    }
+    return t;
+}
+
+// strict "a sorts before b" of `sort -g -k3` with its whole-line last resort
+bool top_less(const TopLine &a, const TopLine &b) {
+    if (a.cls != b.cls) return a.cls < b.cls;
+    if (a.cls == 2 && a.v != b.v) return a.v < b.v;
+    // sort orders NaNs by their bit pattern: the quiet NaN strtold returns for "nan"
+    // before the one it returns for "-nan"
+    if (a.cls == 1 && std::signbit(a.v) != std::signbit(b.v)) return !std::signbit(a.v);
+    return a.s < b.s;   // std::string compares bytes as unsigned char, as memcmp does
+}
+
+struct TopGreater {
+    bool operator()(const TopLine &a, const TopLine &b) const { return top_less(b, a); }
+};
+
+}  // namespace
+
+bool write_tops(const char *samples, const char *out, long k, std::string &err) {
+    std::ifstream in(samples, std::ios::binary);
+    if (!in) {
+        err = std::string("cannot open ") + samples;
+        return false;
+    }
+    // min-heap of the k largest lines seen so far
+    std::priority_queue<TopLine, std::vector<TopLine>, TopGreater> heap;
+    std::string line;
+    while (k > 0 && std::getline(in, line)) {
+        TopLine t = top_key(std::move(line));
+        if ((long)heap.size() < k) heap.push(std::move(t));
+        else if (top_less(heap.top(), t)) {
+            heap.pop();
+            heap.push(std::move(t));
+        }
+        line.clear();
+    }
+    std::vector<TopLine> v;
+    v.reserve(heap.size());
+    while (!heap.empty()) {
+        v.push_back(heap.top());
+        heap.pop();
+    }
+    FILE *fp = fopen(out, "w");
+    if (!fp) {
+        err = std::string("cannot write ") + out;
+        return false;
+    }
+    bool ok = true;
+    for (const TopLine &t : v) ok = ok && fwrite(t.s.data(), 1, t.s.size(), fp) == t.s.size() && fputc('\n', fp) != EOF;
+    return fclose(fp) == 0 && ok;
 }
 
 }  // namespace gpr_host

@@ -35,4 +35,12 @@ float tape_length(const uint8_t *ops, int len);
 float tape_degree(const uint8_t *ops, const float *cs, int len);
 std::string tape_code(const uint8_t *ops, int len);
 
+// tops.txt: the k lines of `samples` with the largest third field, in increasing order,
+// as run-all.sh:45 makes it with `sort -g -k3 samples.txt | tail -n 10000` (C locale).
+// Keys follow sort's general-numeric rule: fields start at a blank-to-non-blank
+// transition, the key is strtold of field 3 with its leading blanks, lines whose key does
+// not parse come first, then NaN, then numbers in order; equal keys fall back to a
+// bytewise comparison of the whole line.  Only k lines are held in memory.
+bool write_tops(const char *samples, const char *out, long k, std::string &err);
+
 }  // namespace gpr_host

@@ -58,12 +58,13 @@ struct Options {
     float resample_quantile = -1.0f;
     int log = 1;
     std::string state_in, state_out;
+    long tops = 0;
 };
 
 enum {
     OPT_SEMANTICS = 1000, OPT_CAP, OPT_DMAX, OPT_NSYM, OPT_SCHEDULE, OPT_PROPOSAL, OPT_SIGMA, OPT_EPS,
     OPT_STRUCT_EVERY, OPT_FIXED, OPT_RNG, OPT_RESAMPLE, OPT_NATS, OPT_QUANTILE, OPT_NOLOG, OPT_STATE_IN,
-    OPT_STATE_OUT, OPT_OPS, OPT_HELP
+    OPT_STATE_OUT, OPT_OPS, OPT_TOPS, OPT_HELP
 };
 
 const struct option kLongOptions[] = {
@@ -102,6 +103,7 @@ const struct option kLongOptions[] = {
     {"no-log", no_argument, nullptr, OPT_NOLOG},
     {"state-in", required_argument, nullptr, OPT_STATE_IN},
     {"state-out", required_argument, nullptr, OPT_STATE_OUT},
+    {"tops", required_argument, nullptr, OPT_TOPS},
     {"help", no_argument, nullptr, OPT_HELP},
     {nullptr, 0, nullptr, 0}};
 
@@ -114,7 +116,7 @@ const char *kUsage =
     "                  [--struct-every 5] [--fixed-structure] [--rng philox|lcg]\n"
     "                  [--ops default|simplified|polynomial|linear|physical]\n"
     "                  [--resample none|map|quantile] [--resample-nats 100] [--resample-quantile 0.5]\n"
-    "                  [--no-log] [--state-in FILE] [--state-out FILE]\n";
+    "                  [--no-log] [--state-in FILE] [--state-out FILE] [--tops K]\n";
 
 bool pick(const char *arg, std::initializer_list<std::pair<const char *, int>> names, int &out) {
     for (auto &p : names)
@@ -182,6 +184,7 @@ int parse(int argc, char **argv, Options &o) {
         case OPT_NOLOG: o.log = 0; break;
         case OPT_STATE_IN: o.state_in = optarg; break;
         case OPT_STATE_OUT: o.state_out = optarg; break;
+        case OPT_TOPS: o.tops = atol(optarg); break;
         case OPT_HELP: fputs(kUsage, stdout); exit(0);
         default: return 1;
         }
@@ -259,31 +262,63 @@ void write_rows(FILE *fp, const Options &o, int block, int rows, const std::vect
     }
 }
 
-// Binary chain state: magic, semantics, rows, N, then ops, cs, len.
-bool save_state(const std::string &path, int sem, int rows, int N, const std::vector<uint8_t> &ops,
-                const std::vector<float> &cs, const std::vector<int32_t> &len) {
+// Binary chain state.  Header: magic, semantics, rows, N, then (current format) the
+// seed, the Philox clock (next step, next resample block) and the number of blocks run;
+// then ops [rows][N] u8, cs [rows][N] f32, len [N] i32.  Files of the older format (no
+// seed, clock or block count) still load: they resume with a fresh clock.
+constexpr int32_t kStateMagicV1 = 0x47505253, kStateMagic = 0x32525047;
+
+struct StateClock {
+    int32_t seed = 0;
+    uint32_t step = 0, block = 0;
+    int32_t blocks_done = 0;
+    bool has_clock = false;
+};
+
+bool save_state(const std::string &path, int sem, int rows, int N, const StateClock &clk,
+                const std::vector<uint8_t> &ops, const std::vector<float> &cs, const std::vector<int32_t> &len) {
     FILE *fp = fopen(path.c_str(), "wb");
     if (!fp) return false;
-    const int32_t hdr[4] = {0x47505253, sem, rows, N};
+    const int32_t hdr[8] = {kStateMagic, sem, rows, N, clk.seed, (int32_t)clk.step, (int32_t)clk.block,
+                            clk.blocks_done};
     bool ok = fwrite(hdr, sizeof hdr, 1, fp) == 1 && fwrite(ops.data(), 1, ops.size(), fp) == ops.size() &&
               fwrite(cs.data(), sizeof(float), cs.size(), fp) == cs.size() &&
               fwrite(len.data(), sizeof(int32_t), len.size(), fp) == len.size();
     return fclose(fp) == 0 && ok;
 }
 
-bool load_state(const std::string &path, int sem, int rows, int N, std::vector<uint8_t> &ops, std::vector<float> &cs,
-                std::vector<int32_t> &len, std::string &err) {
+bool load_state(const std::string &path, int sem, int rows, int N, StateClock &clk, std::vector<uint8_t> &ops,
+                std::vector<float> &cs, std::vector<int32_t> &len, std::string &err) {
     FILE *fp = fopen(path.c_str(), "rb");
     if (!fp) {
         err = "cannot open state file " + path;
         return false;
     }
-    int32_t hdr[4];
-    bool ok = fread(hdr, sizeof hdr, 1, fp) == 1;
-    if (ok && (hdr[0] != 0x47505253 || hdr[1] != sem || hdr[2] != rows || hdr[3] != N)) {
+    int32_t hdr[8];
+    bool ok = fread(hdr, sizeof(int32_t), 4, fp) == 4;
+    if (ok && hdr[0] != kStateMagic && hdr[0] != kStateMagicV1) {
+        err = "state file " + path + " is not a gpuropolis state file";
+        fclose(fp);
+        return false;
+    }
+    if (ok && (hdr[1] != sem || hdr[2] != rows || hdr[3] != N)) {
         err = "state file " + path + " does not match --semantics/--cap/--N";
         fclose(fp);
         return false;
+    }
+    clk = StateClock{};
+    if (ok && hdr[0] == kStateMagic) {
+        ok = fread(hdr + 4, sizeof(int32_t), 4, fp) == 4;
+        clk.seed = hdr[4];
+        clk.step = (uint32_t)hdr[5];
+        clk.block = (uint32_t)hdr[6];
+        clk.blocks_done = hdr[7];
+        clk.has_clock = true;
+        if (ok && clk.blocks_done < 0) {
+            err = "state file " + path + " has a negative block count";
+            fclose(fp);
+            return false;
+        }
     }
     ok = ok && fread(ops.data(), 1, ops.size(), fp) == ops.size() &&
          fread(cs.data(), sizeof(float), cs.size(), fp) == cs.size() &&
@@ -307,8 +342,8 @@ int main(int argc, char **argv) {
         return 1;
     }
     // main.cu:1024-1031
-    if (o.steps <= 0 || o.N <= 0 || o.burn < 0 || o.thin < 1 || o.outer < 0) {
-        fprintf(stderr, "*** ERROR: need --steps > 0, --N > 0, --burn >= 0, --thin >= 1, --outer >= 0\n");
+    if (o.steps <= 0 || o.N <= 0 || o.burn < 0 || o.thin < 1 || o.outer < 0 || o.tops < 0) {
+        fprintf(stderr, "*** ERROR: need --steps > 0, --N > 0, --burn >= 0, --thin >= 1, --outer >= 0, --tops >= 0\n");
         return 1;
     }
     const bool ref = o.semantics == GPR_SEM_REFERENCE;
@@ -332,13 +367,27 @@ int main(int argc, char **argv) {
     if (o.resample_nats >= 0) cfg.resample_nats = o.resample_nats;
     if (o.resample_quantile >= 0) cfg.resample_quantile = o.resample_quantile;
 
+    const int rows = ref ? GPR_REF_SLOTS : o.cap;
+    std::vector<uint8_t> ops((size_t)rows * o.N);
+    std::vector<float> cs((size_t)rows * o.N);
+    std::vector<int32_t> len(o.N);
+    std::vector<double> prior(o.N), ll(o.N);
+    std::string err;
+    // A state file of the current format carries its run's seed and clock: the resumed
+    // run continues that run's random streams and block numbering, whatever --seed says.
+    StateClock clk;
+    if (!o.state_in.empty() && !load_state(o.state_in, o.semantics, rows, o.N, clk, ops, cs, len, err)) {
+        fprintf(stderr, "*** ERROR: %s\n", err.c_str());
+        return 1;
+    }
+
     // main.cu:1071-1073
     int seed = o.seed;
     if (seed == -1) seed = (int)time(nullptr);
+    if (clk.has_clock) seed = clk.seed;
     cfg.seed = (uint64_t)(uint32_t)seed;
 
     std::vector<Datum> D;
-    std::string err;
     if (!load_data_file(o.in.c_str(), o.sdscale, o.first_half, o.even_half, D, err)) {
         fprintf(stderr, "%s\n", err.c_str());
         return 1;
@@ -359,17 +408,9 @@ int main(int argc, char **argv) {
     const int32_t nd = (int32_t)D.size(), nch = o.N;
     if (gpr_set_data(ctx, 1, &nd, x.data(), y.data(), sd.data(), &nch, nullptr) != GPR_OK) return die("gpr_set_data");
 
-    const int rows = ref ? GPR_REF_SLOTS : o.cap;
-    std::vector<uint8_t> ops((size_t)rows * o.N);
-    std::vector<float> cs((size_t)rows * o.N);
-    std::vector<int32_t> len(o.N);
-    std::vector<double> prior(o.N), ll(o.N);
     if (!o.state_in.empty()) {
-        if (!load_state(o.state_in, o.semantics, rows, o.N, ops, cs, len, err)) {
-            fprintf(stderr, "*** ERROR: %s\n", err.c_str());
-            return 1;
-        }
         if (gpr_write_state(ctx, ops.data(), cs.data(), len.data()) != GPR_OK) return die("gpr_write_state");
+        if (gpr_set_clock(ctx, clk.step, clk.block) != GPR_OK) return die("gpr_set_clock");
     } else if (gpr_init_chains(ctx) != GPR_OK) {
         return die("gpr_init_chains");
     }
@@ -401,8 +442,12 @@ int main(int argc, char **argv) {
     }
 
     srand(seed);  // main.cu:1107
+    // A resumed run numbers its blocks on from the saved run's and skips the block
+    // seeds that run drew, so burn-in, thinning and the LCG seeds continue as one run.
+    const int b0 = clk.blocks_done;
+    for (int b = 0; b < b0; b++) (void)rand();
     if (!o.quiet) progress_bar(0.0f);
-    for (int b = 0; b < o.outer; b++) {
+    for (int b = b0; b < b0 + o.outer; b++) {
         gpr_reset_stats(ctx);
         const auto t0 = std::chrono::steady_clock::now();
         if (gpr_run_block(ctx, o.steps, rand()) != GPR_OK) return die("gpr_run_block");
@@ -431,19 +476,28 @@ int main(int argc, char **argv) {
                     st.map_posterior);
             fflush(lg);
         }
-        if (!o.quiet) progress_bar((float)b / (float)o.outer);
+        if (!o.quiet) progress_bar((float)(b - b0) / (float)o.outer);
     }
     if (!o.quiet) progress_bar(1.1f);
     if (!o.state_out.empty()) {
         if (gpr_read_state(ctx, ops.data(), cs.data(), len.data(), nullptr, nullptr) != GPR_OK)
             return die("gpr_read_state");
-        if (!save_state(o.state_out, o.semantics, rows, o.N, ops, cs, len)) {
+        StateClock out;
+        out.seed = seed;
+        out.blocks_done = b0 + o.outer;
+        if (gpr_get_clock(ctx, &out.step, &out.block) != GPR_OK) return die("gpr_get_clock");
+        if (!save_state(o.state_out, o.semantics, rows, o.N, out, ops, cs, len)) {
             fprintf(stderr, "*** ERROR: cannot write state file %s\n", o.state_out.c_str());
             return 1;
         }
     }
     if (lg) fclose(lg);
     gpr_destroy(ctx);
+    // run-all.sh:45: sort -g -k3 samples.txt | tail -n K > tops.txt
+    if (o.tops > 0 && !write_tops(sample_path.c_str(), (o.out + "/tops.txt").c_str(), o.tops, err)) {
+        fprintf(stderr, "*** ERROR: %s\n", err.c_str());
+        return 1;
+    }
     fprintf(stderr, " Completed. \n");
     return 0;
 }

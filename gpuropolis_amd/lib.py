@@ -33,7 +33,8 @@ EXPORTS = ["gpr_config_default", "gpr_create", "gpr_destroy", "gpr_last_error", 
            "gpr_init_chains", "gpr_write_state", "gpr_read_state", "gpr_run_block", "gpr_run_block_async",
            "gpr_resample", "gpr_run_blocks_graph", "gpr_get_stats", "gpr_reset_stats", "gpr_sync", "gpr_stream",
            "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws",
-           "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning", "gpr_set_metric", "gpr_ref_op_set"]
+           "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning", "gpr_set_metric", "gpr_ref_op_set",
+           "gpr_get_clock", "gpr_set_clock"]
 
 
 class Config(C.Structure):
@@ -85,6 +86,8 @@ def load():
         "gpr_init_chains": (C.c_int, [vp]),
         "gpr_write_state": (C.c_int, [vp, vp, vp, vp]),
         "gpr_read_state": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+        "gpr_get_clock": (C.c_int, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "gpr_set_clock": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
         "gpr_run_block": (C.c_int, [vp, C.c_int, C.c_int32]),
         "gpr_run_block_async": (C.c_int, [vp, C.c_int, C.c_int32]),
         "gpr_resample": (C.c_int, [vp]),
@@ -194,6 +197,15 @@ class Context:
         ll = np.zeros(self.N, dtype=np.float64)
         check(load().gpr_read_state(self.h, _p(ops), _p(cs), _p(ln), _p(pr), _p(ll)))
         return ops, cs, ln, pr, ll
+
+    def clock(self):
+        """(step, block): where the next block's and the next resample's Philox streams start."""
+        st, bl = C.c_uint32(), C.c_uint32()
+        check(load().gpr_get_clock(self.h, C.byref(st), C.byref(bl)))
+        return st.value, bl.value
+
+    def set_clock(self, step, block):
+        check(load().gpr_set_clock(self.h, step, block))
 
     def run_block(self, steps, block_seed=0):
         check(load().gpr_run_block(self.h, steps, block_seed))

@@ -140,6 +140,15 @@ int gpr_init_chains(gpr_ctx *ctx);
 int gpr_write_state(gpr_ctx *ctx, const uint8_t *ops, const float *cs, const int32_t *len);
 int gpr_read_state(gpr_ctx *ctx, uint8_t *ops, float *cs, int32_t *len, double *prior, double *ll);
 
+/* The Philox clock: `step` is the MH step index the next block starts at, `block` the
+ * block index the next resample draws with.  gpr_set_data resets both to 0; every block
+ * advances step by its length and block by one.  No counterpart in the reference, which
+ * has no checkpoint (run-all.sh:43 runs one process to the end): a checkpoint saves the
+ * clock with the chains, and gpr_write_state followed by gpr_set_clock continues the
+ * same random streams, so the resumed chains are bit-identical to an uninterrupted run. */
+int gpr_get_clock(const gpr_ctx *ctx, uint32_t *step, uint32_t *block);
+int gpr_set_clock(gpr_ctx *ctx, uint32_t step, uint32_t block);
+
 /* One MCMC block of `steps` MH steps on every chain: the launch of MH_simple_kernel
  * (main.cu:1160, kernel main.cu:679-813).  block_seed plays the role of the rand()
  * seed argument (used by GPR_RNG_LCG); Philox streams use cfg->seed and the global

@@ -43,6 +43,15 @@ int shim_heap_code(const uint8_t *P, int opset, char *buf, int cap) { return put
 float shim_heap_length(const uint8_t *P) { return heap_length(P); }
 float shim_heap_degree(const uint8_t *P, const float *C) { return heap_degree(P, C); }
 
+int shim_tops(const char *in, const char *out, long k, char *err, int errcap) {
+    std::string e;
+    if (!write_tops(in, out, k, e)) {
+        put(e, err, errcap);
+        return -1;
+    }
+    return 0;
+}
+
 int shim_tape(const uint8_t *ops, const float *cs, int len, int which, const char *cf, char *buf, int cap) {
     if (which == 0) return put(tape_string(ops, cs, len, cf), buf, cap);
     if (which == 1) return put(tape_latex(ops, cs, len, cf), buf, cap);

@@ -296,3 +296,77 @@ def test_cli_reports_data_errors_before_touching_a_device(cli, tmp_path):
     r = run_cli(cli, "--in", str(p), "--out", str(tmp_path / "o"))
     assert r.returncode == 1 and "*** ERROR IN PARSING INPUT" in r.stderr
     assert not (tmp_path / "o" / "samples.txt").exists()
+
+
+def test_cli_reports_state_file_errors_before_touching_a_device(cli, tmp_path):
+    data = tmp_path / "d.txt"
+    data.write_text("1 2 1\n2 3 1\n")
+    common = ["--in", str(data), "--out", str(tmp_path / "o"), "--N", "8"]
+    r = run_cli(cli, *common, "--state-in", str(tmp_path / "missing.bin"))
+    assert r.returncode == 1 and "cannot open state file" in r.stderr
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"\0" * 64)
+    r = run_cli(cli, *common, "--state-in", str(bad))
+    assert r.returncode == 1 and "not a gpuropolis state file" in r.stderr
+    other = tmp_path / "other.bin"   # current-format header for N = 16, reference semantics
+    other.write_bytes(np.array([0x32525047, 1, 31, 16, 1, 0, 0, 0], np.int32).tobytes())
+    r = run_cli(cli, *common, "--state-in", str(other))
+    assert r.returncode == 1 and "does not match" in r.stderr
+    short = tmp_path / "short.bin"
+    short.write_bytes(np.array([0x32525047, 1, 31, 8, 1, 0, 0, 0], np.int32).tobytes() + b"\1" * 31)
+    r = run_cli(cli, *common, "--state-in", str(short))
+    assert r.returncode == 1 and "truncated" in r.stderr
+    r = run_cli(cli, *common, "--tops", "-1")
+    assert r.returncode == 1
+
+
+# ------------------------------------------------------------------------ tops.txt
+def _tops(io, src, out, k):
+    err = C.create_string_buffer(4096)
+    assert io.shim_tops(str(src).encode(), str(out).encode(), C.c_long(k), err, len(err)) == 0, err.value
+    return out.read_bytes()
+
+
+def _sort_tail(src, k):
+    env = dict(os.environ, LC_ALL="C")
+    s = subprocess.run(["sort", "-g", "-k3", str(src)], capture_output=True, check=True, env=env).stdout
+    lines = s.splitlines(keepends=True)
+    return b"".join(lines[len(lines) - k:]) if k else b""
+
+
+def _samples_like(rng, nrows):
+    """Header, datum lines and rows shaped as samples.txt, with ties, infinities and
+    lines too short to have a third field."""
+    out = ["# ----", "# -- Parameters:", "# \tInput data path: d.txt", "#", "#"]
+    out += ["# \t%f\t%f\t%f" % (x, 3 * x - 1, 0.5) for x in rng.normal(0, 30, 12)]
+    post = np.round(rng.normal(-40, 25, nrows), 1)
+    post[rng.integers(0, nrows, nrows // 5)] = post[0]          # ties on the key
+    for i, p in enumerate(post):
+        body = "%.3f\t%.3f\t%.3f\t%.1f\tNA\t\"x\"\t\"x\"\t\"x\"\t\"%s\"\t" % (p, p / 3, p - p / 3, 3.0, "ab"[i % 2] * 5)
+        out.append("%d\t%d\t%s" % (i % 97, i // 97, body))
+    out += ["%d\t%d\t-inf\t-inf\t0.000\t" % (5, 1), "%d\t%d\tinf\t1.000\t1.000\t" % (6, 1),
+            "%d\t%d\tnan\t1\t1\t" % (7, 1), "%d\t%d\tnan\t1\t1\t" % (2, 1), "3\t1\t-nan\t1", "2\t1\t-nan\t1", "only two", "", "  7\t8\t  1.5e1 z",
+            "1 2 3.25e-1", "1\t2\t+0x1p3", "1\t2\t-0.000", "1\t2\t0.000"]
+    rng.shuffle(out)
+    return "\n".join(out) + "\n"
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_tops_matches_sort_g_k3_tail(io, tmp_path, seed):
+    """tops.txt equals `LC_ALL=C sort -g -k3 samples.txt | tail -n K` (run-all.sh:45)."""
+    rng = np.random.default_rng(seed)
+    src = tmp_path / "samples.txt"
+    src.write_text(_samples_like(rng, 3000))
+    nlines = len(src.read_bytes().splitlines())
+    for k in (0, 1, 7, 100, 2999, nlines, nlines + 50):
+        assert _tops(io, src, tmp_path / "tops.txt", k) == _sort_tail(src, min(k, nlines)), k
+
+
+def test_tops_without_trailing_newline_and_errors(io, tmp_path):
+    src = tmp_path / "s.txt"
+    src.write_bytes(b"1\t1\t-2.5\ta\n1\t2\t9\tb\n1\t3\t4")
+    assert _tops(io, src, tmp_path / "t.txt", 2) == _sort_tail(src, 2) == b"1\t3\t4\n1\t2\t9\tb\n"
+    err = C.create_string_buffer(4096)
+    assert io.shim_tops(str(tmp_path / "none.txt").encode(), str(tmp_path / "t.txt").encode(), C.c_long(3), err,
+                        len(err)) == -1
+    assert b"cannot open" in err.value

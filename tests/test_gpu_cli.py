@@ -129,10 +129,11 @@ def test_cli_config_semantics_and_state_files(cli, tmp_path):
     text = run(cli, tmp_path / "a", "--semantics", "config", "--cap", "32", "--N", "256", "--steps", "200",
                "--outer", "3", "--seed", "5", "--show-constants", "--state-out", st)
     check_rows(rows_of(text), 256, [1, 2], 32, d, ref=False)
-    assert os.path.getsize(st) == 16 + 256 * 32 * 5 + 256 * 4
+    assert os.path.getsize(st) == 32 + 256 * 32 * 5 + 256 * 4
+    # the resumed run numbers its blocks on from the saved run's three
     text2 = run(cli, tmp_path / "b", "--semantics", "config", "--cap", "32", "--N", "256", "--steps", "200",
                 "--outer", "2", "--seed", "6", "--state-in", st)
-    assert len(rows_of(text2)) == 256
+    assert sorted({int(r.split("\t")[1]) for r in rows_of(text2)}) == [3, 4] and len(rows_of(text2)) == 512
     r = subprocess.run([cli, "--in", DATA, "--out", str(tmp_path / "c"), "--quiet", "--semantics", "config",
                         "--cap", "64", "--N", "256", "--state-in", st], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "does not match" in r.stderr
@@ -151,3 +152,38 @@ def test_cli_op_sets(cli, tmp_path):
         used = {ch for r in rows for ch in r.split("\t")[10][1:-1]}
         say(name, "codes printed:", "".join(sorted(used)))
         assert len(used) >= len(code) - 2
+
+
+@pytest.mark.parametrize("args", [
+    ("--N", "256", "--steps", "100"),                                                   # heap-31, quantile rule
+    ("--N", "256", "--steps", "100", "--rng", "lcg", "--schedule", "lane"),             # the reference's LCG
+    ("--semantics", "config", "--cap", "32", "--N", "256", "--steps", "100"),           # postfix, MAP rule
+    ("--semantics", "config", "--cap", "32", "--N", "128", "--steps", "60", "--proposal", "mala"),
+], ids=["ref", "ref-lcg", "config", "config-mala"])
+def test_cli_resume_is_bit_identical_to_one_run(cli, tmp_path, args):
+    """Four blocks in one run, and two runs of two blocks joined by --state-out/--state-in,
+    print the same rows and leave the same state file: the state file carries the seed,
+    the Philox clock and the block count, and the resumed run skips the saved run's
+    block seeds."""
+    one, half, end = (str(tmp_path / f) for f in ("one.bin", "half.bin", "end.bin"))
+    text = run(cli, tmp_path / "one", *args, "--outer", "4", "--seed", "11", "--show-constants", "--state-out", one)
+    run(cli, tmp_path / "two", *args, "--outer", "2", "--seed", "11", "--show-constants", "--state-out", half)
+    text2 = run(cli, tmp_path / "two", *args, "--outer", "2", "--seed", "99", "--show-constants",
+                "--state-in", half, "--state-out", end)
+    rows, rows2 = rows_of(text), rows_of(text2)
+    say("resume", args, "rows", len(rows), len(rows2))
+    assert len(rows) == 3 * int(args[args.index("--N") + 1])
+    assert rows2 == rows
+    assert open(one, "rb").read() == open(end, "rb").read()
+    assert "# \tSeed: 11" in text2.split("# \tSeed: 11", 1)[1]   # the resumed header prints the saved seed
+
+
+def test_cli_tops_file_is_sort_g_k3_tail(cli, tmp_path):
+    """--tops K writes tops.txt as run-all.sh:45 does: sort -g -k3 samples.txt | tail -n K."""
+    run(cli, tmp_path / "t", "--N", "256", "--steps", "50", "--outer", "4", "--seed", "3", "--tops", "300")
+    got = open(tmp_path / "t" / "tops.txt", "rb").read()
+    s = subprocess.run(["sort", "-g", "-k3", str(tmp_path / "t" / "samples.txt")], capture_output=True, check=True,
+                       env=dict(os.environ, LC_ALL="C")).stdout.splitlines(keepends=True)
+    assert got == b"".join(s[-300:])
+    keys = [float(ln.split(b"\t")[2]) for ln in got.splitlines() if not ln.startswith(b"#")]
+    assert keys == sorted(keys)

@@ -69,3 +69,35 @@ def test_graph_replays_match_eager(gpr, name, kw):
             ctx.run_block(steps)
             ctx.resample()
         assert same_state(ctx.read_state(), s1)
+
+
+@pytest.mark.parametrize("name,kw", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_clock_resume_continues_the_streams(gpr, name, kw):
+    """gpr_get_clock after eager blocks and graph replays; a new context given the saved
+    chains and clock (gpr_write_state + gpr_set_clock) continues bit-identically."""
+    from gpuropolis_amd import workloads as W
+    nch, steps = 1000, 40
+    with gpr.Context(**kw) as ctx:
+        set_data(ctx, name, nch)
+        ctx.init_chains()
+        assert ctx.clock() == (0, 0)
+        ctx.run_block(steps)
+        ctx.resample()
+        ctx.run_blocks_graph(1, steps)
+        assert ctx.clock() == (2 * steps, 2)
+        ops, cs, ln, _, _ = ctx.read_state()
+        ctx.run_block(steps)
+        ctx.resample()
+        ctx.run_block(steps)
+        ctx.resample()
+        want = ctx.read_state()
+        assert ctx.clock() == (4 * steps, 4)
+    with gpr.Context(**kw) as ctx:
+        set_data(ctx, name, nch)
+        ctx.write_state(ops, cs, ln)
+        ctx.set_clock(2 * steps, 2)
+        say(f"{name}: resumed at step {2 * steps}")
+        ctx.run_blocks_graph(2, steps)
+        assert same_state(ctx.read_state(), want), name
+        ctx.set_data([W.dataset_S(1)], [nch])
+        assert ctx.clock() == (0, 0)
