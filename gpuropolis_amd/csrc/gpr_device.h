@@ -572,6 +572,40 @@ __device__ __forceinline__ int subtree_start(int j, int D0, int D1) {
     return 0;
 }
 
+// ---------------------------------------------------------------- subtree generation
+// A structure move draws its new subtree one symbol per lane.  Symbol k of the
+// right-child-first prefix order takes word 0 of the Philox block at counter
+// {chain, step, GPR_P_STRUCT << 24, k} and, when it is CONST, its value from word 1 of
+// that block; word 2 of block 0 picks the node to replace.  The subtree ends at the
+// first symbol after which no operand is missing: a prefix sum of (arity - 1) over the
+// lanes finds it.
+//
+// gpr_gen_symbol per lane: the top 6 bits of a word pick X (16 of 64 buckets), CONST
+// (16), then ADD SUB MUL DIV POW EXP LOG NEG (4 each), packed one code per byte.
+__device__ __forceinline__ int gen_symbol_lane(uint32_t r) {
+    constexpr uint64_t ORDER = (uint64_t)GPR_OP_ADD | ((uint64_t)GPR_OP_SUB << 8) | ((uint64_t)GPR_OP_MUL << 16) |
+                               ((uint64_t)GPR_OP_DIV << 24) | ((uint64_t)GPR_OP_POW << 32) |
+                               ((uint64_t)GPR_OP_EXP << 40) | ((uint64_t)GPR_OP_LOG << 48) |
+                               ((uint64_t)GPR_OP_NEG << 56);
+    const int k = (int)(r >> 26);
+    const int op = (int)((ORDER >> (8 * ((k >> 2) & 7))) & 0xff);
+    return k < 16 ? GPR_OP_X : (k < 32 ? GPR_OP_CONST : op);
+}
+// gpr_gen_logp per lane: log 1/4 for the leaves, log 1/16 for the generated operators,
+// -inf for ops the table never draws.
+__device__ __forceinline__ double gen_logp_lane(int op) {
+    constexpr uint32_t LEAF = (1u << GPR_OP_X) | (1u << GPR_OP_CONST);
+    constexpr uint32_t GEN = (1u << GPR_OP_ADD) | (1u << GPR_OP_SUB) | (1u << GPR_OP_MUL) | (1u << GPR_OP_DIV) |
+                             (1u << GPR_OP_POW) | (1u << GPR_OP_EXP) | (1u << GPR_OP_LOG) | (1u << GPR_OP_NEG);
+    const uint32_t b = 1u << (op & 31);
+    return (b & LEAF) ? GPR_LOG_QUARTER : ((b & GEN) ? GPR_LOG_SIXTEENTH : -__builtin_inf());
+}
+// generation term of one token: its symbol's log probability and, for CONST, the
+// standard normal log density of its value
+__device__ __forceinline__ double gen_term_lane(int op, float c) {
+    return gen_logp_lane(op) + (op == GPR_OP_CONST ? gpr_lphi(c) : 0.0);
+}
+
 // value of a 128-token lane array at token index src (src may differ per lane)
 __device__ __forceinline__ int gather_i(int v0, int v1, int src, bool hi) {
     const int a = shfl_all(v0, src & 63);

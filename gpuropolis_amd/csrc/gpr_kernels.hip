@@ -163,36 +163,47 @@ k_mh_config_wave(GprMHArgs a) {
             if (i < 0) {
             } else if (structural) {
                 c_sp++;
-                WaveStream st;
-                st.init(a.seed, chain, step, GPR_P_STRUCT, 0);
-                const uint32_t r = st.next();
+                const uint32_t sc2 = (uint32_t)GPR_P_STRUCT << 24;
+                const uint32_t klo = (uint32_t)a.seed, khi = (uint32_t)(a.seed >> 32);
+                gpr_u4 gw = gpr_philox(chain, step, sc2, (uint32_t)l, klo, khi);
+                const uint32_t r = (uint32_t)rl_i((int)gw.z, 0);
                 const int j = __builtin_isfinite(post) ? (int)(((uint64_t)r * (uint64_t)cur.len) >> 32) : cur.len - 1;
                 int D0, D1, dm;
                 wave_depths<HI>(cur, D0, D1, dm);
                 const int s0 = subtree_start<HI>(j, D0, D1);
                 const int oldsize = j - s0 + 1;
                 const int maxnew = cap - (cur.len - oldsize);
-                // generate in right-child-first prefix order; postfix is the reverse
+                // this lane's generation terms: its tokens of the old subtree minus its
+                // symbols of the new one, summed over the wave below
+                double tq = 0.0;
+                if (l >= s0 && l <= j) tq = gen_term_lane(cur.o0, cur.c0);
+                if (HI && l + 64 >= s0 && l + 64 <= j) tq += gen_term_lane(cur.o1, cur.c1);
+                // generate symbols 64h + l in right-child-first prefix order; postfix is the reverse
                 int g0 = 0, g1 = 0;
                 float gc0 = 0.0f, gc1 = 0.0f;
                 int need = 1, m = 0;
-                double lq_new = 0.0;
-                bool fail = false;
-                while (need > 0) {
-                    if (m >= maxnew) { fail = true; break; }
-                    const int s = gpr_gen_symbol(st.next());
-                    lq_new += gpr_gen_logp(s);
-                    float cv = 0.0f;
-                    if (s == GPR_OP_CONST) {
-                        cv = gpr_normal(gpr_u01(st.next()));
-                        lq_new += gpr_lphi(cv);
+                bool closed = false;
+#pragma unroll
+                for (int h = 0; h < (HI ? 2 : 1); h++) {
+                    if (h == 1) {
+                        if (maxnew <= 64) break;
+                        gw = gpr_philox(chain, step, sc2, 64u + (uint32_t)l, klo, khi);
                     }
-                    if (!HI || m < 64) { g0 = wl_i(g0, s, m & 63); gc0 = wl_f(gc0, cv, m & 63); }
-                    else { g1 = wl_i(g1, s, m & 63); gc1 = wl_f(gc1, cv, m & 63); }
-                    m++;
-                    need += gpr_arity(s) - 1;
+                    const int sy = gen_symbol_lane(gw.x);
+                    const int nd = need + wave_scan_i(gpr_arity(sy) - 1);   // operands missing after it
+                    const uint64_t z = ballot(nd == 0);
+                    const int cnt = z ? __builtin_ctzll(z) + 1 : 64;
+                    const bool in = l < cnt;
+                    const float cv = sy == GPR_OP_CONST ? gpr_normal(gpr_u01(gw.y)) : 0.0f;
+                    if (in) tq -= gen_term_lane(sy, cv);
+                    if (h == 0) { g0 = in ? sy : 0; gc0 = in ? cv : 0.0f; }
+                    else { g1 = in ? sy : 0; gc1 = in ? cv : 0.0f; }
+                    m += cnt;
+                    if (z) { closed = true; break; }
+                    need = rl_i(nd, 63);
                 }
-                if (fail) continue;
+                // the sequential draw stops with a failure before symbol index maxnew
+                if (!closed || m > maxnew) continue;
                 const int plen = cur.len - oldsize + m;
                 // splice: [0,s0) from cur, [s0, s0+m) reversed generation, rest from cur
 #pragma unroll
@@ -217,15 +228,8 @@ k_mh_config_wave(GprMHArgs a) {
                 int P0, P1, pdm;
                 wave_depths<HI>(prop, P0, P1, pdm);
                 if (pdm > a.dmax) continue;
-                double lq_old = 0.0;
-                for (int t = s0; t <= j; t++) {
-                    const int o = (HI && t >= 64) ? rl_i(cur.o1, t & 63) : rl_i(cur.o0, t & 63);
-                    lq_old += gpr_gen_logp(o);
-                    if (o == GPR_OP_CONST) lq_old += gpr_lphi((HI && t >= 64) ? rl_f(cur.c1, t & 63) : rl_f(cur.c0, t & 63));
-                }
-                const double fwd = -(double)gpr_logint[cur.len] + lq_new;
-                const double bwd = -(double)gpr_logint[plen] + lq_old;
-                extra = bwd - fwd;
+                // (-log plen + lq_old) - (-log len + lq_new)
+                extra = ((double)gpr_logint[cur.len] - (double)gpr_logint[plen]) + wave_sum_d(tq);
             } else {
                 const bool k0 = l < cur.len && cur.o0 == GPR_OP_CONST;
                 const bool k1 = HI && l + 64 < cur.len && cur.o1 == GPR_OP_CONST;
@@ -578,9 +582,11 @@ __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
         const bool structural = !a.fixed_structure && (step % (uint32_t)a.struct_every) == 0;
         if (structural) {
             n_sprop++;
-            gpr_rstream st;
-            st.init(a.seed, (uint32_t)chain, step, GPR_P_STRUCT, 0);
-            const uint32_t r = st.next();
+            // symbol k from word 0 of the Philox block at counter k, its constant from word 1,
+            // the node from word 2 of block 0 (the wave kernel's layout, one lane per symbol)
+            const uint32_t sc2 = (uint32_t)GPR_P_STRUCT << 24;
+            const uint32_t klo = (uint32_t)a.seed, khi = (uint32_t)(a.seed >> 32);
+            const uint32_t r = gpr_philox((uint32_t)chain, step, sc2, 0u, klo, khi).z;
             const int j = __builtin_isfinite(post) ? (int)(((uint64_t)r * (uint64_t)len) >> 32) : len - 1;
             int need = 1, s0 = j;
             while (true) {
@@ -597,10 +603,12 @@ __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
             bool fail = false;
             while (need > 0) {
                 if (m >= maxnew) { fail = true; break; }
-                const int sy = gpr_gen_symbol(st.next());
-                lq_new += gpr_gen_logp(sy);
+                const gpr_u4 w = gpr_philox((uint32_t)chain, step, sc2, (uint32_t)m, klo, khi);
+                const int sy = gpr_gen_symbol(w.x);
                 float cvv = 0.0f;
-                if (sy == GPR_OP_CONST) { cvv = gpr_normal(gpr_u01(st.next())); lq_new += gpr_lphi(cvv); }
+                double term = gpr_gen_logp(sy);
+                if (sy == GPR_OP_CONST) { cvv = gpr_normal(gpr_u01(w.y)); term += gpr_lphi(cvv); }
+                lq_new += term;
                 tp.op(cap - 1 - m) = (uint8_t)sy;
                 tp.cv(cap - 1 - m) = cvv;
                 m++;
@@ -622,8 +630,9 @@ __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
             double lq_old = 0.0;
             for (int t = s0; t <= j; t++) {
                 const int o = gops[(size_t)t * N];
-                lq_old += gpr_gen_logp(o);
-                if (o == GPR_OP_CONST) lq_old += gpr_lphi(gcs[(size_t)t * N]);
+                double term = gpr_gen_logp(o);
+                if (o == GPR_OP_CONST) term += gpr_lphi(gcs[(size_t)t * N]);
+                lq_old += term;
             }
             if (dmx > a.dmax) {
                 for (int t = 0; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }

@@ -760,6 +760,44 @@ static int gen_subtree(gpo_stream *st, const uint8_t sym[64], const double *logp
     return m;
 }
 
+/* A structure move's new subtree.  Symbol k (right-child-first prefix order, as above)
+ * takes word 0 of the Philox block at counter {chain, step, P_STRUCT << 24, k} and, for
+ * CONST, its value from word 1 of that block; word 2 of block 0 is the node draw
+ * (struct_node_word).  This layout lets the device draw every symbol at once, one lane
+ * each.  The generation log probability is the sum over symbols of logp + lphi(value).
+ * Returns the size, or -1 when the subtree does not close within maxlen symbols. */
+static uint32_t struct_block_word(uint64_t seed, uint32_t chain, uint32_t step, uint32_t k, int word) {
+    uint32_t ctr[4] = { chain, step, (uint32_t)GPO_P_STRUCT << 24, k }, key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    uint32_t out[4];
+    gpo_philox4x32_10(ctr, key, out);
+    return out[word];
+}
+static uint32_t struct_node_word(uint64_t seed, uint32_t chain, uint32_t step) {
+    return struct_block_word(seed, chain, step, 0, 2);
+}
+static int gen_subtree_struct(uint64_t seed, uint32_t chain, uint32_t step, const uint8_t sym[64],
+                              const double *logp, int maxlen, uint8_t *out_ops, float *out_cs, double *logq) {
+    uint8_t g_op[256]; float g_c[256];
+    int need = 1, m = 0;
+    double lq = 0.0;
+    while (need > 0) {
+        if (m >= maxlen) return -1;
+        int s = sym[struct_block_word(seed, chain, step, (uint32_t)m, 0) >> 26];
+        float c = 0.0f;
+        double term = logp[s];
+        if (s == GPO_CONST) {
+            c = gpo_normal_icdf(gpo_u01(struct_block_word(seed, chain, step, (uint32_t)m, 1)));
+            term += lphi(c);
+        }
+        lq += term;
+        g_op[m] = (uint8_t)s; g_c[m] = c; m++;
+        need += gpo_arity(s) - 1;
+    }
+    for (int i = 0; i < m; i++) { out_ops[i] = g_op[m - 1 - i]; out_cs[i] = g_c[m - 1 - i]; }
+    *logq = lq;
+    return m;
+}
+
 int gpo_cfg_init_chain(const gpo_cfg *cfg, uint32_t chain, uint8_t *ops, float *cs) {
     uint8_t sym[64]; double logp[GPO_NUM_OPS];
     gpo_cfg_gen_table(sym, logp);
@@ -869,16 +907,14 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
         uint32_t step = (uint32_t)(step0 + (uint64_t)i);
         double cur = prior + ll;
         if (!cfg->fixed_structure && (step % (uint32_t)cfg->struct_every) == 0) {
-            gpo_stream st;
-            gpo_stream_init(&st, cfg->seed, chain, step, GPO_P_STRUCT, 0);
-            uint32_t r = gpo_stream_next(&st);
+            uint32_t r = struct_node_word(cfg->seed, chain, step);
             int j = isfinite(cur) ? (int)(((uint64_t)r * (uint64_t)len) >> 32) : len - 1;
             int s0 = gpo_subtree_start(ops, j);
             int oldsize = j - s0 + 1;
             int maxnew = cfg->cap - (len - oldsize);
             uint8_t nops[256]; float ncs[256]; double lq_new;
             stt->n_struct_prop++;
-            int m = gen_subtree(&st, sym, logp, maxnew, nops, ncs, &lq_new);
+            int m = gen_subtree_struct(cfg->seed, chain, step, sym, logp, maxnew, nops, ncs, &lq_new);
             if (m < 0) continue;
             int plen = len - oldsize + m;
             memcpy(pops, ops, (size_t)s0); memcpy(pcs, cs, sizeof(float) * (size_t)s0);
@@ -887,7 +923,11 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
             memcpy(pcs + s0 + m, cs + j + 1, sizeof(float) * (size_t)(len - j - 1));
             if (gpo_tape_depth(pops, plen) > cfg->dmax) continue;
             double lq_old = 0.0;
-            for (int t = s0; t <= j; t++) { lq_old += logp[ops[t]]; if (ops[t] == GPO_CONST) lq_old += lphi(cs[t]); }
+            for (int t = s0; t <= j; t++) {
+                double term = logp[ops[t]];
+                if (ops[t] == GPO_CONST) term += lphi(cs[t]);
+                lq_old += term;
+            }
             double pprior = gpo_tape_prior(pops, pcs, plen, lns, 0);
             double pll = gpo_tape_loglik(pops, pcs, plen, x, y, sd, n, 2);
             double fwd = -log((double)len) + lq_new;
