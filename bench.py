@@ -35,12 +35,12 @@ HBM_PEAK = 8.0e12
 # HBM bytes per MH launch (FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc, scaled by the
 # same counters on a plain copy of known size) for the default configuration, keyed by
 # (schedule, chains, cap, MH steps per block, seed); the profiles/ summaries hold the runs.
-MEASURED_TRAFFIC = {("wave", 131072, 64, 1000, 0): 261877728}
+MEASURED_TRAFFIC = {("wave", 131072, 64, 1000, 0): 267709984}
 # VALU and SALU instructions per chain-step of the same launches (rocprofv3 --pmc
 # SQ_INSTS_VALU, SQ_INSTS_SALU over chains x steps; the profiles/ summary holds the run).
 # With the live launch time they give the share of issue slots each unit is busy: one
 # wave64 instruction per SIMD every 4 cycles, 1024 SIMDs at 2.4 GHz.
-MEASURED_ISSUE = {("wave", 131072, 64, 1000, 0): (310.8, 324.8)}
+MEASURED_ISSUE = {("wave", 131072, 64, 1000, 0): (297.9, 299.6)}
 
 METRIC = "MH steps/sec + node×datapoint evals/sec (% VALU peak); ESS/sec MALA vs RW"
 
