@@ -64,6 +64,31 @@ def test_prior_only_structure_distribution():
     assert chi2 < 90, chi2          # 45 dof, p ~ 1e-4
 
 
+def test_prior_only_structure_distribution_cap4():
+    """The same law with subtrees of up to four symbols (280 structures): multi-symbol
+    draws, constants inside them and the dropped-operand cases all reach the chains."""
+    from scipy.stats import chi2 as chi2_dist
+    cap, nch, steps = 4, 20000, 300
+    cfg = O.make_cfg(cap=cap, seed=12, struct_every=1)
+    ops, cs, ln = O.cfg_init(cfg, nch)
+    empty = np.zeros(0, dtype=np.float32)
+    O.cfg_run_block(cfg, ops, cs, ln, empty, empty, empty, 0, steps)
+    S = sorted(structures(cap))
+    assert len(S) == 2 + 6 + 38 + 234
+    w = np.array([10.0 ** -len(s) for s in S])
+    p = w / w.sum()
+    index = {s: i for i, s in enumerate(S)}
+    counts = np.zeros(len(S))
+    for c in range(nch):
+        counts[index[tuple(O.OPS[o] for o in ops[:ln[c], c])]] += 1
+    for L in range(1, cap + 1):
+        q = p[[len(s) == L for s in S]].sum()
+        got = np.mean(ln == L)
+        assert abs(got - q) < 4 * np.sqrt(q * (1 - q) / nch), (L, got, q)
+    x2 = ((counts - nch * p) ** 2 / (nch * p)).sum()
+    assert x2 < chi2_dist.ppf(1 - 1e-4, len(S) - 1), x2
+
+
 def const_posterior(y, s):
     grid = np.linspace(y.mean() - 2, y.mean() + 2, 400001)
     lp = -np.log1p(grid ** 2) - ((y[None, :] - grid[:, None]) ** 2).sum(1) / (2 * s * s) if len(y) < 64 else None
