@@ -149,6 +149,9 @@ extern "C" int gpr_create(const gpr_config *cfg, gpr_ctx **out) {
     if (cfg->dmax < 1 || cfg->dmax > GPR_MAX_DEPTH) return fail(GPR_E_ARG, "dmax must be in [1, 16]");
     if (cfg->struct_every < 1) return fail(GPR_E_ARG, "struct_every must be >= 1");
     if (cfg->nsym < 2) return fail(GPR_E_ARG, "nsym must be >= 2");
+    if (!(cfg->resample_quantile >= 0.0f && cfg->resample_quantile < 1.0f))
+        return fail(GPR_E_ARG, "resample_quantile must be in [0, 1)");
+    if (!(cfg->resample_nats >= 0.0)) return fail(GPR_E_ARG, "resample_nats must be >= 0");
     if (cfg->rng == GPR_RNG_LCG && cfg->semantics != GPR_SEM_REFERENCE)
         return fail(GPR_E_ARG, "the LCG stream exists only with reference semantics");
     if (cfg->ops < 0 || cfg->ops >= GPR_NUM_OPSETS) return fail(GPR_E_ARG, "ops must be a GPR_OPS_* op set");

@@ -706,7 +706,9 @@ __global__ void __launch_bounds__(GPR_WG) k_eval_wave(GprEvalArgs a) {
     if (chain >= a.nchains) return;
     const int N = a.nchains;
     WaveProg p;
-    p.len = uni(a.len[chain]);
+    // a length outside [0, cap] is clamped: the kernel never reads or writes past the
+    // caller's cap rows or its LDS slab (a malformed tape yields a meaningless value)
+    p.len = min(max(uni(a.len[chain]), 0), a.cap);
     p.o0 = l < p.len ? a.ops[(size_t)l * N + chain] : 0;
     p.c0 = l < p.len ? a.cs[(size_t)l * N + chain] : 0.0f;
     p.o1 = (HI && l + 64 < p.len) ? a.ops[(size_t)(l + 64) * N + chain] : 0;
@@ -742,7 +744,9 @@ __global__ void __launch_bounds__(GPR_WG) k_grad_wave(GprEvalArgs a) {
     const int N = a.nchains;
     float *lv = gpr_lds_slab + (size_t)wave * a.cap * 64;
     WaveProg p;
-    p.len = uni(a.len[chain]);
+    // a length outside [0, cap] is clamped: the kernel never reads or writes past the
+    // caller's cap rows or its LDS slab (a malformed tape yields a meaningless value)
+    p.len = min(max(uni(a.len[chain]), 0), a.cap);
     p.o0 = l < p.len ? a.ops[(size_t)l * N + chain] : 0;
     p.c0 = l < p.len ? a.cs[(size_t)l * N + chain] : 0.0f;
     p.o1 = (HI && l + 64 < p.len) ? a.ops[(size_t)(l + 64) * N + chain] : 0;
@@ -787,7 +791,7 @@ __global__ void __launch_bounds__(GPR_LANE_WG) k_eval_lane(GprEvalArgs a) {
     const int chain = blockIdx.x * GPR_LANE_WG + tid;
     const bool live = chain < a.nchains;
     const int N = a.nchains;
-    const int len = live ? a.len[chain] : 0;
+    const int len = live ? min(max(a.len[chain], 0), a.cap) : 0;
     for (int t = 0; t < len; t++) {
         s_o[t * GPR_LANE_WG + tid] = a.ops[(size_t)t * N + chain];
         s_c[t * GPR_LANE_WG + tid] = a.cs[(size_t)t * N + chain];

@@ -705,7 +705,9 @@ __global__ void __launch_bounds__(R_SEL_WG) k_rs_quantile_select(GprResampleArgs
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_prefix = 0;
-        s_rank = (int)((float)d.nchains * a.quantile);
+        // the rank of the cutoff, cutoff = sorted[(int)(N * q)] (main.cu:1219); gpr_create
+        // keeps q in [0, 1), and the clamp keeps the select inside the keys regardless
+        s_rank = min(max((int)((float)d.nchains * a.quantile), 0), d.nchains - 1);
     }
     __syncthreads();
     for (int pass = 7; pass >= 0; pass--) {

@@ -180,7 +180,13 @@ int parse(int argc, char **argv, Options &o) {
                 return 1;
             break;
         case OPT_NATS: o.resample_nats = atof(optarg); break;
-        case OPT_QUANTILE: o.resample_quantile = (float)atof(optarg); break;
+        case OPT_QUANTILE:
+            o.resample_quantile = (float)atof(optarg);
+            if (!(o.resample_quantile >= 0.0f && o.resample_quantile < 1.0f)) {
+                fprintf(stderr, "--resample-quantile must be in [0, 1)\n");
+                return 1;
+            }
+            break;
         case OPT_NOLOG: o.log = 0; break;
         case OPT_STATE_IN: o.state_in = optarg; break;
         case OPT_STATE_OUT: o.state_out = optarg; break;

@@ -211,13 +211,19 @@ void *gpr_stream(gpr_ctx *ctx);
 /* Gaussian log-likelihood of nchains postfix tapes against dataset `dataset` of ctx:
  * sum_i -(f(x_i)-y_i)^2/(2 sd_i^2) - log sd_i - log(2 pi)/2, -inf if any f(x_i) or term
  * is not finite (compute_likelihood, main.cu:454-468).  Node values FP32, compensated
- * FP32 sums.  d_ops/d_cs are [cap][nchains] device arrays, d_len [nchains]. */
+ * FP32 sums.  d_ops/d_cs are [cap][nchains] device arrays, d_len [nchains].
+ * Preconditions (not checked: the tapes are device memory): each tape is well-formed
+ * postfix of GPR_OP_* codes, len in [1, cap], evaluation stack depth <= GPR_MAX_DEPTH.
+ * A length outside [0, cap] is clamped, so no access leaves the caller's arrays; a tape
+ * that breaks the other conditions yields an unspecified value. */
 int gpr_eval_loglik(gpr_ctx *ctx, int dataset, const uint8_t *d_ops, const float *d_cs,
                     const int32_t *d_len, int nchains, int cap, int schedule, double *d_ll);
 
 /* d log posterior / d cs[t] by a reverse-mode adjoint over each tape (no reference
  * counterpart: the reference has no gradients).  d_grad is [cap][nchains] FP32 (0 on
- * tokens without a constant); d_lp receives the log posterior (FP64). */
+ * tokens without a constant); d_lp receives the log posterior (FP64).  Preconditions
+ * as gpr_eval_loglik's (the torch wrapper gpuropolis_amd/autograd.py checks them on the
+ * host before the call). */
 int gpr_grad_logpost(gpr_ctx *ctx, int dataset, const uint8_t *d_ops, const float *d_cs,
                      const int32_t *d_len, int nchains, int cap, float *d_grad, double *d_lp);
 

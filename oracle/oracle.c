@@ -836,6 +836,16 @@ static int cfg_accept(const gpo_cfg *cfg, uint32_t chain, uint32_t step, double 
     return (double)gpo_logf(u) < a;
 }
 
+/* accept margin of a decision (gpo_chain_stats.min_margin): |a - log u| over the
+ * likelihood scale, where both posteriors are finite */
+static void cfg_margin(const gpo_cfg *cfg, uint32_t chain, uint32_t step, double cur, double prop, double a,
+                       double ll_cur, double ll_prop, double *m) {
+    if (!isfinite(cur) || !isfinite(prop)) return;
+    float u = gpo_u01(step_word(cfg->seed, chain, step, GPO_P_ACCEPT, 0, 0));
+    double r = fabs(a - (double)gpo_logf(u)) / (fabs(ll_cur) + fabs(ll_prop) + 1.0);
+    if (r < *m) *m = r;
+}
+
 /* The proposal metric M = L L^T of the constant tokens cfg->mrows (no reference
  * counterpart).  Derived once per chain: M rounded to FP32 from FP64 sums, and L^-1 in
  * FP64 by forward substitution.  The proposal uses d = L z and the drift M g (FP32 sums
@@ -898,6 +908,7 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
     gpo_cfg_gen_table(sym, logp);
     const double lns = log((double)cfg->nsym);
     int len = *lenp;
+    stt->min_margin = INFINITY;
     double prior = gpo_tape_prior(ops, cs, len, lns, 0);
     double ll = gpo_tape_loglik(ops, cs, len, x, y, sd, n, 2);
     double grad[256];
@@ -933,6 +944,7 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
             double fwd = -log((double)len) + lq_new;
             double bwd = -log((double)plen) + lq_old;
             double a = (pprior + pll) - cur + bwd - fwd;
+            cfg_margin(cfg, chain, step, cur, pprior + pll, a, ll, pll, &stt->min_margin);
             if (cfg_accept(cfg, chain, step, cur, pprior + pll, a)) {
                 memcpy(ops, pops, (size_t)plen); memcpy(cs, pcs, sizeof(float) * (size_t)plen);
                 len = plen; prior = pprior; ll = pll; grad_ok = 0;
@@ -1017,6 +1029,7 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
             double pprior = gpo_tape_prior(ops, pcs, len, lns, 0);
             double pll = gpo_tape_loglik(ops, pcs, len, x, y, sd, n, 2);
             double a = (pprior + pll) - cur + a_extra;
+            cfg_margin(cfg, chain, step, cur, pprior + pll, a, ll, pll, &stt->min_margin);
             if (cfg_accept(cfg, chain, step, cur, pprior + pll, a)) {
                 memcpy(cs, pcs, sizeof(float) * (size_t)len);
                 prior = pprior; ll = pll;
@@ -1033,6 +1046,14 @@ void gpo_cfg_run_block(const gpo_cfg *cfg, int nchains, uint32_t chain0, uint64_
                        uint8_t *ops, float *cs, int32_t *len,
                        const float *x, const float *y, const float *sd, int n,
                        double *prior, double *ll, long *counters, int nthreads) {
+    gpo_cfg_run_block_m(cfg, nchains, chain0, step0, steps, ops, cs, len, x, y, sd, n, prior, ll, counters, nthreads,
+                        NULL);
+}
+
+void gpo_cfg_run_block_m(const gpo_cfg *cfg, int nchains, uint32_t chain0, uint64_t step0, int steps,
+                         uint8_t *ops, float *cs, int32_t *len,
+                         const float *x, const float *y, const float *sd, int n,
+                         double *prior, double *ll, long *counters, int nthreads, double *margin) {
     long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -1050,6 +1071,7 @@ void gpo_cfg_run_block(const gpo_cfg *cfg, int nchains, uint32_t chain0, uint64_
             cs[(size_t)t * nchains + c] = t < L ? v[t] : 0.0f;
         }
         prior[c] = st.prior; ll[c] = st.ll;
+        if (margin) margin[c] = st.min_margin;
         c0 += st.n_struct_prop; c1 += st.n_struct_acc; c2 += st.n_const_prop; c3 += st.n_const_acc;
     }
     (void)nthreads;
@@ -1173,10 +1195,23 @@ static void ref_draws_from(src_t *s, int kind, int n, int arg, double *out) {
 }
 
 /* MH_simple_kernel, main.cu:679-813, one chain (AoS heap arrays) */
+/* accept margin of a reference decision: the step's random_float u drawn from a copy of
+ * the draw source (the LCG state included), so the chain's own draws are untouched */
+static void ref_margin(const src_t *s, double cur, double prop, double acc, double ll_cur, double ll_prop,
+                       double *m) {
+    if (!ref_valid(cur) || !ref_valid(prop)) return;
+    src_t c = *s;
+    uint32_t rx2 = s->lcg ? *s->rx : 0;
+    if (s->lcg) c.rx = &rx2;
+    float u = dev_random_float(&c);
+    double r = fabs(acc - (u > 0.0f ? log((double)u) : -1e300)) / (fabs(ll_cur) + fabs(ll_prop) + 1.0);
+    if (r < *m) *m = r;
+}
+
 static void ref_run_chain(const gpo_refcfg *cfg, uint32_t chain, uint64_t step0, int steps,
                           int32_t random_seed, uint8_t *P, float *C,
                           const float *x, const float *y, const float *sd, int n,
-                          double *prior_out, double *ll_out) {
+                          double *prior_out, double *ll_out, double *margin) {
     double cur_prior = gpo_ref_prior_ops(P, C, 1, cfg->ops);
     double cur_ll = gpo_ref_likelihood(P, C, 1, x, y, sd, n);
     double cur_post = cur_prior + cur_ll;
@@ -1201,6 +1236,7 @@ static void ref_run_chain(const gpo_refcfg *cfg, uint32_t chain, uint64_t step0,
             double ppost = pp + pl;
             double acc = ppost - cur_post + backward - forward;
             src_sub(&s, cfg, &rx, chain, step, GPO_P_ACCEPT, 0);
+            if (margin) ref_margin(&s, cur_post, ppost, acc, cur_ll, pl, margin);
             if ((ref_valid(ppost) && (acc > 0.0 || dev_random_float(&s) < gpo_expf((float)acc))) || !ref_valid(cur_post)) {
                 cur_post = ppost; cur_ll = pl; cur_prior = pp;
             } else {
@@ -1235,6 +1271,7 @@ static void ref_run_chain(const gpo_refcfg *cfg, uint32_t chain, uint64_t step0,
             double ppost = pp + pl;
             double acc = ppost - cur_post + backward - forward;
             src_sub(&s, cfg, &rx, chain, step, GPO_P_ACCEPT, 0);
+            if (margin) ref_margin(&s, cur_post, ppost, acc, cur_ll, pl, margin);
             if ((ref_valid(ppost) && (acc > 0.0 || dev_random_float(&s) < gpo_expf((float)acc))) || !ref_valid(cur_post)) {
                 cur_post = ppost; cur_ll = pl; cur_prior = pp;
             } else {
@@ -1249,6 +1286,14 @@ void gpo_ref_run_block(const gpo_refcfg *cfg, int nchains, uint32_t chain0, uint
                        int32_t random_seed, uint8_t *hop, float *hc,
                        const float *x, const float *y, const float *sd, int n,
                        double *prior, double *ll, int nthreads) {
+    gpo_ref_run_block_m(cfg, nchains, chain0, step0, steps, random_seed, hop, hc, x, y, sd, n, prior, ll, nthreads,
+                        NULL);
+}
+
+void gpo_ref_run_block_m(const gpo_refcfg *cfg, int nchains, uint32_t chain0, uint64_t step0, int steps,
+                         int32_t random_seed, uint8_t *hop, float *hc,
+                         const float *x, const float *y, const float *sd, int n,
+                         double *prior, double *ll, int nthreads, double *margin) {
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
@@ -1256,7 +1301,10 @@ void gpo_ref_run_block(const gpo_refcfg *cfg, int nchains, uint32_t chain0, uint
     for (int c = 0; c < nchains; c++) {
         uint8_t P[GPO_HEAP]; float C[GPO_HEAP];
         for (int i = 0; i < GPO_HEAP; i++) { P[i] = hop[(size_t)i * nchains + c]; C[i] = hc[(size_t)i * nchains + c]; }
-        ref_run_chain(cfg, chain0 + (uint32_t)c, step0, steps, random_seed, P, C, x, y, sd, n, &prior[c], &ll[c]);
+        double m = INFINITY;
+        ref_run_chain(cfg, chain0 + (uint32_t)c, step0, steps, random_seed, P, C, x, y, sd, n, &prior[c], &ll[c],
+                      margin ? &m : NULL);
+        if (margin) margin[c] = m;
         for (int i = 0; i < GPO_HEAP; i++) { hop[(size_t)i * nchains + c] = P[i]; hc[(size_t)i * nchains + c] = C[i]; }
     }
     (void)nthreads;

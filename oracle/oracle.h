@@ -128,9 +128,15 @@ typedef struct {
 void gpo_cfg_gen_table(uint8_t sym[64], double logp[GPO_NUM_OPS]);
 int gpo_cfg_init_chain(const gpo_cfg *cfg, uint32_t chain, uint8_t *ops, float *cs);
 
+/* min_margin: the smallest |a - log u| / (|LL| + |LL'| + 1) over the chain's accept
+ * decisions that compared a with its uniform (both posteriors finite), where a is the
+ * log acceptance ratio, u the step's accept uniform and LL, LL' the current and proposed
+ * log-likelihoods: how close the chain came to a decision that a last-bit difference of
+ * a likelihood could flip.  +inf when no decision compared. */
 typedef struct {
     double prior, ll;
     long n_struct_prop, n_struct_acc, n_const_prop, n_const_acc;
+    double min_margin;
 } gpo_chain_stats;
 
 /* run `steps` MH steps of one chain starting at global step `step0` */
@@ -144,6 +150,11 @@ void gpo_cfg_run_block(const gpo_cfg *cfg, int nchains, uint32_t chain0, uint64_
                        uint8_t *ops, float *cs, int32_t *len,
                        const float *x, const float *y, const float *sd, int n,
                        double *prior, double *ll, long *counters, int nthreads);
+/* the same, also writing each chain's min_margin (margin may be NULL) */
+void gpo_cfg_run_block_m(const gpo_cfg *cfg, int nchains, uint32_t chain0, uint64_t step0, int steps,
+                         uint8_t *ops, float *cs, int32_t *len,
+                         const float *x, const float *y, const float *sd, int n,
+                         double *prior, double *ll, long *counters, int nthreads, double *margin);
 
 /* MAP-relative resampling of one dataset's chains (SoA, in place) */
 int gpo_cfg_resample(int nchains, uint32_t chain0, uint32_t block, uint64_t seed, int cap,
@@ -176,6 +187,12 @@ void gpo_ref_run_block(const gpo_refcfg *cfg, int nchains, uint32_t chain0, uint
                        int32_t random_seed, uint8_t *hop, float *hc,
                        const float *x, const float *y, const float *sd, int n,
                        double *prior, double *ll, int nthreads);
+/* the same, also writing each chain's accept margin as gpo_chain_stats.min_margin, with
+ * a = the reference's acceptance log-ratio and u its random_float (margin may be NULL) */
+void gpo_ref_run_block_m(const gpo_refcfg *cfg, int nchains, uint32_t chain0, uint64_t step0, int steps,
+                         int32_t random_seed, uint8_t *hop, float *hc,
+                         const float *x, const float *y, const float *sd, int n,
+                         double *prior, double *ll, int nthreads, double *margin);
 void gpo_ref_init(uint64_t seed, int nchains, uint32_t chain0, uint8_t *hop, float *hc, int set);
 /* n draws of a reference transform (random_int/float/normal/geometric, host variants) */
 void gpo_ref_draws(int kind, int rng, uint64_t seed, uint32_t lcg_seed, int n, int arg, double *out);

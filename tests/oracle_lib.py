@@ -49,7 +49,8 @@ class GpoRefCfg(C.Structure):
 
 class GpoChainStats(C.Structure):
     _fields_ = [("prior", C.c_double), ("ll", C.c_double), ("n_struct_prop", C.c_long),
-                ("n_struct_acc", C.c_long), ("n_const_prop", C.c_long), ("n_const_acc", C.c_long)]
+                ("n_struct_acc", C.c_long), ("n_const_prop", C.c_long), ("n_const_acc", C.c_long),
+                ("min_margin", C.c_double)]
 
 
 def build():
@@ -104,6 +105,10 @@ def lib():
                                      C.POINTER(GpoChainStats)]),
         "gpo_cfg_run_block": (None, [C.POINTER(GpoCfg), C.c_int, C.c_uint32, C.c_uint64, C.c_int, _u8p, _f32p,
                                      _i32p, _f32p, _f32p, _f32p, C.c_int, _f64p, _f64p, _i64p, C.c_int]),
+        "gpo_cfg_run_block_m": (None, [C.POINTER(GpoCfg), C.c_int, C.c_uint32, C.c_uint64, C.c_int, _u8p, _f32p,
+                                       _i32p, _f32p, _f32p, _f32p, C.c_int, _f64p, _f64p, _i64p, C.c_int, _f64p]),
+        "gpo_ref_run_block_m": (None, [C.POINTER(GpoRefCfg), C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_int32,
+                                       _u8p, _f32p, _f32p, _f32p, _f32p, C.c_int, _f64p, _f64p, C.c_int, _f64p]),
         "gpo_cfg_resample": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_double,
                                        _u8p, _f32p, _i32p, _f64p, _f64p, _i32p]),
         "gpo_ref_run_block": (None, [C.POINTER(GpoRefCfg), C.c_int, C.c_uint32, C.c_uint64, C.c_int, C.c_int32,
@@ -264,24 +269,28 @@ def cfg_init(cfg, nchains, chain0=0):
     return ops, cs, ln
 
 
-def cfg_run_block(cfg, ops, cs, ln, x, y, sd, step0, steps, chain0=0, nthreads=0):
+def cfg_run_block(cfg, ops, cs, ln, x, y, sd, step0, steps, chain0=0, nthreads=0, margin=None):
+    """margin: optional float64 [nchains] receiving each chain's accept margin (the
+    smallest |a - log u| over the likelihood scale among its decisions)."""
     nch = ln.shape[0]
     prior = np.zeros(nch, dtype=np.float64)
     ll = np.zeros(nch, dtype=np.float64)
     cnt = np.zeros(4, dtype=np.int64)
     x, y, sd = (np.ascontiguousarray(v, dtype=np.float32) for v in (x, y, sd))
-    lib().gpo_cfg_run_block(C.byref(cfg), nch, chain0, step0, steps, u8(ops), f32(cs), i32(ln),
-                            f32(x), f32(y), f32(sd), len(x), f64(prior), f64(ll), ptr(cnt, _i64p), nthreads)
+    lib().gpo_cfg_run_block_m(C.byref(cfg), nch, chain0, step0, steps, u8(ops), f32(cs), i32(ln),
+                              f32(x), f32(y), f32(sd), len(x), f64(prior), f64(ll), ptr(cnt, _i64p), nthreads,
+                              None if margin is None else f64(margin))
     return prior, ll, cnt
 
 
-def ref_run_block(refcfg, hop, hc, x, y, sd, step0, steps, random_seed=0, chain0=0, nthreads=0):
+def ref_run_block(refcfg, hop, hc, x, y, sd, step0, steps, random_seed=0, chain0=0, nthreads=0, margin=None):
     nch = hop.shape[1]
     prior = np.zeros(nch, dtype=np.float64)
     ll = np.zeros(nch, dtype=np.float64)
     x, y, sd = (np.ascontiguousarray(v, dtype=np.float32) for v in (x, y, sd))
-    lib().gpo_ref_run_block(C.byref(refcfg), nch, chain0, step0, steps, random_seed, u8(hop), f32(hc),
-                            f32(x), f32(y), f32(sd), len(x), f64(prior), f64(ll), nthreads)
+    lib().gpo_ref_run_block_m(C.byref(refcfg), nch, chain0, step0, steps, random_seed, u8(hop), f32(hc),
+                              f32(x), f32(y), f32(sd), len(x), f64(prior), f64(ll), nthreads,
+                              None if margin is None else f64(margin))
     return prior, ll
 
 

@@ -286,6 +286,9 @@ def test_cli_rejects_bad_arguments(cli, tmp_path):
     assert r.returncode == 1
     r = run_cli(cli, "--thin", "0")
     assert r.returncode == 1
+    for q in ("1", "1.5", "-0.1", "nan"):
+        r = run_cli(cli, "--resample-quantile", q)
+        assert r.returncode == 1 and "--resample-quantile must be in [0, 1)" in r.stderr, q
 
 
 def test_cli_reports_data_errors_before_touching_a_device(cli, tmp_path):

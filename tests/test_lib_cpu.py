@@ -77,3 +77,31 @@ def test_op_sets_need_reference_semantics(gpr):
         gpr.Context(semantics=gpr.SEM_CONFIG, ops=2)
     with pytest.raises(gpr.GprError, match="op set"):
         gpr.Context(semantics=gpr.SEM_REFERENCE, ops=7)
+
+
+def test_autograd_wrapper_checks_tapes_on_the_host():
+    """The gradient wrapper rejects tapes the adjoint kernel cannot take: lengths outside
+    [1, cap], unknown op codes, malformed postfix, stacks deeper than 16."""
+    import torch
+    from gpuropolis_amd import lib as G
+    from gpuropolis_amd.autograd import check_tapes
+    X, C, ADD, EXP = G.OP["X"], G.OP["CONST"], G.OP["ADD"], G.OP["EXP"]
+
+    def tapes(progs, cap=8):
+        ops = torch.zeros((cap, len(progs)), dtype=torch.uint8)
+        for i, p in enumerate(progs):
+            ops[:len(p), i] = torch.tensor(p, dtype=torch.uint8)
+        return ops, torch.tensor([len(p) for p in progs], dtype=torch.int32)
+
+    check_tapes(*tapes([[X], [X, C, ADD, EXP], [C, X, X, ADD, ADD]]))
+    for bad in ([X, ADD], [X, X], [ADD], [X, C, ADD, ADD], [X, 200]):
+        with pytest.raises(G.GprError):
+            check_tapes(*tapes([[X], bad]))
+    ops, ln = tapes([[X, C, ADD]])
+    for L in (0, 9):
+        with pytest.raises(G.GprError):
+            check_tapes(ops, torch.tensor([L], dtype=torch.int32))
+    deep = [X] * 17 + [ADD] * 16
+    check_tapes(*tapes([[X] * 16 + [ADD] * 15], cap=40))
+    with pytest.raises(G.GprError):
+        check_tapes(*tapes([deep], cap=40))

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from nist_models import MODELS
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 ARITH = ["ONE", "CONST", "INV", "A", "B", "PLUS", "MINUS", "RMINUS", "CPLUS", "TIMES", "CTIMES", "DIV", "RDIV",
@@ -131,32 +132,6 @@ def test_tape_depth_and_subtree():
 def nist(name):
     a = np.loadtxt(os.path.join(GOLDEN, "nist", name, "data.txt"))
     return a[:, 0], a[:, 1], a[:, 2]
-
-
-def C(v):
-    return ("CONST", float(np.float32(v)))
-
-
-MODELS = {
-    "BoxBOD": (lambda b, x: b[0] * (1 - np.exp(-b[1] * x)),
-               lambda b: [C(b[0]), "ONE", C(-b[1]), "X", "MUL", "EXP", "SUB", "MUL"]),
-    "MGH09": (lambda b, x: b[0] * (x ** 2 + x * b[1]) / (x ** 2 + x * b[2] + b[3]),
-              lambda b: [C(b[0]), "X", "X", "MUL", "X", C(b[1]), "MUL", "ADD", "MUL", "X", "X", "MUL", "X", C(b[2]),
-                         "MUL", "ADD", C(b[3]), "ADD", "DIV"]),
-    "MGH10": (lambda b, x: b[0] * np.exp(b[1] / (x + b[2])),
-              lambda b: [C(b[0]), C(b[1]), "X", C(b[2]), "ADD", "DIV", "EXP", "MUL"]),
-    "Eckerle4": (lambda b, x: (b[0] / b[1]) * np.exp(-0.5 * ((x - b[2]) / b[1]) ** 2),
-                 lambda b: [C(b[0]), C(b[1]), "DIV", C(-0.5), "X", C(b[2]), "SUB", C(b[1]), "DIV", "X", C(b[2]), "SUB",
-                            C(b[1]), "DIV", "MUL", "MUL", "EXP", "MUL"]),
-    "Bennett5": (lambda b, x: b[0] * (b[1] + x) ** (-1 / b[2]),
-                 lambda b: [C(b[0]), C(b[1]), "X", "ADD", C(-1 / b[2]), "POW", "MUL"]),
-    "Rat42": (lambda b, x: b[0] / (1 + np.exp(b[1] - b[2] * x)),
-              lambda b: [C(b[0]), "ONE", C(b[1]), C(b[2]), "X", "MUL", "SUB", "EXP", "ADD", "DIV"]),
-    "Rat43": (lambda b, x: b[0] / ((1 + np.exp(b[1] - b[2] * x)) ** (1 / b[3])),
-              lambda b: [C(b[0]), "ONE", C(b[1]), C(b[2]), "X", "MUL", "SUB", "EXP", "ADD", C(1 / b[3]), "POW", "DIV"]),
-    "Thurber": (lambda b, x: (b[0] + b[1] * x + b[2] * x ** 2 + b[3] * x ** 3) / (1 + b[4] * x + b[5] * x ** 2 + b[6] * x ** 3),
-                None),
-}
 
 
 @pytest.mark.parametrize("name", sorted(MODELS))
