@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -952,6 +953,69 @@ extern "C" int gpr_grad_logpost(gpr_ctx *c, int dataset, const uint8_t *d_ops, c
     a.lnsym = log((double)c->cfg.nsym);
     HIP_TRY(gpr_launch_grad_wave(a, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
+}
+
+// ---------------------------------------------------------------- no-allocation evaluation
+// gpr_eval_loglik_noalloc_async keeps the dataset header and per-point weights, which
+// gpr_set_data allocates and fills for gpr_eval_loglik, in this static device array:
+// [0, 256) the GprDataset header, then GPR_NOALLOC_MAX_POINTS floats of weights.  Each
+// device holds its own copy at its own address; host code uses only the address
+// hipGetSymbolAddress returns for the current device, never the array's name.
+#define GPR_NOALLOC_HDR_BYTES 256
+static __device__ __align__(256) unsigned char scratch[GPR_NOALLOC_HDR_BYTES + sizeof(float) * GPR_NOALLOC_MAX_POINTS];
+#define GPR_NOALLOC_MAX_DEVICES 64
+static void *g_scratch_ptr[GPR_NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// scratch's device address on `device` (current), looked up once per device
+static int scratch_ptr(int device, unsigned char **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GPR_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = (unsigned char *)g_scratch_ptr[device];
+    return GPR_OK;
+}
+
+extern "C" int gpr_eval_loglik_noalloc_async(int device, const float *d_x, const float *d_y, const float *d_sd, int n,
+                                             const uint8_t *d_ops, const float *d_cs, const int32_t *d_len,
+                                             int nchains, int cap, int schedule, double *d_ll, void *stream) {
+    if (!d_x || !d_y || !d_sd || !d_ops || !d_cs || !d_len || !d_ll) return fail(GPR_E_ARG, "null device array");
+    if (n < 1 || n > GPR_NOALLOC_MAX_POINTS)
+        return fail(GPR_E_ARG, "n = " + std::to_string(n) + " points: the no-allocation entry takes 1 to " +
+                                   std::to_string(GPR_NOALLOC_MAX_POINTS) + " (its static scratch size)");
+    if (nchains < 1) return fail(GPR_E_ARG, "nchains must be >= 1");
+    if (cap < 1 || cap > GPR_MAX_CAP) return fail(GPR_E_ARG, "cap must be in [1, 128]");
+    if (schedule != GPR_SCHED_AUTO && schedule != GPR_SCHED_LANE && schedule != GPR_SCHED_WAVE)
+        return fail(GPR_E_ARG, "bad schedule");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev || device >= GPR_NOALLOC_MAX_DEVICES)
+        return fail(GPR_E_ARG, "invalid GPU device " + std::to_string(device));
+    HIP_TRY(hipSetDevice(device));
+    unsigned char *sp = nullptr;
+    if (int r = scratch_ptr(device, &sp)) return r;
+    GprDataset *hdr = (GprDataset *)sp;
+    float *w = (float *)(sp + GPR_NOALLOC_HDR_BYTES);
+    const hipStream_t s = (hipStream_t)stream;
+    const int kc = kc_for(n);
+    HIP_TRY(gpr_launch_dataset_prep(d_sd, n, nchains, cap, kc, hdr, w, s));
+    GprEvalArgs a{};
+    a.ops = d_ops; a.cs = d_cs; a.len = d_len; a.nchains = nchains; a.cap = cap;
+    a.dx = d_x; a.dy = d_y; a.dw = w;
+    a.d.n = n;
+    a.d.kc = kc;
+    a.d.cap = cap;
+    a.d.nchains = nchains;
+    a.dd = hdr;
+    a.ll = d_ll;
+    if (schedule == GPR_SCHED_LANE) HIP_TRY(gpr_launch_eval_lane(a, s));
+    else HIP_TRY(gpr_launch_eval_wave(a, s));
     return GPR_OK;
 }
 

@@ -95,6 +95,9 @@ struct GprEvalArgs {
     int cap;
     const float *dx, *dy, *dw;
     GprDataset d;
+    // when set, the kernels read the dataset header from this device copy instead of d
+    // (the no-allocation entry computes its normaliser on the device)
+    const GprDataset *dd;
     double *ll;
     float *grad;            // gradient output (grad kernel)
     double lnsym;
@@ -131,6 +134,10 @@ hipError_t gpr_launch_mh_config_lane(const GprMHArgs &a, int cap_max, hipStream_
 hipError_t gpr_launch_mh_ref_lane(const GprMHArgs &a, hipStream_t s);
 hipError_t gpr_launch_mh_ref_wave(const GprMHArgs &a, int kc, int grid, hipStream_t s);
 hipError_t gpr_launch_eval_wave(const GprEvalArgs &a, hipStream_t s);
+// header (n, kc, lconst, wconst, ...) and per-point weights of one dataset from its sd
+// column, on the device: the values gpr_set_data computes on the host
+hipError_t gpr_launch_dataset_prep(const float *sd, int n, int nchains, int cap, int kc, GprDataset *hdr, float *w,
+                                   hipStream_t s);
 hipError_t gpr_launch_eval_lane(const GprEvalArgs &a, hipStream_t s);
 hipError_t gpr_launch_grad_wave(const GprEvalArgs &a, hipStream_t s);
 hipError_t gpr_launch_init_config(const GprMHArgs &a, hipStream_t s);

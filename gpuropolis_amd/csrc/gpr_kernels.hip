@@ -705,6 +705,7 @@ __global__ void __launch_bounds__(GPR_WG) k_eval_wave(GprEvalArgs a) {
     const int chain = uni((int)blockIdx.x * GPR_WAVES_PER_WG + wave);
     if (chain >= a.nchains) return;
     const int N = a.nchains;
+    const GprDataset d = a.dd ? *a.dd : a.d;
     WaveProg p;
     // a length outside [0, cap] is clamped: the kernel never reads or writes past the
     // caller's cap rows or its LDS slab (a malformed tape yields a meaningless value)
@@ -713,10 +714,10 @@ __global__ void __launch_bounds__(GPR_WG) k_eval_wave(GprEvalArgs a) {
     p.c0 = l < p.len ? a.cs[(size_t)l * N + chain] : 0.0f;
     p.o1 = (HI && l + 64 < p.len) ? a.ops[(size_t)(l + 64) * N + chain] : 0;
     p.c1 = (HI && l + 64 < p.len) ? a.cs[(size_t)(l + 64) * N + chain] : 0.0f;
-    const int nchunks = (a.d.n + 64 * KC - 1) / (64 * KC);
+    const int nchunks = (d.n + 64 * KC - 1) / (64 * KC);
     Chunk<KC> c0;
-    c0.load(a.dx + a.d.off, a.dy + a.d.off, a.dw + a.d.off, a.d.n, 0);
-    const double ll = wave_loglik<KC, HI>(p, a.d, a.dx + a.d.off, a.dy + a.d.off, a.dw + a.d.off, c0, nchunks);
+    c0.load(a.dx + d.off, a.dy + d.off, a.dw + d.off, d.n, 0);
+    const double ll = wave_loglik<KC, HI>(p, d, a.dx + d.off, a.dy + d.off, a.dw + d.off, c0, nchunks);
     if (l == 0) a.ll[chain] = ll;
 }
 
@@ -791,19 +792,20 @@ __global__ void __launch_bounds__(GPR_LANE_WG) k_eval_lane(GprEvalArgs a) {
     const int chain = blockIdx.x * GPR_LANE_WG + tid;
     const bool live = chain < a.nchains;
     const int N = a.nchains;
+    const GprDataset d = a.dd ? *a.dd : a.d;
     const int len = live ? min(max(a.len[chain], 0), a.cap) : 0;
     for (int t = 0; t < len; t++) {
         s_o[t * GPR_LANE_WG + tid] = a.ops[(size_t)t * N + chain];
         s_c[t * GPR_LANE_WG + tid] = a.cs[(size_t)t * N + chain];
     }
     float acc = 0.0f, comp = 0.0f;
-    for (int base = 0; base < a.d.n; base += GPR_LANE_DTILE) {
-        const int cnt = min(GPR_LANE_DTILE, a.d.n - base);
+    for (int base = 0; base < d.n; base += GPR_LANE_DTILE) {
+        const int cnt = min(GPR_LANE_DTILE, d.n - base);
         __syncthreads();
         for (int i = tid; i < cnt; i += GPR_LANE_WG) {
-            s_x[i] = a.dx[a.d.off + base + i];
-            s_y[i] = a.dy[a.d.off + base + i];
-            s_w[i] = a.dw[a.d.off + base + i];
+            s_x[i] = a.dx[d.off + base + i];
+            s_y[i] = a.dy[d.off + base + i];
+            s_w[i] = a.dw[d.off + base + i];
         }
         __syncthreads();
         for (int i = 0; i < cnt; i++) {
@@ -833,7 +835,7 @@ __global__ void __launch_bounds__(GPR_LANE_WG) k_eval_lane(GprEvalArgs a) {
     }
     if (live) {
         const double sum = (double)acc - (double)comp;
-        double ll = a.d.lconst - (a.d.wconst > 0.0 ? a.d.wconst * sum : sum);
+        double ll = d.lconst - (d.wconst > 0.0 ? d.wconst * sum : sum);
         if (!__builtin_isfinite(sum)) ll = -GPR_INF;
         a.ll[chain] = ll;
     }
@@ -848,6 +850,63 @@ hipError_t gpr_launch_eval_lane(const GprEvalArgs &a, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_eval_lane, dim3(grid), dim3(GPR_LANE_WG), sh, s, a);
+    return hipGetLastError();
+}
+
+// Dataset header and weights from the sd column, as gpr_set_data forms them on the host:
+// lconst = -sum_i 0.5 (logf(2 pi) + 2 logf(sd_i)) in FP64, and either w_i = 1 with
+// wconst = 1 / (2 sd^2) when every sd is equal, or w_i = 1 / (2 sd_i^2) rounded to float
+// and wconst = 0.  logf is the float rounding of the FP64 log (both correctly rounded);
+// the FP64 sum runs per thread, then as a fixed tree.  A sd that is not finite and
+// positive makes lconst NaN, so every likelihood of the call is NaN.
+#define GPR_PREP_WG 256
+__global__ void __launch_bounds__(GPR_PREP_WG) k_dataset_prep(const float *sd, int n, int nchains, int cap, int kc,
+                                                              GprDataset *hdr, float *w) {
+    __shared__ double s_part[GPR_PREP_WG];
+    __shared__ int s_diff, s_bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) { s_diff = 0; s_bad = 0; }
+    __syncthreads();
+    const float s0 = sd[0];
+    double lc = 0.0;
+    bool diff = false, bad = false;
+    for (int i = tid; i < n; i += GPR_PREP_WG) {
+        const float s = sd[i];
+        bad = bad || !(s > 0.0f) || !__builtin_isfinite(s);
+        diff = diff || s != s0;
+        const float ls = (float)log((double)s);
+        lc -= 0.5 * ((double)0x1.d67f1ep+0f + 2.0 * (double)ls);
+    }
+    s_part[tid] = lc;
+    if (diff) atomicOr(&s_diff, 1);
+    if (bad) atomicOr(&s_bad, 1);
+    __syncthreads();
+    for (int o = GPR_PREP_WG / 2; o > 0; o >>= 1) {
+        if (tid < o) s_part[tid] += s_part[tid + o];
+        __syncthreads();
+    }
+    const bool same = s_diff == 0;
+    for (int i = tid; i < n; i += GPR_PREP_WG) {
+        const double s = sd[i];
+        w[i] = same ? 1.0f : (float)(1.0 / (2.0 * s * s));
+    }
+    if (tid == 0) {
+        GprDataset g{};
+        g.n = n;
+        g.off = 0;
+        g.chain0 = 0;
+        g.nchains = nchains;
+        g.cap = cap;
+        g.kc = kc;
+        g.lconst = s_bad ? __builtin_nan("") : s_part[0];
+        g.wconst = same ? 1.0 / (2.0 * (double)s0 * (double)s0) : 0.0;
+        *hdr = g;
+    }
+}
+
+hipError_t gpr_launch_dataset_prep(const float *sd, int n, int nchains, int cap, int kc, GprDataset *hdr, float *w,
+                                   hipStream_t s) {
+    hipLaunchKernelGGL(k_dataset_prep, dim3(1), dim3(GPR_PREP_WG), 0, s, sd, n, nchains, cap, kc, hdr, w);
     return hipGetLastError();
 }
 

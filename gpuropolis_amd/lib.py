@@ -34,7 +34,8 @@ EXPORTS = ["gpr_config_default", "gpr_create", "gpr_destroy", "gpr_last_error", 
            "gpr_resample", "gpr_run_blocks_graph", "gpr_get_stats", "gpr_reset_stats", "gpr_sync", "gpr_stream",
            "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws",
            "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning", "gpr_set_metric", "gpr_ref_op_set",
-           "gpr_get_clock", "gpr_set_clock"]
+           "gpr_get_clock", "gpr_set_clock", "gpr_eval_loglik_noalloc_async"]
+NOALLOC_MAX_POINTS = 65536
 
 
 class Config(C.Structure):
@@ -106,6 +107,8 @@ def load():
         "gpr_ref_op_set": (C.c_int, [C.c_int, vp, C.c_int, C.POINTER(C.c_char_p)]),
         "gpr_test_ref_decode": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, vp]),
         "gpr_test_draws": (C.c_int, [vp, vp, C.c_int, C.c_uint64, vp]),
+        "gpr_eval_loglik_noalloc_async": (C.c_int, [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int,
+                                                    C.c_int, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -273,6 +276,14 @@ class Context:
 
     def grad_logpost_dev(self, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp):
         check(load().gpr_grad_logpost(self.h, dataset, d_ops, d_cs, d_len, nchains, cap, d_grad, d_lp))
+
+
+def eval_loglik_noalloc_async(device, d_x, d_y, d_sd, n, d_ops, d_cs, d_len, nchains, cap, schedule, d_ll,
+                              stream=None):
+    """gpr_eval_loglik_noalloc_async: every array is a device address (int); the work is
+    enqueued on `stream` (a hipStream_t as int, None for the null stream)."""
+    check(load().gpr_eval_loglik_noalloc_async(device, d_x, d_y, d_sd, n, d_ops, d_cs, d_len, nchains, cap,
+                                               schedule, d_ll, stream))
 
 
 def ref_op_set(ops):

@@ -227,6 +227,24 @@ int gpr_eval_loglik(gpr_ctx *ctx, int dataset, const uint8_t *d_ops, const float
 int gpr_grad_logpost(gpr_ctx *ctx, int dataset, const uint8_t *d_ops, const float *d_cs,
                      const int32_t *d_len, int nchains, int cap, float *d_grad, double *d_lp);
 
+/* gpr_eval_loglik without a context and without allocating anything.  No counterpart in
+ * the reference (which allocates everything up front, main.cu:1123-1147); it is for
+ * callers that cannot allocate device memory: code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc.  The caller passes the dataset (d_x, d_y, d_sd:
+ * n device floats; sd finite and > 0, else every result is NaN), the tapes as for
+ * gpr_eval_loglik, and the output.  The dataset's derived values (the normaliser and
+ * the per-point weights that gpr_set_data computes on the host) are formed on the device
+ * in a static device array of each GPU, sized for n <= GPR_NOALLOC_MAX_POINTS (about
+ * 256 KiB, held until the process ends); a larger n is rejected with GPR_E_ARG.  Every
+ * call on one GPU uses that GPU's array: calls that overlap on one GPU (different
+ * streams or host threads) race, and must be ordered by the caller.  The work is
+ * enqueued on `stream` (a hipStream_t; NULL is the null stream) and the call returns
+ * without waiting.  No hipMalloc, hipFree or other allocating call is made. */
+#define GPR_NOALLOC_MAX_POINTS 65536
+int gpr_eval_loglik_noalloc_async(int device, const float *d_x, const float *d_y, const float *d_sd, int n,
+                                  const uint8_t *d_ops, const float *d_cs, const int32_t *d_len, int nchains,
+                                  int cap, int schedule, double *d_ll, void *stream);
+
 /* Heap-31 -> postfix decode of the reachable tree (program_fold, main.cu:371-393):
  * ops/cs receive up to 63 tokens; returns the token count or a negative error. */
 int gpr_ref_decode(const uint8_t *hop, const float *hc, size_t stride, uint8_t *ops, float *cs);

@@ -105,3 +105,17 @@ def test_autograd_wrapper_checks_tapes_on_the_host():
     check_tapes(*tapes([[X] * 16 + [ADD] * 15], cap=40))
     with pytest.raises(G.GprError):
         check_tapes(*tapes([deep], cap=40))
+
+
+def test_noalloc_eval_rejects_sizes_before_any_device_call(gpr):
+    """The no-allocation evaluator checks its sizes first: n beyond its static scratch, or
+    below 1, and bad caps are errors with a message, with no device touched."""
+    L = gpr.load()
+    fake = 0x1000     # never dereferenced: the argument checks come first
+    for n, cap in ((gpr.NOALLOC_MAX_POINTS + 1, 32), (0, 32), (100, 0), (100, 129)):
+        rc = L.gpr_eval_loglik_noalloc_async(0, fake, fake, fake, n, fake, fake, fake, 4, cap, gpr.SCHED_WAVE, fake,
+                                             None)
+        assert rc == -1, (n, cap)
+    L.gpr_eval_loglik_noalloc_async(0, fake, fake, fake, gpr.NOALLOC_MAX_POINTS + 1, fake, fake, fake, 4, 32,
+                                    gpr.SCHED_WAVE, fake, None)
+    assert b"static scratch" in L.gpr_last_error()
