@@ -1,27 +1,36 @@
 """Benchmark: the MH sampler on one MI355X, prints ONE JSON line.
 
-Workload (BASELINE configuration "plain method, large"): one dataset of n=250 fp32
-points, x ~ U[-5,5], y = a seeded random target expression (<= 12 nodes, constants
-N(0, 2^2)) + noise of sd 1% of the y range; 131072 chains, postfix programs of at
-most 64 tokens; a bench step is one MCMC block of 1000 MH steps on every chain
-followed by MAP-relative resampling (chains > 100 nats below the MAP copy a good
-chain), all on the device.  The config runs 20 blocks: --steps 20 by default.
+Headline workload (BASELINE configuration "full form"): 1,048,576 chains over 256
+datasets of uneven size (n log-uniform in [16, 1024], x ~ U[-10, 10], y = a seeded random
+target expression of at most 24 nodes + noise of sd 1% of the y range), postfix programs
+of at most 128 tokens, MALA constant moves (eps 0.01) and subtree structure moves in the
+batched persistent launch, MAP-relative resampling (chains > 100 nats below their
+dataset's MAP copy a good chain), all on the device.  A bench step is one MCMC block of
+1000 MH steps on every chain followed by the resampling; --steps blocks are timed after
+--warmup blocks.
 
-value       whole-job MH steps per second (chains x steps / wall time of the K blocks)
-roofline    the MH kernel: algorithmic FP32 ops per launch (n * (non-leaf tokens + 4)
-            + constants, summed over every evaluated program, counted by the kernel)
-            divided by the kernel's average launch time (HIP events on its stream),
-            against the VALU lane-op issue peak (256 CU x 64 lanes x 2.4 GHz); the FP32
-            datasheet peak (157.3 TFLOP/s, packed FMA) is reported beside it.  For the
-            default configuration it also carries the HBM bytes per launch and the share
-            of VALU and SALU issue slots in use, from rocprofv3 counters of the same
-            launch (MEASURED_TRAFFIC, MEASURED_ISSUE) over the live launch time.
-cpu_baseline the oracle's serial restatement of the same sampler (OpenMP over chains)
-            on this host's cores, on a bounded sample of the same workload.
+value        whole-job MH steps per second (chains x steps / wall time of the K blocks)
+roofline     the MH kernel of the headline (one persistent launch per block): algorithmic
+             FP32 ops per launch (per evaluated program n * (non-leaf tokens + 4) +
+             constants for the forward evaluation, plus n * (non-leaf tokens + 2 +
+             constants) for each reverse sweep of a MALA gradient, counted by the kernel)
+             over the kernel's average launch time (HIP events on the stream it runs on),
+             against the VALU lane-op issue peak (256 CU x 64 lanes x 2.4 GHz); the FP32
+             datasheet peak (157.3 TFLOP/s, packed FMA) is reported beside it.  HBM bytes
+             per launch and the VALU/SALU issue shares come from rocprofv3 counters of the
+             same launch (MEASURED_TRAFFIC, MEASURED_ISSUE), over the live launch time.
+cpu_baseline the oracle's serial restatement of the same sampler (OpenMP over chains) on
+             this host, on a bounded sample of the same workload; beside it the 1-thread
+             rate and the reference semantics (the reference's heap-31 sampler on the 18
+             NIST datasets) at both thread counts.
+Sub-records: the plain large workload on both evaluator schedules, the batched workload,
+the NIST batch in both semantics, and MALA against random walk on the differentiation
+workload (ESS per second).
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -33,58 +42,132 @@ FP32_DATASHEET_PEAK = 157.3e12            # FLOP/s, packed FMA
 HBM_PEAK = 8.0e12
 
 # HBM bytes per MH launch (FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc, scaled by the
-# same counters on a plain copy of known size) for the default configuration, keyed by
-# (schedule, chains, cap, MH steps per block, seed); the profiles/ summaries hold the runs.
-MEASURED_TRAFFIC = {("wave", 131072, 64, 1000, 0): 267709984}
+# same counters on a plain copy of known size), keyed by (workload, chains, MH steps per
+# block, seed); the profiles/ summaries hold the runs.  Valid for the kernel build they
+# were measured on.
+MEASURED_TRAFFIC = {}
 # VALU and SALU instructions per chain-step of the same launches (rocprofv3 --pmc
-# SQ_INSTS_VALU, SQ_INSTS_SALU over chains x steps; the profiles/ summary holds the run).
-# With the live launch time they give the share of issue slots each unit is busy: one
-# wave64 instruction per SIMD every 4 cycles, 1024 SIMDs at 2.4 GHz.
-MEASURED_ISSUE = {("wave", 131072, 64, 1000, 0): (297.9, 299.6)}
+# SQ_INSTS_VALU, SQ_INSTS_SALU over chains x steps).  With the live launch time they give
+# the share of issue slots each unit is busy: one wave64 instruction per SIMD every 4
+# cycles, 1024 SIMDs at 2.4 GHz.
+MEASURED_ISSUE = {}
 
 METRIC = "MH steps/sec + node×datapoint evals/sec (% VALU peak); ESS/sec MALA vs RW"
 
 
-def cpu_baseline(ds, cap, seed, sample_chains, sample_steps):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def host_threads():
+    """Threads the CPU baseline uses: the host's share of cores for this process (the
+    OpenMP thread count the environment grants, else the CPUs this process may run on)."""
+    env = os.environ.get("GPR_CPU_THREADS") or os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(seed=0, scale=1.0):
+    """The oracle (oracle/oracle.c, serial C with OpenMP over chains) on bounded samples:
+    the full form's config sampler (MALA + subtree moves, cap 128) on 16 of its 256
+    datasets (every 16th), and the reference sampler (heap-31 programs, the reference's
+    moves) on the 18 NIST datasets, each with all granted threads and with one."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np
     import oracle_lib as O
+    from gpuropolis_amd import workloads as W
     O.build()
-    threads = int(os.environ.get("GPR_CPU_THREADS", min(16, os.cpu_count() or 1)))
-    cfg = O.make_cfg(cap=cap, seed=seed)
-    ops, cs, ln = O.cfg_init(cfg, sample_chains)
-    x, y, sd = ds
-    t0 = time.perf_counter()
-    O.cfg_run_block(cfg, ops, cs, ln, x, y, sd, 0, sample_steps, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": sample_chains * sample_steps / dt, "unit": "MH steps/s", "cores": threads, "kind": "port",
-            "sample": f"{sample_chains} chains x {sample_steps} MH steps of the same workload "
-                      f"(oracle/oracle.c, OpenMP, {dt:.1f} s)"}
+    threads = host_threads()
+    dsets, _ = W.datasets_F(seed)
+    sel = list(range(0, len(dsets), 16))
+    names = sorted(os.listdir(NIST_DIR))
+    nist = [nist_data(nm) for nm in names]
+
+    def run_config(nthreads, nch, steps):
+        cfg = O.make_cfg(cap=128, seed=seed, proposal=1, eps=0.01)
+        t0, tot = time.perf_counter(), 0
+        for d in sel:
+            x, y, sd = dsets[d]
+            ops, cs, ln = O.cfg_init(cfg, nch, chain0=d * 4096)
+            O.cfg_run_block(cfg, ops, cs, ln, x, y, sd, 0, steps, chain0=d * 4096, nthreads=nthreads)
+            tot += nch * steps
+        dt = time.perf_counter() - t0
+        return tot / dt, dt, f"{len(sel)} of the full form's datasets (every 16th) x {nch} chains x {steps} MH steps " \
+                             f"from the initial programs"
+
+    def run_reference(nthreads, nch, steps):
+        cfg = O.GpoRefCfg(0, seed, 0)
+        t0, tot = time.perf_counter(), 0
+        for d, (x, y, sd) in enumerate(nist):
+            hop, hc = O.ref_init(seed, nch, chain0=d * 8192)
+            O.ref_run_block(cfg, hop, hc, x, y, sd, 0, steps, chain0=d * 8192, nthreads=nthreads)
+            tot += nch * steps
+        dt = time.perf_counter() - t0
+        return tot / dt, dt, f"the {len(nist)} NIST datasets x {nch} chains x {steps} MH steps from the initial programs"
+
+    def sz(n):
+        return max(8, int(n * scale))
+
+    runs = {}
+    for key, fn, nch_all, nch_one in (("config", run_config, 512, 64), ("reference", run_reference, 512, 40)):
+        v, dt, what = fn(threads, sz(nch_all), 1000)
+        v1, dt1, what1 = fn(1, sz(nch_one), 1000)
+        runs[key] = {"all_threads": {"value": v, "threads": threads, "seconds": dt, "sample": what},
+                     "one_thread": {"value": v1, "threads": 1, "seconds": dt1, "sample": what1}}
+    c = runs["config"]["all_threads"]
+    return {"value": c["value"], "unit": "MH steps/s", "cores": threads, "kind": "port",
+            "sample": c["sample"] + f" (config sampler of the headline, {c['seconds']:.1f} s)",
+            "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                     "affinity_cpus": len(os.sched_getaffinity(0)), "threads_granted": threads},
+            "runs": runs}
 
 
 def step_split(args):
-    """Cost per chain-step of the two step types: programs from two warm-up blocks of the
-    large workload, then one block with structure moves off (constant steps only) and
-    one with a structure move at every step, each on the default launch shape."""
+    """Cost per chain-step of the step types, on the default launch shape: programs from
+    warm-up blocks of a workload, then one block with structure moves off (constant
+    steps only) and one with a structure move at every step.  --step-split large: the
+    plain large workload (random walk); full: the full form (MALA), also with random-walk
+    constant steps (sigma 0.01) on the same programs."""
     from gpuropolis_amd import lib as G
     from gpuropolis_amd import workloads as W
-    ds, _ = W.dataset_L(args.seed)
-    ctx = G.Context(cap=args.cap, seed=args.seed, schedule=G.SCHED_WAVE)
-    ctx.set_data([ds], [args.chains])
+    if args.step_split == "large":
+        dsets, chains, base = [W.dataset_L(args.seed)[0]], [args.chains], dict(cap=args.cap)
+        warm = 2
+    else:
+        dsets = W.datasets_F(args.seed)[0]
+        chains = [4096] * len(dsets)
+        base = dict(cap=128, proposal=G.PROP_MALA, eps=0.01)
+        warm = 1
+    ctx = G.Context(seed=args.seed, schedule=G.SCHED_WAVE, **base)
+    ctx.set_data(dsets, chains)
     ctx.init_chains()
-    for _ in range(2):
+    for _ in range(warm):
         ctx.run_block(args.mh_steps)
         ctx.resample()
     ops, cs, ln, _, _ = ctx.read_state()
     ctx.close()
-    out = {"mean_tokens": float(ln.mean())}
-    for name, kw in (("constant_steps", dict(fixed_structure=1)), ("structure_steps", dict(struct_every=1))):
-        ctx = G.Context(cap=args.cap, seed=args.seed, schedule=G.SCHED_WAVE, **kw)
-        ctx.set_data([ds], [args.chains])
+    out = {"workload": args.step_split, "mean_tokens": float(ln.mean())}
+    variants = [("constant_steps", dict(fixed_structure=1)), ("structure_steps", dict(struct_every=1))]
+    if args.step_split == "full":
+        variants.append(("constant_steps_random_walk", dict(fixed_structure=1, proposal=G.PROP_RW, sigma=0.01)))
+    for name, kw in variants:
+        ctx = G.Context(seed=args.seed, schedule=G.SCHED_WAVE, **{**base, **kw})
+        ctx.set_data(dsets, chains)
         ctx.write_state(ops, cs, ln)
         ctx.run_block(args.mh_steps)
-        ms = ctx.stats().last_block_ms
-        out[name] = {"kernel_ms": ms, "ns_per_chain_step": ms * 1e6 / (args.chains * args.mh_steps)}
+        st = ctx.stats()
+        ms = st.last_block_ms
+        out[name] = {"kernel_ms": ms, "ns_per_chain_step": ms * 1e6 / (sum(chains) * args.mh_steps),
+                     "acceptance": (st.const_accepted + st.struct_accepted) / max(1, st.const_proposed +
+                                                                                  st.struct_proposed),
+                     "alg_flops": st.alg_flops, "node_evals": st.node_evals}
         ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -265,18 +348,20 @@ def batched(G, seed=0, blocks=3, mh_steps=1000):
 NIST_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "nist")
 
 
+def nist_data(name):
+    import numpy as np
+    a = np.loadtxt(os.path.join(NIST_DIR, name, "data.txt"), comments="#", ndmin=2)
+    return tuple(np.ascontiguousarray(a[:, k], dtype=np.float32) for k in range(3))
+
+
 def nist_batch(G, seed=0, blocks=2, mh_steps=1000, chains=8192, tune_steps=50):
     """The NIST real-data batch: all 18 NIST StRD datasets (n 6 to 236, the reference's
     data-sources/NIST data.txt files) in one persistent launch, 8192 chains each (the
     reference's run-all.sh N), blocks of MH steps plus resampling.  Both semantics: the
     reference's heap-31 sampler with its median re-draw rule, and the postfix sampler
     (cap 64) with MAP-100 resampling."""
-    import numpy as np
     names = sorted(os.listdir(NIST_DIR))
-    dsets = []
-    for nm in names:
-        a = np.loadtxt(os.path.join(NIST_DIR, nm, "data.txt"), comments="#", ndmin=2)
-        dsets.append(tuple(np.ascontiguousarray(a[:, k], dtype=np.float32) for k in range(3)))
+    dsets = [nist_data(nm) for nm in names]
     out = {"workload": f"NIST batch: {len(names)} NIST StRD datasets (n {min(len(d[0]) for d in dsets)} to "
                        f"{max(len(d[0]) for d in dsets)}), {chains} chains each, one persistent launch per "
                        f"size class, blocks of {mh_steps} MH steps + resampling", "blocks": blocks}
@@ -303,14 +388,17 @@ def nist_batch(G, seed=0, blocks=2, mh_steps=1000, chains=8192, tune_steps=50):
         ctx.close()
         out[name] = {"mh_steps_per_s": st.steps / wall, "kernel_ms_per_block": kms / blocks,
                      "node_evals_per_s": st.node_evals / wall,
-                     "alg_tflops": st.alg_flops / (kms / 1e3) / 1e12}
+                     "alg_tflops": st.alg_flops / (kms / 1e3) / 1e12,
+                     "frac_valu_peak": st.alg_flops / (kms / 1e3) / VALU_LANE_OP_PEAK}
     return out
 
 
-def full_form(G, seed=0, blocks=1, mh_steps=200, chains=1 << 20):
+def full_form(G, seed=0, blocks=5, warmup=1, mh_steps=1000, chains=1 << 20, tune_steps=20):
     """The full form: 1,048,576 chains over 256 datasets (n log-uniform in [16, 1024], x on
     [-10, 10]), programs of at most 128 tokens, MALA constant moves and subtree structure
-    moves in the batched persistent launch, MAP-100 resampling."""
+    moves in the batched persistent launch, MAP-100 resampling.  Returns the stats, the
+    wall time of the timed blocks, the summed device time of their MH launches (HIP events
+    on the context stream, which the single persistent launch runs on), and the tuning."""
     from gpuropolis_amd import workloads as W
     dsets, _ = W.datasets_F(seed)
     nd = len(dsets)
@@ -318,8 +406,12 @@ def full_form(G, seed=0, blocks=1, mh_steps=200, chains=1 << 20):
     ctx = G.Context(cap=128, seed=seed, schedule=G.SCHED_WAVE, proposal=G.PROP_MALA, eps=0.01)
     ctx.set_data(dsets, per)
     ctx.init_chains()
-    ctx.run_block(mh_steps)
-    ctx.resample()
+    for _ in range(warmup):
+        ctx.run_block(mh_steps)
+        ctx.resample()
+    table = None
+    if tune_steps > 0:
+        table, _ = ctx.autotune(tune_steps)
     ctx.reset_stats()
     kms = 0.0
     ctx.sync()
@@ -332,39 +424,75 @@ def full_form(G, seed=0, blocks=1, mh_steps=200, chains=1 << 20):
     wall = time.perf_counter() - t0
     st = ctx.stats()
     ctx.close()
-    return {"workload": f"full form: {sum(per)} chains, {nd} datasets (n log-uniform [16, 1024]), cap 128, "
-                        f"MALA constants (eps 0.01) + subtree structure moves, MAP-100 resampling, "
-                        f"blocks of {mh_steps} MH steps", "blocks": blocks,
-            "mh_steps_per_s": st.steps / wall, "kernel_ms_per_block": kms / blocks,
-            "node_evals_per_s": st.node_evals / wall, "alg_tflops": st.alg_flops / (kms / 1e3) / 1e12,
-            "acceptance": {"structure": st.struct_accepted / max(1, st.struct_proposed),
-                           "constants": st.const_accepted / max(1, st.const_proposed)}}
+    return st, wall, kms, table, sum(per), nd
+
+
+def roofline(kname, flops_per_launch, launch_s, key):
+    achieved = flops_per_launch / launch_s
+    r = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OP_PEAK / 1e12, "unit": "TFLOP/s",
+         "frac": achieved / VALU_LANE_OP_PEAK, "traffic": None, "kernel": kname,
+         "flops_per_launch": flops_per_launch, "avg_launch_ms": launch_s * 1e3,
+         "peak_fp32_datasheet": FP32_DATASHEET_PEAK / 1e12, "frac_fp32_datasheet": achieved / FP32_DATASHEET_PEAK}
+    issue = MEASURED_ISSUE.get(key)
+    if issue is not None:
+        slots = 1024 * 2.4e9 / 4 * launch_s
+        chain_steps = key[1] * key[2]
+        r["issue_frac"] = {"valu": issue[0] * chain_steps / slots, "salu": issue[1] * chain_steps / slots}
+    traffic = MEASURED_TRAFFIC.get(key)
+    if traffic is not None:
+        r["traffic"] = traffic
+        r["hbm_gbs"] = traffic / launch_s / 1e9
+        r["hbm_frac"] = traffic / launch_s / HBM_PEAK
+    return r
+
+
+def large(G, args):
+    """The plain large workload (131072 chains, cap 64, one dataset n = 250, random-walk
+    constants): 20 blocks on the wave schedule, autotuned, with its roofline; then the lane
+    schedule on a few blocks."""
+    from gpuropolis_amd import workloads as W
+    ds, _ = W.dataset_L(args.seed)
+    st, wall, kernel_ms, table = time_blocks(G, ds, args, G.SCHED_WAVE, args.large_blocks, 1,
+                                             tune=args.tune_steps > 0)
+    launch_s = kernel_ms / 1e3 / args.large_blocks
+    out = {"workload": f"plain method, large: {args.chains} chains, postfix cap 64, one dataset n=250, "
+                       f"blocks of {args.mh_steps} MH steps + MAP-100 resample",
+           "blocks": args.large_blocks, "mh_steps_per_s": st.steps / wall, "node_evals_per_s": st.node_evals / wall,
+           "launch_shape": table,
+           "roofline": roofline("k_mh_config_wave<1,false,false,false>", st.alg_flops / args.large_blocks, launch_s,
+                                ("large", args.chains, args.mh_steps, args.seed))}
+    if args.lane_blocks > 0:
+        o_st, o_wall, o_ms, _ = time_blocks(G, ds, args, G.SCHED_LANE, args.lane_blocks, 1)
+        out["schedules"] = {
+            "wave": {"mh_steps_per_s": st.steps / wall, "kernel_ms_per_block": launch_s * 1e3},
+            "lane": {"mh_steps_per_s": o_st.steps / o_wall, "blocks": args.lane_blocks,
+                     "kernel_ms_per_block": o_ms / args.lane_blocks,
+                     "alg_tflops": o_st.alg_flops / (o_ms / 1e3) / 1e12}}
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed MCMC blocks")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed MCMC blocks of the full form")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed blocks of the full form before tuning")
     ap.add_argument("--mh-steps", type=int, default=1000, help="MH steps per block")
-    ap.add_argument("--chains", type=int, default=131072)
-    ap.add_argument("--cap", type=int, default=64)
-    ap.add_argument("--schedule", choices=["wave", "lane"], default="wave")
+    ap.add_argument("--chains", type=int, default=131072, help="chains of the plain large workload")
+    ap.add_argument("--cap", type=int, default=64, help="token cap of the plain large workload")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-chains", type=int, default=65536)
-    ap.add_argument("--cpu-steps", type=int, default=1000)
-    ap.add_argument("--other-schedule-blocks", type=int, default=2,
-                    help="blocks timed on the other evaluator schedule for comparison (0: skip)")
+    ap.add_argument("--cpu-scale", type=float, default=1.0, help="scale of the CPU baseline's chain samples")
+    ap.add_argument("--large-blocks", type=int, default=20, help="timed blocks of the plain large workload (0: skip)")
+    ap.add_argument("--lane-blocks", type=int, default=1, help="blocks of the large workload on the lane schedule")
     ap.add_argument("--batched-blocks", type=int, default=3, help="timed blocks of the batched workload (0: skip)")
     ap.add_argument("--no-ess", dest="ess", action="store_false", help="skip the MALA vs random-walk ESS run")
     ap.add_argument("--nist-blocks", type=int, default=2, help="timed blocks of the NIST batch (0: skip)")
-    ap.add_argument("--full-blocks", type=int, default=1, help="timed blocks of the full form (0: skip)")
-    ap.add_argument("--tune-steps", type=int, default=50,
+    ap.add_argument("--tune-steps", type=int, default=20,
                     help="autotune the wave launch shapes on this many MH steps after warm-up (0: defaults)")
-    ap.add_argument("--step-split", action="store_true",
-                    help="only time one block of constant steps and one of structure steps on the large "
-                         "workload's programs (after two warm-up blocks) and print their cost per chain-step")
+    ap.add_argument("--only-headline", action="store_true", help="time the full form only (profiling runs)")
+    ap.add_argument("--step-split", choices=["large", "full"], default=None,
+                    help="only time one block of constant steps and one of structure steps on a workload's "
+                         "programs (after warm-up blocks) and print their cost per chain-step")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -379,24 +507,14 @@ def main():
               f"(datasets and chains shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     from gpuropolis_amd import lib as G
-    from gpuropolis_amd import workloads as W
 
-    ds, _target = W.dataset_L(args.seed)
-    sched = G.SCHED_WAVE if args.schedule == "wave" else G.SCHED_LANE
-    st, wall, kernel_ms, table = time_blocks(G, ds, args, sched, args.steps, args.warmup,
-                                             tune=args.tune_steps > 0 and args.schedule == "wave")
-    mh_steps = st.steps
-    value = mh_steps / wall
-    evals_per_s = st.node_evals / wall
-    flops_per_launch = st.alg_flops / args.steps
-    avg_launch_s = kernel_ms / 1e3 / args.steps
-    achieved = flops_per_launch / avg_launch_s
-    kname = "k_mh_config_wave" if args.schedule == "wave" else "k_mh_config_lane"
+    st, wall, kms, table, nchains, nd = full_form(G, args.seed, args.steps, args.warmup, args.mh_steps,
+                                                  tune_steps=args.tune_steps)
+    launch_s = kms / 1e3 / args.steps
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": st.steps / wall,
         "unit": "MH steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -407,50 +525,29 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "plain method, large: 131072 chains, postfix cap 64, one dataset n=250, "
-                               "blocks of 1000 MH steps + MAP-100 resample",
-                   "chains": args.chains, "cap": args.cap, "n": 250, "mh_steps_per_block": args.mh_steps,
-                   "schedule": args.schedule},
-        "node_evals_per_s": evals_per_s,
+        "config": {"workload": f"full form: {nchains} chains, {nd} datasets (n log-uniform [16, 1024], x on "
+                               f"[-10, 10], targets <= 24 nodes + 1% noise), postfix cap 128, MALA constants "
+                               f"(eps 0.01) + subtree structure moves, one persistent launch per block, blocks of "
+                               f"{args.mh_steps} MH steps + MAP-100 resample",
+                   "chains": nchains, "datasets": nd, "cap": 128, "mh_steps_per_block": args.mh_steps},
+        "node_evals_per_s": st.node_evals / wall,
         "launch_shape": table,
         "acceptance": {"structure": st.struct_accepted / max(1, st.struct_proposed),
                        "constants": st.const_accepted / max(1, st.const_proposed)},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OP_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / VALU_LANE_OP_PEAK, "traffic": None,
-                     "kernel": kname, "flops_per_launch": flops_per_launch,
-                     "avg_launch_ms": avg_launch_s * 1e3,
-                     "peak_fp32_datasheet": FP32_DATASHEET_PEAK / 1e12,
-                     "frac_fp32_datasheet": achieved / FP32_DATASHEET_PEAK},
+        "roofline": roofline("k_mh_config_wave<1,true,true,false>", st.alg_flops / args.steps, launch_s,
+                             ("full", nchains, args.mh_steps, args.seed)),
     }
-    issue = MEASURED_ISSUE.get((args.schedule, args.chains, args.cap, args.mh_steps, args.seed))
-    if issue is not None:
-        slots = 1024 * 2.4e9 / 4 * avg_launch_s
-        chain_steps = args.chains * args.mh_steps
-        out["roofline"]["issue_frac"] = {"valu": issue[0] * chain_steps / slots, "salu": issue[1] * chain_steps / slots}
-    traffic = MEASURED_TRAFFIC.get((args.schedule, args.chains, args.cap, args.mh_steps, args.seed))
-    if traffic is not None:
-        out["roofline"]["traffic"] = traffic
-        out["roofline"]["hbm_gbs"] = traffic / avg_launch_s / 1e9
-        out["roofline"]["hbm_frac"] = traffic / avg_launch_s / HBM_PEAK
-    if args.other_schedule_blocks > 0:
-        other = "lane" if args.schedule == "wave" else "wave"
-        o_st, o_wall, o_ms, _ = time_blocks(G, ds, args, G.SCHED_LANE if other == "lane" else G.SCHED_WAVE,
-                                         args.other_schedule_blocks, 1)
-        out["schedules"] = {
-            args.schedule: {"mh_steps_per_s": value, "kernel_ms_per_block": avg_launch_s * 1e3},
-            other: {"mh_steps_per_s": o_st.steps / o_wall, "blocks": args.other_schedule_blocks,
-                    "kernel_ms_per_block": o_ms / args.other_schedule_blocks,
-                    "alg_tflops": o_st.alg_flops / (o_ms / 1e3) / 1e12}}
-    if args.batched_blocks > 0:
-        out["batched"] = batched(G, args.seed, args.batched_blocks, args.mh_steps)
-    if args.nist_blocks > 0:
-        out["nist"] = nist_batch(G, args.seed, args.nist_blocks, args.mh_steps)
-    if args.full_blocks > 0:
-        out["full"] = full_form(G, args.seed, args.full_blocks)
-    if args.ess:
-        out["ess"] = ess_rational(G, args.seed)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(ds, args.cap, args.seed, args.cpu_chains, args.cpu_steps)
+    if not args.only_headline:
+        if args.large_blocks > 0:
+            out["large"] = large(G, args)
+        if args.batched_blocks > 0:
+            out["batched"] = batched(G, args.seed, args.batched_blocks, args.mh_steps)
+        if args.nist_blocks > 0:
+            out["nist"] = nist_batch(G, args.seed, args.nist_blocks, args.mh_steps)
+        if args.ess:
+            out["ess"] = ess_rational(G, args.seed)
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = cpu_baseline(args.seed, args.cpu_scale)
     print(json.dumps(out), flush=True)
 
 

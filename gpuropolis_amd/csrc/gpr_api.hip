@@ -217,6 +217,14 @@ static int dev_alloc(T **p, size_t count) {
 }
 #define ALLOC(p, n) do { int _r = dev_alloc(&(p), (n)); if (_r) return _r; } while (0)
 
+// Under MALA every evaluation runs at one point per lane whatever the class's kc, so the
+// size classes share one kernel: they are merged into the n > 128 class, one persistent
+// launch (per register form) over every dataset, largest first.
+static bool mala_merged(const gpr_ctx *c) {
+    return c->cfg.semantics == GPR_SEM_CONFIG && c->cfg.proposal == GPR_PROP_MALA;
+}
+static int bucket_class(const gpr_ctx *c, int n) { return mala_merged(c) ? 2 : cls_of(n); }
+
 // Tiles of wpg chains of one dataset, grouped by size class (one persistent launch per
 // class); larger datasets first within a class (longest-processing-time order).
 static void build_tiles(gpr_ctx *c) {
@@ -228,7 +236,7 @@ static void build_tiles(gpr_ctx *c) {
             KernelBucket b{k, hi != 0, t.kc, t.wpg, t.wgpc, (int)c->htiles.size(), 0, 0};
             std::vector<int> order;
             for (int d = 0; d < c->nd; d++)
-                if (cls_of(c->hds[d].n) == k && ((c->hds[d].cap > 64) == (hi != 0))) order.push_back(d);
+                if (bucket_class(c, c->hds[d].n) == k && ((c->hds[d].cap > 64) == (hi != 0))) order.push_back(d);
             std::stable_sort(order.begin(), order.end(), [&](int a, int bb) { return c->hds[a].n > c->hds[bb].n; });
             for (int d : order)
                 for (int s0 = 0; s0 < c->hds[d].nchains; s0 += t.wpg)
@@ -238,15 +246,15 @@ static void build_tiles(gpr_ctx *c) {
         }
 }
 
-// MALA token slabs of the two-register kernels (cap > 64) live in global memory, one per
-// resident wave (workgroups x waves per workgroup) of each two-register launch; the
+// MALA token slabs of programs longer than GPR_MALA_LDS_ROWS tokens live in global
+// memory, one per resident wave (workgroups x waves per workgroup) of each launch; the
 // launches of a block run concurrently, so each has its own region.  Sized whenever the
 // tiles change, before any launch, so never inside a graph capture.
 static int ensure_gslab(gpr_ctx *c) {
-    if (c->cfg.semantics != GPR_SEM_CONFIG || c->cfg.proposal != GPR_PROP_MALA || c->rows <= 64) return GPR_OK;
+    if (c->cfg.semantics != GPR_SEM_CONFIG || c->cfg.proposal != GPR_PROP_MALA || c->rows <= GPR_MALA_LDS_ROWS)
+        return GPR_OK;
     size_t need = 0;
     for (KernelBucket &kb : c->buckets) {
-        if (!kb.hi) continue;
         if (kb.gslab_off != need) c->gexec_stale = true;
         kb.gslab_off = need;
         need += (size_t)std::max(1, std::min(kb.ntiles, c->cus * kb.wgpc)) * (size_t)kb.wpg * (size_t)c->rows * 64;
@@ -503,7 +511,7 @@ static hipError_t launch_bucket(gpr_ctx *c, const GprMHArgs &a, int b, hipStream
     ab.tile_counter = c->d_tile_counter + b;
     ab.dmax_cap = c->rows;
     ab.wpg = kb.wpg;
-    ab.mala_gslab = kb.hi && c->d_gslab ? c->d_gslab + kb.gslab_off : nullptr;
+    ab.mala_gslab = c->d_gslab ? c->d_gslab + kb.gslab_off : nullptr;
     const int grid = std::max(1, std::min(kb.ntiles, c->cus * kb.wgpc));
     if (c->cfg.semantics == GPR_SEM_REFERENCE) return gpr_launch_mh_ref_wave(ab, kb.kc, grid, s);
     return gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, c->cfg.proposal == GPR_PROP_MALA, grid, s);
@@ -827,6 +835,7 @@ extern "C" int gpr_autotune(gpr_ctx *c, int steps, gpr_tune *out, int max_out, i
             for (int wpg : {1, 2, 4})
                 for (int wpc : {8, 16, 32}) {
                     if (c->cfg.semantics == GPR_SEM_REFERENCE && hi) continue;
+                    if (mala && kc != 1) continue;      // MALA evaluates one point per lane
                     const ClassTune cand{kc, wpg, wpc / wpg, 0.0};
                     c->ct[k][hi] = cand;
                     build_tiles(c);

@@ -918,31 +918,6 @@ __device__ double wave_loglik_grad(const WaveProg &p, const GradPlan &g, const G
     return d.lconst - (folded ? d.wconst * s : s);
 }
 
-// Log-likelihood through the forward sweep alone, bit-identical to wave_loglik<KC, HI>
-// (same per-lane point order and Kahan updates).  lv: as for wave_loglik_grad.
-template <bool HI>
-__device__ double wave_loglik_slab(const WaveProg &p, const GradPlan &g, const GprDataset &d, const float *dx,
-                                   const float *dy, const float *dw, float *lv) {
-    const int l = lane_id();
-    float *lvl = lv + l;
-    float acc = 0.0f, comp = 0.0f;
-    const int nch = (d.n + 63) / 64;
-    for (int ch = 0; ch < nch; ch++) {
-        const int i = ch * 64 + l;
-        const bool valid = i < d.n;
-        const float x = valid ? dx[i] : 0.0f, y = valid ? dy[i] : 0.0f, w = valid ? dw[i] : 0.0f;
-        const float r = grad_forward<HI>(p, g, x, lvl) - y;
-        const float term = valid ? (r * r) * w : 0.0f;
-        const float yv = term - comp;
-        const float tv = acc + yv;
-        comp = (tv - acc) - yv;
-        acc = tv;
-    }
-    const double s = wave_sum_d((double)acc - (double)comp);
-    if (!__builtin_isfinite(s)) return -GPR_INF;
-    return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
-}
-
 // FP64 sum of v over the lanes where m holds, in token order (lanes of the low half,
 // then the high half), as a serial loop over tokens would add them.
 template <bool HI>

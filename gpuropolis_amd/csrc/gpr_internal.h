@@ -40,11 +40,16 @@ struct GprTile {
 };
 
 #define GPR_WAVES_PER_WG 4
+// MALA: token-value rows each wave of the MH kernel keeps in LDS; the adjoint sweeps of
+// longer programs use the wave's slab in global memory (GprMHArgs::mala_gslab)
+#define GPR_MALA_LDS_ROWS 20
 #define GPR_WG 256
 
 // Counters accumulated by the MH kernels (device, u64).
 // FLOPS: algorithmic FLOPs n*(N_op + 4) + N_const per evaluated program (N_op = non-leaf
-// tokens; the +4 is residual, square, weight, accumulate)
+// tokens; the +4 is residual, square, weight, accumulate), plus n*(N_op + 2 + N_const) per
+// MALA gradient (one adjoint per operator token, the residual's derivative, and each
+// constant's per-point gradient summed)
 enum { GPR_CNT_SPROP = 0, GPR_CNT_SACC, GPR_CNT_CPROP, GPR_CNT_CACC, GPR_CNT_EVALS, GPR_CNT_STEPS, GPR_CNT_FLOPS, GPR_NCNT };
 
 struct GprMHArgs {
@@ -79,8 +84,8 @@ struct GprMHArgs {
     int rng;
     int dmax_cap;           // largest per-dataset cap (LDS sizing: lane kernel tapes, MALA slabs)
     int wpg;                // waves (= chains of a tile) per workgroup of the wave kernels: 1, 2 or 4
-    // MALA token slabs of tapes over 64 tokens (the two-register kernels): dmax_cap rows
-    // per resident wave (workgroup * wpg + wave) in global memory; null without such tapes
+    // MALA token slabs of tapes over GPR_MALA_LDS_ROWS tokens: dmax_cap rows per resident
+    // wave (workgroup * wpg + wave) in global memory; null when no cap exceeds those rows
     float *mala_gslab;
     // reference semantics: op set (GPR_OPS_*) and log(NOPS + 1) of its prior
     int opset;

@@ -20,11 +20,13 @@
 // ---------------------------------------------------------------- config MH, wave schedule
 // MALA: constant moves are Langevin proposals c' = c + h g(c) + eps z (h = eps^2/2, g the
 // gradient of the log posterior from the adjoint sweep) with the Hastings ratio of the
-// two Gaussian proposal densities.  Under MALA every evaluation uses the stack-free
-// sweep at one point per lane (KC = 1): each wave keeps its token values in a slab of
-// dmax_cap * 64 floats, in LDS for tapes of up to 64 tokens and in global memory for the
-// two-register kernels (an LDS slab of 128 rows, 32 KB, would leave room for five waves
-// per CU), and the current structure's plan in registers.
+// two Gaussian proposal densities.  Under MALA the interpreter evaluates every
+// likelihood at one point per lane (KC = 1), with early rejection; the gradients of the
+// proposals that survive it come from the stack-free adjoint sweeps, whose token values
+// sit in the wave's LDS region for programs of at most GPR_MALA_LDS_ROWS tokens and in
+// the wave's slab in global memory for longer ones (LDS rows for every token of a
+// 128-token cap, 32 KB per wave, would leave room for five waves per CU).  The current
+// structure's sweep plan stays in registers.
 extern __shared__ float gpr_lds_slab[];
 
 // gpr_logf of the integers 0 .. GPR_MAX_CAP (program lengths), filled once per device by
@@ -47,14 +49,19 @@ k_mh_config_wave(GprMHArgs a) {
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
     const int N = a.nchains;
-    // dynamic LDS: the value stacks of the wpg waves, or under MALA their token-value slabs
-    float *ls = MALA ? nullptr : gpr_lds_slab + (size_t)wave * ((GPR_MAX_DEPTH + 2) * 64 * KC);
+    // dynamic LDS: the value stacks of the wpg waves.  Under MALA each wave's LDS region
+    // (MROWS rows of 64 floats) also holds the token values of the adjoint sweeps of
+    // programs of at most GPR_MALA_LDS_ROWS tokens (the interpreter's stack and the sweeps
+    // are never in use at once); longer programs sweep through the wave's slab in global
+    // memory, lvg.
+    constexpr int SROWS = GPR_MAX_DEPTH + 2;
+    constexpr int MROWS = SROWS > GPR_MALA_LDS_ROWS ? SROWS : GPR_MALA_LDS_ROWS;
+    float *ls = gpr_lds_slab + (size_t)wave * ((MALA ? MROWS : SROWS) * 64 * KC);
     // after the wpg value stacks: each wave's priors of the (up to 16) steps of its draw batch
     double *bpri = MALA ? nullptr
                         : (double *)(gpr_lds_slab + (size_t)a.wpg * ((GPR_MAX_DEPTH + 2) * 64 * KC)) + wave * 16;
-    float *lv = !MALA ? nullptr
-                : HI  ? a.mala_gslab + ((size_t)blockIdx.x * a.wpg + wave) * (size_t)a.dmax_cap * 64
-                      : gpr_lds_slab + (size_t)wave * a.dmax_cap * 64;
+    float *lvg = (MALA && a.mala_gslab) ? a.mala_gslab + ((size_t)blockIdx.x * a.wpg + wave) * (size_t)a.dmax_cap * 64
+                                        : nullptr;
     const uint32_t step0 = a.clock ? (uint32_t)uni((int)a.clock[0]) : a.step0;
     // launch counters, lane k holds counter k (folded in once per tile)
     unsigned long long cnt = 0;
@@ -124,10 +131,11 @@ k_mh_config_wave(GprMHArgs a) {
         bool grad_ok = false;
         AcceptDraws adraw;
         // token classes of the current program (MALA: with the sweeps' plan)
+        // (MALA: the interpreter evaluates every likelihood; the sweeps' plan serves the
+        // gradients, which only proposals that survive early rejection need)
         GradPlan gplan{};
-        OpMasks cmask{};
+        OpMasks cmask = op_masks<HI>(cur);
         if (MALA) gplan = grad_plan<HI>(cur);
-        else cmask = op_masks<HI>(cur);
         // per-program values that change only with the structure: (operator tokens + 4)
         // and constants of the current program for the work counters, and whether its
         // random-walk moves can be batched (see BATCH below)
@@ -155,8 +163,11 @@ k_mh_config_wave(GprMHArgs a) {
             const double post = prior + ll;
             WaveProg prop = cur;
             double extra = 0.0;
-            bool have_pll = false;
+            bool have_pll = false;   // MALA constant step: the LL (pll_m, over mala_pts points) is known
             double pll_m = 0.0;
+            int mala_pts = 0;
+            bool have_prior = false; // MALA constant step: the prior (pre_prior) is known
+            double pre_prior = 0.0;
             bool bprior = false;   // the prior comes from the batch: bpri[bu]
             int bu = 0;
             const bool structural = struct_slot && !a.fixed_structure;
@@ -305,7 +316,9 @@ k_mh_config_wave(GprMHArgs a) {
                     if (!grad_ok) {
                         double h0, h1;
                         bool gok;
-                        (void)wave_loglik_grad<HI>(cur, gplan, d, dx, dy, dw, lv, h0, h1, gok);
+                        (void)wave_loglik_grad<HI>(cur, gplan, d, dx, dy, dw,
+                                                   cur.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1, gok);
+                        c_fl += (unsigned long long)(cur_opq - 2 + cur_nc) * (unsigned long long)d.n;
                         cg0 = gok ? (float)h0 : 0.0f;
                         cg1 = gok ? (float)h1 : 0.0f;
                         if (mk) cmg = metric_Mv(mM, mk, gather_f(cg0, cg1, mrow, HI));
@@ -331,33 +344,58 @@ k_mh_config_wave(GprMHArgs a) {
                             if (slot1 >= 0) prop.c1 = v1;
                         }
                     }
-                    double h0, h1;
-                    bool pok;
-                    pll_m = wave_loglik_grad<HI>(prop, gplan, d, dx, dy, dw, lv, h0, h1, pok);
+                    // The reverse proposal's log density bq is <= 0, so a = pprior + pll + bq -
+                    // fq - post is at most pprior + pll - fq - post: a proposal whose LL lies
+                    // below post - pprior + fq - 18 gives a < -18 whatever its gradient and is
+                    // rejected without a draw (cfg_accept).  The interpreter evaluates the LL
+                    // with that bound and stops such proposals early; only the others (and
+                    // every proposal from an invalid current state, which always moves) get
+                    // the adjoint sweeps.  Their LL from the sweeps equals the interpreter's
+                    // bit for bit (same operations, same per-lane order).
+                    pre_prior = wave_prior<HI>(prop, a.lnsym);
+                    have_prior = true;
+                    double rb = -GPR_INF;
+                    if (__builtin_isfinite(post) && __builtin_isfinite(pre_prior) && __builtin_isfinite(fq))
+                        rb = post - pre_prior + fq - 18.0 -
+                             1e-12 * (__builtin_fabs(post) + __builtin_fabs(pre_prior) + __builtin_fabs(fq));
+                    if (HI && prop.len <= 64)
+                        pll_m = wave_loglik<KC, false, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &cmask, rb,
+                                                             &mala_pts);
+                    else
+                        pll_m = wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &cmask, rb,
+                                                          &mala_pts);
                     have_pll = true;
-                    pg0 = pok ? (float)h0 : 0.0f;
-                    pg1 = pok ? (float)h1 : 0.0f;
-                    const float b0 = gpr_axpy(h, pg0, prop.c0), b1 = gpr_axpy(h, pg1, prop.c1);
-                    const double r0 = (double)cur.c0 - (double)b0, r1 = (double)cur.c1 - (double)b1;
-                    double bq = wave_ordered_sum_d<HI>(-r0 * r0 / inv2e2, i0, -r1 * r1 / inv2e2, i1);
-                    if (mk) {
-                        pmg = metric_Mv(mM, mk, gather_f(pg0, pg1, mrow, HI));
-                        const float mb = gpr_axpy(h, pmg, cn);
-                        bq += metric_logq(mLi, mk, l < mk ? (double)cr - (double)mb : 0.0, inv2e2);
+                    if (!__builtin_isfinite(pll_m) && __builtin_isfinite(post)) {
+                        extra = -GPR_INF;      // rejected: invalid, or below the bound
+                    } else {
+                        double h0, h1;
+                        bool pok;
+                        (void)wave_loglik_grad<HI>(prop, gplan, d, dx, dy, dw,
+                                                   prop.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1, pok);
+                        c_fl += (unsigned long long)(cur_opq - 2 + cur_nc) * (unsigned long long)d.n;
+                        pg0 = pok ? (float)h0 : 0.0f;
+                        pg1 = pok ? (float)h1 : 0.0f;
+                        const float b0 = gpr_axpy(h, pg0, prop.c0), b1 = gpr_axpy(h, pg1, prop.c1);
+                        const double r0 = (double)cur.c0 - (double)b0, r1 = (double)cur.c1 - (double)b1;
+                        double bq = wave_ordered_sum_d<HI>(-r0 * r0 / inv2e2, i0, -r1 * r1 / inv2e2, i1);
+                        if (mk) {
+                            pmg = metric_Mv(mM, mk, gather_f(pg0, pg1, mrow, HI));
+                            const float mb = gpr_axpy(h, pmg, cn);
+                            bq += metric_logq(mLi, mk, l < mk ? (double)cr - (double)mb : 0.0, inv2e2);
+                        }
+                        extra = pok ? bq - fq : -GPR_INF;
                     }
-                    extra = pok ? bq - fq : -GPR_INF;
                 }
                 }
             }
-            const double pprior = bprior ? bpri[bu] : wave_prior<HI>(prop, a.lnsym);
+            const double pprior = bprior ? bpri[bu] : (have_prior ? pre_prior : wave_prior<HI>(prop, a.lnsym));
             // token classes: the current program's, or the structural proposal's
-            GradPlan pplan{};
             OpMasks pmask{};
             double pll;
             int pts = d.n;
-            if (MALA) {
-                pplan = structural ? grad_plan<HI>(prop) : gplan;
-                pll = have_pll ? pll_m : wave_loglik_slab<HI>(prop, pplan, d, dx, dy, dw, lv);
+            if (have_pll) {
+                pll = pll_m;
+                pts = min(mala_pts, d.n);
             } else {
                 pmask = structural ? op_masks<HI>(prop) : cmask;
                 // a = pprior + pll - post + extra < -17 rejects without a draw (cfg_accept);
@@ -402,7 +440,8 @@ k_mh_config_wave(GprMHArgs a) {
                     c_sa++;
                     grad_ok = false;
                     zblk = 0xffffffffu;
-                    if (MALA) gplan = pplan; else cmask = pmask;
+                    cmask = pmask;
+                    if (MALA) gplan = grad_plan<HI>(cur);
                     cur_opq = p_opq;
                     cur_nc = p_nc;
                     cur_bok = batch_ok<BATCH, HI>(cur, mk);
@@ -449,7 +488,8 @@ k_mh_config_wave(GprMHArgs a) {
 
 template <int KC, bool HI, bool MALA, bool MET>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
-    const size_t sh = MALA ? (HI ? 0 : sizeof(float) * 64 * (size_t)a.dmax_cap * a.wpg)
+    const size_t mrows = (size_t)std::max(GPR_MALA_LDS_ROWS, GPR_MAX_DEPTH + 2);
+    const size_t sh = MALA ? sizeof(float) * 64 * mrows * a.wpg
                            : (sizeof(float) * (GPR_MAX_DEPTH + 2) * 64 * KC + sizeof(double) * 16) * (size_t)a.wpg;
     if (sh > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA, MET>,

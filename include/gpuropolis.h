@@ -103,7 +103,8 @@ typedef struct {
     long long const_proposed, const_accepted;
     long long steps;            /* chain-steps executed */
     long long node_evals;       /* executed token x datapoint evaluations */
-    long long alg_flops;        /* algorithmic FLOPs: n*(non-leaf tokens + 4) + constants per evaluation */
+    long long alg_flops;        /* algorithmic FLOPs: n*(non-leaf tokens + 4) + constants per evaluation,
+                                   plus n*(non-leaf tokens + 2 + constants) per MALA gradient */
     double last_block_ms;       /* device time of the last MH block (HIP events) */
     int schedule_used;          /* GPR_SCHED_LANE or GPR_SCHED_WAVE */
     int map_chain;              /* chain of the MAP after the last resample (dataset 0) */
