@@ -515,6 +515,207 @@ __device__ double wave_loglik(const WaveProg &p, const GprDataset &d, const floa
     return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
 }
 
+// ---------------------------------------------------------------- forward-mode gradient
+// The MALA proposals of short programs with few constants get their gradient in the same
+// interpreter pass as their likelihood: every stack entry carries the value and its
+// tangents with respect to the program's constant tokens (constant of rank j: tangent j),
+// W = 1 + GPR_DUAL_NT floats, pushed to the wave's LDS stack as one 16-byte store.  The
+// values take the same operations on the same operands as wave_eval_ls, so the LL is
+// bit-identical to it (and early rejection stops a pass at the same chunk); each lane
+// adds the per-point gradient (-2 r w) f' of its points, and one wave sum per constant
+// ends the pass.  Programs qualify when they use only the config grammar (x, constant,
+// one, + - * / pow exp log neg), hold at most GPR_DUAL_NT constant tokens and no
+// constant-operand token, and stay within GPR_DUAL_DEPTH stack entries.
+#define GPR_DUAL_NT 3
+#define GPR_DUAL_DEPTH 6
+#define GPR_DUAL_W (1 + GPR_DUAL_NT)
+// floats of the dual stack of one wave: GPR_DUAL_DEPTH + 2 levels of 64 lanes x W
+#define GPR_DUAL_LDS_FLOATS ((GPR_DUAL_DEPTH + 2) * 64 * GPR_DUAL_W)
+
+// A zero tangent contributes nothing, whatever the partial derivative (an infinite or NaN
+// partial of a constant-free operand must not reach the constants, as in the adjoint).
+__device__ __forceinline__ float tmul(float t, float partial) { return t != 0.0f ? t * partial : 0.0f; }
+
+template <bool HI>
+__device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
+    constexpr uint32_t GRAMMAR = (1u << GPR_OP_X) | (1u << GPR_OP_CONST) | (1u << GPR_OP_ONE) | (1u << GPR_OP_ADD) |
+                                 (1u << GPR_OP_SUB) | (1u << GPR_OP_MUL) | (1u << GPR_OP_DIV) | (1u << GPR_OP_POW) |
+                                 (1u << GPR_OP_EXP) | (1u << GPR_OP_LOG) | (1u << GPR_OP_NEG);
+    const int l = lane_id();
+    const bool v0 = l < p.len, v1 = HI && l + 64 < p.len;
+    const bool bad = (v0 && !((GRAMMAR >> (p.o0 & 31)) & 1u)) || (v1 && !((GRAMMAR >> (p.o1 & 31)) & 1u));
+    return ballot(bad) == 0 && const_count<HI>(p) <= GPR_DUAL_NT && dmax <= GPR_DUAL_DEPTH;
+}
+
+// LL of the wave's program (as wave_loglik<1, HI, true>) and, when the pass completes,
+// gs[j] = sum over points of (-2 r w) df/dc_j for the constant token of rank j (token
+// order; cm = the constant-token masks).  ls: the wave's dual stack region.
+template <bool HI>
+__device__ double wave_loglik_dual(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
+                                   const float *dw, const Chunk<1> &c0, int nchunks, float *ls, const OpMasks &m,
+                                   const uint64_t (&cm)[2], double reject_below, int *points,
+                                   float (&gs)[GPR_DUAL_NT]) {
+    constexpr int W = GPR_DUAL_W;
+    const int l = lane_id();
+    float *lane = ls + 2 * 64 * W + l * W;   // level -2 of the stack below T, U
+    float acc = 0.0f, comp = 0.0f;
+    float ga[GPR_DUAL_NT];
+#pragma unroll
+    for (int j = 0; j < GPR_DUAL_NT; j++) ga[j] = 0.0f;
+    const bool folded = d.wconst > 0.0;
+    const float wf = (float)d.wconst;
+    const int nlo = __popcll(cm[0]);
+    if (points) *points = d.n;
+    for (int ch = 0; ch < nchunks; ch++) {
+        Chunk<1> c;
+        if (ch == 0) c = c0; else c.load(dx, dy, dw, d.n, ch * 64);
+        const float xv = c.x[0];
+        float T[W], U[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) { T[j] = 0.0f; U[j] = 0.0f; }
+        int sp = 0;
+        for (int t = 0; t < p.len; t++) {
+            const int li = t & 63;
+            const int h = (HI && t >= 64) ? 1 : 0;
+            const uint64_t bit = 1ull << li;
+            if (m.leaf[h] & bit) {
+                // the leaf's value and tangent: x (0), or constant of rank rk (e_rk)
+                const bool isx = (m.x[h] & bit) != 0;
+                const float cv = h ? rl_f(p.c1, li) : rl_f(p.c0, li);
+                const int rk = isx ? -1 : (h ? nlo : 0) + __popcll(cm[h] & (bit - 1));
+                const float b = isx ? xv : cv;
+                if (m.cheap[h] & (bit << 1)) {
+                    // leaf followed by +, - or *: T = T op leaf, stack untouched
+                    const uint64_t nb = bit << 1;
+                    if (m.mul[h] & nb) {
+                        const float tv = T[0];
+                        T[0] = T[0] * b;
+#pragma unroll
+                        for (int j = 1; j < W; j++) T[j] = tmul(T[j], b) + (rk == j - 1 ? tv : 0.0f);
+                    } else {
+                        const float sgn = (m.sub[h] & nb) ? -1.0f : 1.0f;
+                        T[0] = __builtin_fmaf(sgn, b, T[0]);
+#pragma unroll
+                        for (int j = 1; j < W; j++) T[j] = rk == j - 1 ? T[j] + sgn : T[j];
+                    }
+                    t++;
+                    continue;
+                }
+                ls_store<W>(lane + (sp - 2) * 64 * W, U);
+#pragma unroll
+                for (int j = 0; j < W; j++) U[j] = T[j];
+                sp++;
+                T[0] = b;
+#pragma unroll
+                for (int j = 1; j < W; j++) T[j] = rk == j - 1 ? 1.0f : 0.0f;
+                continue;
+            }
+            if (m.cheap[h] & bit) {
+                // U op T for +, -, *
+                if (m.mul[h] & bit) {
+                    const float uv = U[0], tv = T[0];
+                    T[0] = U[0] * T[0];
+#pragma unroll
+                    for (int j = 1; j < W; j++) T[j] = tmul(U[j], tv) + tmul(T[j], uv);
+                } else {
+                    const float sgn = (m.sub[h] & bit) ? -1.0f : 1.0f;
+                    T[0] = __builtin_fmaf(sgn, T[0], U[0]);
+#pragma unroll
+                    for (int j = 1; j < W; j++) T[j] = __builtin_fmaf(sgn, T[j], U[j]);
+                }
+                ls_load<W>(lane + (sp - 3) * 64 * W, U);
+                sp--;
+                continue;
+            }
+            const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
+            if (op == GPR_OP_ONE) {
+                ls_store<W>(lane + (sp - 2) * 64 * W, U);
+#pragma unroll
+                for (int j = 0; j < W; j++) { U[j] = T[j]; T[j] = j == 0 ? 1.0f : 0.0f; }
+                sp++;
+            } else if (op == GPR_OP_DIV) {
+                const float a = U[0], b = T[0];
+                const float q = a / b;
+                const float ib = 1.0f / b, nq = -(a / (b * b));
+#pragma unroll
+                for (int j = 1; j < W; j++) T[j] = tmul(U[j], ib) + tmul(T[j], nq);
+                T[0] = q;
+                ls_load<W>(lane + (sp - 3) * 64 * W, U);
+                sp--;
+            } else if (op == GPR_OP_POW) {
+                const float a = U[0], b = T[0];
+                const float v = powf(a, b);
+                const float pa = b * powf(a, b - 1.0f), pb = v * logf(a);
+#pragma unroll
+                for (int j = 1; j < W; j++) T[j] = tmul(U[j], pa) + tmul(T[j], pb);
+                T[0] = v;
+                ls_load<W>(lane + (sp - 3) * 64 * W, U);
+                sp--;
+            } else if (op == GPR_OP_EXP) {
+                const float v = expf(T[0]);
+#pragma unroll
+                for (int j = 1; j < W; j++) T[j] = tmul(T[j], v);
+                T[0] = v;
+            } else if (op == GPR_OP_LOG) {
+                const float a = T[0];
+                const float ia = 1.0f / a;
+                T[0] = logf(a);
+#pragma unroll
+                for (int j = 1; j < W; j++) T[j] = tmul(T[j], ia);
+            } else {   // GPR_OP_NEG (dual_ok admits no other op)
+#pragma unroll
+                for (int j = 0; j < W; j++) T[j] = -T[j];
+            }
+        }
+        const int i = ch * 64 + l;
+        const bool valid = i < d.n;
+        const float r = T[0] - c.y[0];
+        const float term = valid ? (r * r) * c.w[0] : 0.0f;
+        const float yv = term - comp;
+        const float tv = acc + yv;
+        comp = (tv - acc) - yv;
+        acc = tv;
+        const float G = (-2.0f * r) * (folded ? wf : c.w[0]);
+#pragma unroll
+        for (int j = 0; j < GPR_DUAL_NT; j++) ga[j] = valid ? __builtin_fmaf(G, T[j + 1], ga[j]) : ga[j];
+        if (ch + 1 < nchunks && reject_below > -GPR_INF) {
+            const double lb = (double)wave_sum_f(acc - comp) * (1.0 - 1e-4);
+            const double ub = d.lconst - (d.wconst > 0.0 ? d.wconst * lb : lb);
+            if (!(ub >= reject_below)) {
+                if (points) *points = (ch + 1) * 64;
+                return -GPR_INF;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < GPR_DUAL_NT; j++) gs[j] = j < nlo + (HI ? __popcll(cm[1]) : 0) ? wave_sum_f(ga[j]) : 0.0f;
+    const double s = wave_sum_d((double)acc - (double)comp);
+    if (!__builtin_isfinite(s)) return -GPR_INF;
+    return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
+}
+
+// The gradient of the log posterior in the lane layout of wave_loglik_grad: lane t of g0
+// (token t < 64) or g1 holds d/dc_t (from gs by the constant's rank) plus the Cauchy
+// prior's -2c/(1+c^2); ok is false when one of them is not finite.
+template <bool HI>
+__device__ __forceinline__ void dual_grad_lanes(const WaveProg &p, const uint64_t (&cm)[2],
+                                                const float (&gs)[GPR_DUAL_NT], double &g0, double &g1, bool &ok) {
+    const int l = lane_id();
+    const uint64_t below = (1ull << l) - 1ull;
+    const int r0 = __popcll(cm[0] & below);
+    const int r1 = __popcll(cm[0]) + (HI ? __popcll(cm[1] & below) : 0);
+    const bool c0 = (cm[0] >> l) & 1ull, c1 = HI && ((cm[1] >> l) & 1ull);
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < GPR_DUAL_NT; j++) {
+        s0 = r0 == j ? gs[j] : s0;
+        s1 = r1 == j ? gs[j] : s1;
+    }
+    g0 = c0 ? (double)s0 - 2.0 * (double)p.c0 / (1.0 + (double)p.c0 * (double)p.c0) : 0.0;
+    g1 = c1 ? (double)s1 - 2.0 * (double)p.c1 / (1.0 + (double)p.c1 * (double)p.c1) : 0.0;
+    ok = ballot((c0 && !__builtin_isfinite(g0)) || (c1 && !__builtin_isfinite(g1))) == 0;
+}
+
 // Description-length prior plus Cauchy(0,1) on every constant token, summed in token
 // order in FP64 (the same order as the oracle, so the value is bit-reproducible).
 template <bool HI>

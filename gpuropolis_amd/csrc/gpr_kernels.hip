@@ -40,11 +40,18 @@ __global__ void k_fill_logint() {
 // The plain one-point-per-lane kernel is latency bound: six waves per SIMD (80 VGPRs) run
 // it faster than the four that fit unspilled, although the compiler then spills per-tile
 // values to scratch at every tile start and reloads some of them in the step loop.
+// The MALA kernel likewise: unconstrained it takes about 160 VGPRs (three waves per SIMD);
+// held to five waves (96 VGPRs, per-tile values spilled) it runs the full form's
+// constant steps 15 % and its structure steps 21 % faster.
 // MET: some dataset of the launch has a proposal metric.  The metric path holds many
 // wave-uniform values (matrix rows, pointers), so launches without a metric use a
 // build without it, whose scalar registers then fit with far fewer spills.
+#ifndef GPR_MALA_WAVES_PER_EU
+#define GPR_MALA_WAVES_PER_EU 5
+#endif
 template <int KC, bool HI, bool MALA, bool MET>
-__global__ void __launch_bounds__(GPR_WG) __attribute__((amdgpu_waves_per_eu((!MALA && KC == 1) ? 6 : 1)))
+__global__ void __launch_bounds__(GPR_WG)
+__attribute__((amdgpu_waves_per_eu(MALA ? GPR_MALA_WAVES_PER_EU : (KC == 1 ? 6 : 1))))
 k_mh_config_wave(GprMHArgs a) {
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
@@ -56,7 +63,9 @@ k_mh_config_wave(GprMHArgs a) {
     // memory, lvg.
     constexpr int SROWS = GPR_MAX_DEPTH + 2;
     constexpr int MROWS = SROWS > GPR_MALA_LDS_ROWS ? SROWS : GPR_MALA_LDS_ROWS;
-    float *ls = gpr_lds_slab + (size_t)wave * ((MALA ? MROWS : SROWS) * 64 * KC);
+    // (the forward-mode interpreter's stack of value + tangents shares it too)
+    constexpr int MFLOATS = MROWS * 64 > GPR_DUAL_LDS_FLOATS ? MROWS * 64 : GPR_DUAL_LDS_FLOATS;
+    float *ls = gpr_lds_slab + (size_t)wave * (MALA ? MFLOATS : SROWS * 64 * KC);
     // after the wpg value stacks: each wave's priors of the (up to 16) steps of its draw batch
     double *bpri = MALA ? nullptr
                         : (double *)(gpr_lds_slab + (size_t)a.wpg * ((GPR_MAX_DEPTH + 2) * 64 * KC)) + wave * 16;
@@ -111,7 +120,7 @@ k_mh_config_wave(GprMHArgs a) {
         double prior = 0.0, ll = 0.0;
         // per-lane constant-proposal block, reused by four consecutive steps
         gpr_u4 wlo = {0, 0, 0, 0}, whi = {0, 0, 0, 0};
-        uint32_t wblk = 0xffffffffu;
+        uint32_t wblk = 0xffffffffu, whblk = 0xffffffffu;
         // Batched random-walk moves (BATCH: no MALA; per program: tape within the low half,
         // no metric, at most 16 constant tokens and no constant-operand token).  The draw of
         // step s for a constant is word s & 3 of the Philox block s >> 2 of its token, so a
@@ -139,6 +148,17 @@ k_mh_config_wave(GprMHArgs a) {
         // per-program values that change only with the structure: (operator tokens + 4)
         // and constants of the current program for the work counters, and whether its
         // random-walk moves can be batched (see BATCH below)
+        // MALA: whether the current program's gradient comes from the forward-mode
+        // interpreter (dual_ok), with its constant-token masks
+        bool cur_dual = false;
+        uint64_t ccm[2] = {0, 0};
+        if (MALA) {
+            int D0, D1, dm;
+            wave_depths<HI>(cur, D0, D1, dm);
+            cur_dual = mk == 0 && dual_ok<HI>(cur, dm);
+            ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
+            ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
+        }
         int cur_opq, cur_nc;
         prog_counts<HI>(cur, cur_opq, cur_nc);
         bool cur_bok = batch_ok<BATCH, HI>(cur, mk);
@@ -287,13 +307,18 @@ k_mh_config_wave(GprMHArgs a) {
                 if (BATCH || (step >> 2) != wblk) {
                     wblk = step >> 2;
                     wlo = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 0);
-                    if (HI) whi = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 1);
+                }
+                // the high half's block only for programs that reach it
+                const bool hi_live = HI && cur.len > 64;
+                if (hi_live && (BATCH || (step >> 2) != whblk)) {
+                    whblk = step >> 2;
+                    whi = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 1);
                 }
                 const int wi = (int)(step & 3);
                 // identity-metric constants; metric rows move together below
                 const bool i0 = k0 && slot0 < 0, i1 = k1 && slot1 < 0;
                 const float z0 = gpr_normal(gpr_u01(gpr_u4_word(wlo, wi)));
-                const float z1 = HI ? gpr_normal(gpr_u01(gpr_u4_word(whi, wi))) : 0.0f;
+                const float z1 = hi_live ? gpr_normal(gpr_u01(gpr_u4_word(whi, wi))) : 0.0f;
                 float zr = 0.0f, cr = 0.0f, dr = 0.0f;
                 if (mk) {
                     zr = gather_f(z0, z1, mrow, HI);
@@ -316,8 +341,17 @@ k_mh_config_wave(GprMHArgs a) {
                     if (!grad_ok) {
                         double h0, h1;
                         bool gok;
-                        (void)wave_loglik_grad<HI>(cur, gplan, d, dx, dy, dw,
-                                                   cur.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1, gok);
+                        if (cur_dual) {
+                            float gs[GPR_DUAL_NT];
+                            int np_;
+                            const double cl = wave_loglik_dual<HI>(cur, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks,
+                                                                   ls, cmask, ccm, -GPR_INF, &np_, gs);
+                            dual_grad_lanes<HI>(cur, ccm, gs, h0, h1, gok);
+                            gok = gok && __builtin_isfinite(cl);
+                        } else {
+                            (void)wave_loglik_grad<HI>(cur, gplan, d, dx, dy, dw,
+                                                       cur.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1, gok);
+                        }
                         c_fl += (unsigned long long)(cur_opq - 2 + cur_nc) * (unsigned long long)d.n;
                         cg0 = gok ? (float)h0 : 0.0f;
                         cg1 = gok ? (float)h1 : 0.0f;
@@ -358,7 +392,13 @@ k_mh_config_wave(GprMHArgs a) {
                     if (__builtin_isfinite(post) && __builtin_isfinite(pre_prior) && __builtin_isfinite(fq))
                         rb = post - pre_prior + fq - 18.0 -
                              1e-12 * (__builtin_fabs(post) + __builtin_fabs(pre_prior) + __builtin_fabs(fq));
-                    if (HI && prop.len <= 64)
+                    // programs that qualify get the LL and the gradient from one forward-mode
+                    // pass of the interpreter (bound included)
+                    float dgs[GPR_DUAL_NT];
+                    if (cur_dual)
+                        pll_m = wave_loglik_dual<HI>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls, cmask,
+                                                     ccm, rb, &mala_pts, dgs);
+                    else if (HI && prop.len <= 64)
                         pll_m = wave_loglik<KC, false, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &cmask, rb,
                                                              &mala_pts);
                     else
@@ -370,8 +410,13 @@ k_mh_config_wave(GprMHArgs a) {
                     } else {
                         double h0, h1;
                         bool pok;
-                        (void)wave_loglik_grad<HI>(prop, gplan, d, dx, dy, dw,
-                                                   prop.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1, pok);
+                        if (cur_dual) {
+                            dual_grad_lanes<HI>(prop, ccm, dgs, h0, h1, pok);
+                            pok = pok && __builtin_isfinite(pll_m);
+                        } else {
+                            (void)wave_loglik_grad<HI>(prop, gplan, d, dx, dy, dw,
+                                                       prop.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1, pok);
+                        }
                         c_fl += (unsigned long long)(cur_opq - 2 + cur_nc) * (unsigned long long)d.n;
                         pg0 = pok ? (float)h0 : 0.0f;
                         pg1 = pok ? (float)h1 : 0.0f;
@@ -445,6 +490,13 @@ k_mh_config_wave(GprMHArgs a) {
                     cur_opq = p_opq;
                     cur_nc = p_nc;
                     cur_bok = batch_ok<BATCH, HI>(cur, mk);
+                    if (MALA) {
+                        int D0, D1, dm;
+                        wave_depths<HI>(cur, D0, D1, dm);
+                        cur_dual = mk == 0 && dual_ok<HI>(cur, dm);
+                        ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
+                        ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
+                    }
                     cur_b16 = cur_bok && const_count<HI>(cur) <= 4;
                 } else {
                     c_ca++;
@@ -488,8 +540,9 @@ k_mh_config_wave(GprMHArgs a) {
 
 template <int KC, bool HI, bool MALA, bool MET>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
-    const size_t mrows = (size_t)std::max(GPR_MALA_LDS_ROWS, GPR_MAX_DEPTH + 2);
-    const size_t sh = MALA ? sizeof(float) * 64 * mrows * a.wpg
+    const size_t mfloats = std::max((size_t)std::max(GPR_MALA_LDS_ROWS, GPR_MAX_DEPTH + 2) * 64,
+                                    (size_t)GPR_DUAL_LDS_FLOATS);
+    const size_t sh = MALA ? sizeof(float) * mfloats * a.wpg
                            : (sizeof(float) * (GPR_MAX_DEPTH + 2) * 64 * KC + sizeof(double) * 16) * (size_t)a.wpg;
     if (sh > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA, MET>,

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libgpuropolis.so")
+# GPR_LIB names another build of the library (A/B measurements of build variants)
+LIB_PATH = os.environ.get("GPR_LIB") or os.path.join(PKG, "libgpuropolis.so")
 
 OPS = ["X", "CONST", "ONE", "ADD", "SUB", "MUL", "DIV", "POW", "RSUB", "RDIV", "RPOW", "NEG", "EXP",
        "LOG", "INV", "SQRT", "SIN", "ASIN", "COS", "ACOS", "TAN", "ATAN", "ABS", "CADD", "CMUL",
