@@ -485,6 +485,32 @@ extern "C" int gpr_read_state(gpr_ctx *c, uint8_t *ops, float *cs, int32_t *len,
     return GPR_OK;
 }
 
+// Heap state in main.cu's types (int op codes), converted on the host.
+extern "C" int gpr_write_heap_i32(gpr_ctx *c, const int32_t *P, const float *C) {
+    if (!c || !c->have_data) return fail(GPR_E_STATE, "gpr_set_data must come first");
+    if (c->cfg.semantics != GPR_SEM_REFERENCE) return fail(GPR_E_ARG, "heap arrays need reference semantics");
+    if (!P || !C) return fail(GPR_E_ARG, "null heap arrays");
+    const size_t n = (size_t)GPR_REF_SLOTS * (size_t)c->N;
+    std::vector<uint8_t> ops(n);
+    for (size_t i = 0; i < n; i++) {
+        if (P[i] < 0 || P[i] >= GPR_REF_NCODES)
+            return fail(GPR_E_ARG, "heap op code " + std::to_string(P[i]) + " at element " + std::to_string(i) +
+                                       " is outside [0, " + std::to_string(GPR_REF_NCODES) + ")");
+        ops[i] = (uint8_t)P[i];
+    }
+    return gpr_write_state(c, ops.data(), C, nullptr);
+}
+
+extern "C" int gpr_read_heap_i32(gpr_ctx *c, int32_t *P, float *C, double *prior, double *ll) {
+    if (!c || !c->have_data) return fail(GPR_E_STATE, "no state");
+    if (c->cfg.semantics != GPR_SEM_REFERENCE) return fail(GPR_E_ARG, "heap arrays need reference semantics");
+    const size_t n = (size_t)GPR_REF_SLOTS * (size_t)c->N;
+    std::vector<uint8_t> ops(P ? n : 0);
+    if (int r = gpr_read_state(c, P ? ops.data() : nullptr, C, nullptr, prior, ll)) return r;
+    for (size_t i = 0; P && i < n; i++) P[i] = ops[i];
+    return GPR_OK;
+}
+
 // The host copies of the clock are authoritative between calls: graph replays upload
 // them before the launch and every block advances them on return.
 extern "C" int gpr_get_clock(const gpr_ctx *c, uint32_t *step, uint32_t *block) {

@@ -479,7 +479,7 @@ __device__ __forceinline__ bool batch_ok(const WaveProg &p, int mk) {
 }
 
 template <int KC, bool HI, bool LS = false>
-__device__ double wave_loglik(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
+__device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
                               const float *dw, const Chunk<KC> &c0, int nchunks, float *ls = nullptr,
                               const OpMasks *pm = nullptr, double reject_below = -GPR_INF, int *points = nullptr) {
     float acc = 0.0f, comp = 0.0f;
@@ -551,7 +551,7 @@ __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
 // gs[j] = sum over points of (-2 r w) df/dc_j for the constant token of rank j (token
 // order; cm = the constant-token masks).  ls: the wave's dual stack region.
 template <bool HI>
-__device__ double wave_loglik_dual(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
+__device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
                                    const float *dw, const Chunk<1> &c0, int nchunks, float *ls, const OpMasks &m,
                                    const uint64_t (&cm)[2], double reject_below, int *points,
                                    float (&gs)[GPR_DUAL_NT]) {
@@ -719,7 +719,7 @@ __device__ __forceinline__ void dual_grad_lanes(const WaveProg &p, const uint64_
 // Description-length prior plus Cauchy(0,1) on every constant token, summed in token
 // order in FP64 (the same order as the oracle, so the value is bit-reproducible).
 template <bool HI>
-__device__ double wave_prior(const WaveProg &p, double lnsym) {
+__device__ __forceinline__ double wave_prior(const WaveProg &p, double lnsym) {
     const int l = lane_id();
     const bool v0 = l < p.len, v1 = HI && (l + 64) < p.len;
     int dl = p.len + __popcll(ballot(v0 && p.o0 >= GPR_OP_CADD)) + (HI ? __popcll(ballot(v1 && p.o1 >= GPR_OP_CADD)) : 0);
@@ -1079,7 +1079,7 @@ __device__ __forceinline__ void grad_chunk_sums(uint64_t m0, uint64_t m1, const 
 // lv: this wave's LDS slab of at least len * 64 floats.  Returns false in ok when the
 // LL or a constant's gradient is not finite.
 template <bool HI>
-__device__ double wave_loglik_grad(const WaveProg &p, const GradPlan &g, const GprDataset &d, const float *dx,
+__device__ __forceinline__ double wave_loglik_grad(const WaveProg &p, const GradPlan &g, const GprDataset &d, const float *dx,
                                    const float *dy, const float *dw, float *lv, double &g0, double &g1, bool &ok) {
     const int l = lane_id();
     float *lvl = lv + l;

@@ -40,9 +40,10 @@ struct GprTile {
 };
 
 #define GPR_WAVES_PER_WG 4
-// MALA: token-value rows each wave of the MH kernel keeps in LDS; the adjoint sweeps of
-// longer programs use the wave's slab in global memory (GprMHArgs::mala_gslab)
-#define GPR_MALA_LDS_ROWS 20
+// MALA: token-value rows each wave of the MH kernel keeps in LDS (8 KB, the size the
+// forward-mode interpreter's stack takes anyway); the adjoint sweeps of longer programs
+// use the wave's slab in global memory (GprMHArgs::mala_gslab)
+#define GPR_MALA_LDS_ROWS 32
 #define GPR_WG 256
 
 // Counters accumulated by the MH kernels (device, u64).

@@ -42,16 +42,20 @@ __global__ void k_fill_logint() {
 // values to scratch at every tile start and reloads some of them in the step loop.
 // The MALA kernel likewise: unconstrained it takes about 160 VGPRs (three waves per SIMD);
 // held to five waves (96 VGPRs, per-tile values spilled) it runs the full form's
-// constant steps 15 % and its structure steps 21 % faster.
+// constant steps 15 % and its structure steps 21 % faster.  The metric build runs at four
+// waves (128 VGPRs): unconstrained it takes 138 and three waves, 20 % slower.
 // MET: some dataset of the launch has a proposal metric.  The metric path holds many
 // wave-uniform values (matrix rows, pointers), so launches without a metric use a
 // build without it, whose scalar registers then fit with far fewer spills.
 #ifndef GPR_MALA_WAVES_PER_EU
 #define GPR_MALA_WAVES_PER_EU 5
 #endif
+#ifndef GPR_MALA_MET_WAVES_PER_EU
+#define GPR_MALA_MET_WAVES_PER_EU 4
+#endif
 template <int KC, bool HI, bool MALA, bool MET>
 __global__ void __launch_bounds__(GPR_WG)
-__attribute__((amdgpu_waves_per_eu(MALA ? GPR_MALA_WAVES_PER_EU : (KC == 1 ? 6 : 1))))
+__attribute__((amdgpu_waves_per_eu(MALA ? (MET ? GPR_MALA_MET_WAVES_PER_EU : GPR_MALA_WAVES_PER_EU) : (KC == 1 ? 6 : 1))))
 k_mh_config_wave(GprMHArgs a) {
     const int wave = uni((int)(threadIdx.x >> 6));
     const int l = lane_id();
@@ -155,7 +159,7 @@ k_mh_config_wave(GprMHArgs a) {
         if (MALA) {
             int D0, D1, dm;
             wave_depths<HI>(cur, D0, D1, dm);
-            cur_dual = mk == 0 && dual_ok<HI>(cur, dm);
+            cur_dual = !MET && mk == 0 && dual_ok<HI>(cur, dm);
             ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
             ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
         }
@@ -394,18 +398,21 @@ k_mh_config_wave(GprMHArgs a) {
                              1e-12 * (__builtin_fabs(post) + __builtin_fabs(pre_prior) + __builtin_fabs(fq));
                     // programs that qualify get the LL and the gradient from one forward-mode
                     // pass of the interpreter (bound included)
+                    // The others are pre-checked by the plain interpreter when the data
+                    // spans several chunks (one chunk cannot stop early), then swept.
                     float dgs[GPR_DUAL_NT];
+                    const bool precheck = !MET && !cur_dual && nchunks > 1;
                     if (cur_dual)
                         pll_m = wave_loglik_dual<HI>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls, cmask,
                                                      ccm, rb, &mala_pts, dgs);
-                    else if (HI && prop.len <= 64)
+                    else if (precheck && HI && prop.len <= 64)
                         pll_m = wave_loglik<KC, false, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &cmask, rb,
                                                              &mala_pts);
-                    else
+                    else if (precheck)
                         pll_m = wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &cmask, rb,
                                                           &mala_pts);
                     have_pll = true;
-                    if (!__builtin_isfinite(pll_m) && __builtin_isfinite(post)) {
+                    if ((cur_dual || precheck) && !__builtin_isfinite(pll_m) && __builtin_isfinite(post)) {
                         extra = -GPR_INF;      // rejected: invalid, or below the bound
                     } else {
                         double h0, h1;
@@ -414,8 +421,13 @@ k_mh_config_wave(GprMHArgs a) {
                             dual_grad_lanes<HI>(prop, ccm, dgs, h0, h1, pok);
                             pok = pok && __builtin_isfinite(pll_m);
                         } else {
-                            (void)wave_loglik_grad<HI>(prop, gplan, d, dx, dy, dw,
-                                                       prop.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1, pok);
+                            const double gl = wave_loglik_grad<HI>(prop, gplan, d, dx, dy, dw,
+                                                                   prop.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1,
+                                                                   pok);
+                            if (!precheck) {   // the sweeps' LL is the interpreter's, bit for bit
+                                pll_m = gl;
+                                mala_pts = d.n;
+                            }
                         }
                         c_fl += (unsigned long long)(cur_opq - 2 + cur_nc) * (unsigned long long)d.n;
                         pg0 = pok ? (float)h0 : 0.0f;
@@ -493,7 +505,7 @@ k_mh_config_wave(GprMHArgs a) {
                     if (MALA) {
                         int D0, D1, dm;
                         wave_depths<HI>(cur, D0, D1, dm);
-                        cur_dual = mk == 0 && dual_ok<HI>(cur, dm);
+                        cur_dual = !MET && mk == 0 && dual_ok<HI>(cur, dm);
                         ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
                         ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
                     }

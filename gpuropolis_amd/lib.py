@@ -35,7 +35,8 @@ EXPORTS = ["gpr_config_default", "gpr_create", "gpr_destroy", "gpr_last_error", 
            "gpr_resample", "gpr_run_blocks_graph", "gpr_get_stats", "gpr_reset_stats", "gpr_sync", "gpr_stream",
            "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws",
            "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning", "gpr_set_metric", "gpr_ref_op_set",
-           "gpr_get_clock", "gpr_set_clock", "gpr_eval_loglik_noalloc_async"]
+           "gpr_get_clock", "gpr_set_clock", "gpr_eval_loglik_noalloc_async", "gpr_write_heap_i32",
+           "gpr_read_heap_i32"]
 NOALLOC_MAX_POINTS = 65536
 
 
@@ -108,6 +109,8 @@ def load():
         "gpr_ref_op_set": (C.c_int, [C.c_int, vp, C.c_int, C.POINTER(C.c_char_p)]),
         "gpr_test_ref_decode": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, vp]),
         "gpr_test_draws": (C.c_int, [vp, vp, C.c_int, C.c_uint64, vp]),
+        "gpr_write_heap_i32": (C.c_int, [vp, vp, vp]),
+        "gpr_read_heap_i32": (C.c_int, [vp, vp, vp, vp, vp]),
         "gpr_eval_loglik_noalloc_async": (C.c_int, [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int,
                                                     C.c_int, vp, vp]),
     }
@@ -201,6 +204,20 @@ class Context:
         ll = np.zeros(self.N, dtype=np.float64)
         check(load().gpr_read_state(self.h, _p(ops), _p(cs), _p(ln), _p(pr), _p(ll)))
         return ops, cs, ln, pr, ll
+
+    def write_heap_i32(self, P, Cs):
+        """Reference semantics: heap op codes as int32 [31][N] (main.cu's op type) and constants."""
+        P = np.ascontiguousarray(P, dtype=np.int32)
+        Cs = np.ascontiguousarray(Cs, dtype=np.float32)
+        check(load().gpr_write_heap_i32(self.h, _p(P), _p(Cs)))
+
+    def read_heap_i32(self):
+        P = np.zeros((31, self.N), dtype=np.int32)
+        Cs = np.zeros((31, self.N), dtype=np.float32)
+        pr = np.zeros(self.N, dtype=np.float64)
+        ll = np.zeros(self.N, dtype=np.float64)
+        check(load().gpr_read_heap_i32(self.h, _p(P), _p(Cs), _p(pr), _p(ll)))
+        return P, Cs, pr, ll
 
     def clock(self):
         """(step, block): where the next block's and the next resample's Philox streams start."""

@@ -141,6 +141,14 @@ int gpr_init_chains(gpr_ctx *ctx);
 int gpr_write_state(gpr_ctx *ctx, const uint8_t *ops, const float *cs, const int32_t *len);
 int gpr_read_state(gpr_ctx *ctx, uint8_t *ops, float *cs, int32_t *len, double *prior, double *ll);
 
+/* The same copies for reference semantics in the reference's own types: P holds the heap
+ * op codes as int (typedef int op, main.cu:45; the pinned host arrays of main.cu:1123-1130)
+ * and C the constants, both [31][N], so a caller keeping main.cu's arrays passes them
+ * unchanged (the cudaMemcpy calls of main.cu:1143-1144, 1163-1164 and 1233-1234).  Codes
+ * outside [0, GPR_REF_NCODES) are rejected.  prior and ll may be NULL. */
+int gpr_write_heap_i32(gpr_ctx *ctx, const int32_t *P, const float *C);
+int gpr_read_heap_i32(gpr_ctx *ctx, int32_t *P, float *C, double *prior, double *ll);
+
 /* The Philox clock: `step` is the MH step index the next block starts at, `block` the
  * block index the next resample draws with.  gpr_set_data resets both to 0; every block
  * advances step by its length and block by one.  No counterpart in the reference, which

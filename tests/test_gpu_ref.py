@@ -175,3 +175,34 @@ def test_ref_op_set_trajectories_match_oracle(oracle, gpr, sched, ops, dname):
     codes, _ = oracle.opset(ops)
     assert set(np.unique(g_hop)) <= set(codes)
 
+
+
+def test_heap_arrays_in_the_references_types(oracle, gpr):
+    """gpr_write_heap_i32 / gpr_read_heap_i32 take main.cu's own arrays (int op codes,
+    main.cu:45, 1123-1130): the state they write is the uint8 state, a block run from it
+    equals the block run from the same state written as uint8, and codes outside the op
+    table are rejected."""
+    x, y, sd = data("S")
+    nch = 256
+    hop, hc = oracle.ref_init(12, nch)
+    out = []
+    for form in ("u8", "i32"):
+        with gpr.Context(semantics=gpr.SEM_REFERENCE, seed=12, schedule=gpr.SCHED_WAVE) as ctx:
+            ctx.set_data([(x, y, sd)], [nch])
+            if form == "u8":
+                ctx.write_state(hop, hc)
+            else:
+                ctx.write_heap_i32(hop.astype(np.int32), hc)
+                g_hop, g_hc, _, _, _ = ctx.read_state()
+                assert np.array_equal(g_hop, hop) and np.array_equal(g_hc.view(np.uint32), hc.view(np.uint32))
+                bad = hop.astype(np.int32)
+                bad[5, 7] = gpr.ROP["KB"] + 1
+                with pytest.raises(gpr.GprError, match="outside"):
+                    ctx.write_heap_i32(bad, hc)
+            say(f"heap arrays as {form}: one block")
+            ctx.run_block(50, 7)
+            out.append(ctx.read_heap_i32())
+    for u, v in zip(out[0], out[1]):
+        assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
+    P, Cs, pr, ll = out[1]
+    assert P.dtype == np.int32 and P.min() >= 0 and P.max() < len(gpr.REF_OPS)
