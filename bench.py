@@ -490,6 +490,8 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=20,
                     help="autotune the wave launch shapes on this many MH steps after warm-up (0: defaults)")
     ap.add_argument("--only-headline", action="store_true", help="time the full form only (profiling runs)")
+    ap.add_argument("--only", choices=["large", "batched", "nist", "ess", "cpu"], default=None,
+                    help="run one sub-record alone and print it (profiling runs)")
     ap.add_argument("--step-split", choices=["large", "full"], default=None,
                     help="only time one block of constant steps and one of structure steps on a workload's "
                          "programs (after warm-up blocks) and print their cost per chain-step")
@@ -508,6 +510,15 @@ def main():
         sys.exit(2)
 
     from gpuropolis_amd import lib as G
+
+    if args.only:
+        run = {"large": lambda: large(G, args),
+               "batched": lambda: batched(G, args.seed, args.batched_blocks, args.mh_steps),
+               "nist": lambda: nist_batch(G, args.seed, args.nist_blocks, args.mh_steps),
+               "ess": lambda: ess_rational(G, args.seed),
+               "cpu": lambda: cpu_baseline(args.seed, args.cpu_scale)}[args.only]
+        print(json.dumps({args.only: run()}), flush=True)
+        return
 
     st, wall, kms, table, nchains, nd = full_form(G, args.seed, args.steps, args.warmup, args.mh_steps,
                                                   tune_steps=args.tune_steps)

@@ -547,6 +547,108 @@ __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
     return ballot(bad) == 0 && const_count<HI>(p) <= GPR_DUAL_NT && dmax <= GPR_DUAL_DEPTH;
 }
 
+// One token (two for a leaf fused with the next +, - or *) of the forward-mode
+// interpreter, tokens of half H of the tape (t < 64 for H = 0): the half is a
+// compile-time constant, so the token-class masks and the tape words are read without
+// per-token selects.
+template <int H>
+__device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, const uint64_t (&cm)[2], int nlo,
+                                           float xv, float *lane, float (&T)[GPR_DUAL_W], float (&U)[GPR_DUAL_W],
+                                           int &sp, int &t) {
+    constexpr int W = GPR_DUAL_W;
+    const int li = t & 63;
+    constexpr int h = H;
+    const uint64_t bit = 1ull << li;
+    if (m.leaf[h] & bit) {
+        // the leaf's value and tangent: x (0), or constant of rank rk (e_rk)
+        const bool isx = (m.x[h] & bit) != 0;
+        const float cv = h ? rl_f(p.c1, li) : rl_f(p.c0, li);
+        const int rk = isx ? -1 : (h ? nlo : 0) + __popcll(cm[h] & (bit - 1));
+        const float b = isx ? xv : cv;
+        if (m.cheap[h] & (bit << 1)) {
+            // leaf followed by +, - or *: T = T op leaf, stack untouched
+            const uint64_t nb = bit << 1;
+            if (m.mul[h] & nb) {
+                const float tv = T[0];
+                T[0] = T[0] * b;
+#pragma unroll
+                for (int j = 1; j < W; j++) T[j] = tmul(T[j], b) + (rk == j - 1 ? tv : 0.0f);
+            } else {
+                const float sgn = (m.sub[h] & nb) ? -1.0f : 1.0f;
+                T[0] = __builtin_fmaf(sgn, b, T[0]);
+#pragma unroll
+                for (int j = 1; j < W; j++) T[j] = rk == j - 1 ? T[j] + sgn : T[j];
+            }
+            t++;
+            return;
+        }
+        ls_store<W>(lane + (sp - 2) * 64 * W, U);
+#pragma unroll
+        for (int j = 0; j < W; j++) U[j] = T[j];
+        sp++;
+        T[0] = b;
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = rk == j - 1 ? 1.0f : 0.0f;
+        return;
+    }
+    if (m.cheap[h] & bit) {
+        // U op T for +, -, *
+        if (m.mul[h] & bit) {
+            const float uv = U[0], tv = T[0];
+            T[0] = U[0] * T[0];
+#pragma unroll
+            for (int j = 1; j < W; j++) T[j] = tmul(U[j], tv) + tmul(T[j], uv);
+        } else {
+            const float sgn = (m.sub[h] & bit) ? -1.0f : 1.0f;
+            T[0] = __builtin_fmaf(sgn, T[0], U[0]);
+#pragma unroll
+            for (int j = 1; j < W; j++) T[j] = __builtin_fmaf(sgn, T[j], U[j]);
+        }
+        ls_load<W>(lane + (sp - 3) * 64 * W, U);
+        sp--;
+        return;
+    }
+    const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
+    if (op == GPR_OP_ONE) {
+        ls_store<W>(lane + (sp - 2) * 64 * W, U);
+#pragma unroll
+        for (int j = 0; j < W; j++) { U[j] = T[j]; T[j] = j == 0 ? 1.0f : 0.0f; }
+        sp++;
+    } else if (op == GPR_OP_DIV) {
+        const float a = U[0], b = T[0];
+        const float q = a / b;
+        const float ib = 1.0f / b, nq = -(a / (b * b));
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(U[j], ib) + tmul(T[j], nq);
+        T[0] = q;
+        ls_load<W>(lane + (sp - 3) * 64 * W, U);
+        sp--;
+    } else if (op == GPR_OP_POW) {
+        const float a = U[0], b = T[0];
+        const float v = powf(a, b);
+        const float pa = b * powf(a, b - 1.0f), pb = v * logf(a);
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(U[j], pa) + tmul(T[j], pb);
+        T[0] = v;
+        ls_load<W>(lane + (sp - 3) * 64 * W, U);
+        sp--;
+    } else if (op == GPR_OP_EXP) {
+        const float v = expf(T[0]);
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(T[j], v);
+        T[0] = v;
+    } else if (op == GPR_OP_LOG) {
+        const float a = T[0];
+        const float ia = 1.0f / a;
+        T[0] = logf(a);
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(T[j], ia);
+    } else {   // GPR_OP_NEG (dual_ok admits no other op)
+#pragma unroll
+        for (int j = 0; j < W; j++) T[j] = -T[j];
+    }
+}
+
 // LL of the wave's program (as wave_loglik<1, HI, true>) and, when the pass completes,
 // gs[j] = sum over points of (-2 r w) df/dc_j for the constant token of rank j (token
 // order; cm = the constant-token masks).  ls: the wave's dual stack region.
@@ -574,99 +676,11 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
 #pragma unroll
         for (int j = 0; j < W; j++) { T[j] = 0.0f; U[j] = 0.0f; }
         int sp = 0;
-        for (int t = 0; t < p.len; t++) {
-            const int li = t & 63;
-            const int h = (HI && t >= 64) ? 1 : 0;
-            const uint64_t bit = 1ull << li;
-            if (m.leaf[h] & bit) {
-                // the leaf's value and tangent: x (0), or constant of rank rk (e_rk)
-                const bool isx = (m.x[h] & bit) != 0;
-                const float cv = h ? rl_f(p.c1, li) : rl_f(p.c0, li);
-                const int rk = isx ? -1 : (h ? nlo : 0) + __popcll(cm[h] & (bit - 1));
-                const float b = isx ? xv : cv;
-                if (m.cheap[h] & (bit << 1)) {
-                    // leaf followed by +, - or *: T = T op leaf, stack untouched
-                    const uint64_t nb = bit << 1;
-                    if (m.mul[h] & nb) {
-                        const float tv = T[0];
-                        T[0] = T[0] * b;
-#pragma unroll
-                        for (int j = 1; j < W; j++) T[j] = tmul(T[j], b) + (rk == j - 1 ? tv : 0.0f);
-                    } else {
-                        const float sgn = (m.sub[h] & nb) ? -1.0f : 1.0f;
-                        T[0] = __builtin_fmaf(sgn, b, T[0]);
-#pragma unroll
-                        for (int j = 1; j < W; j++) T[j] = rk == j - 1 ? T[j] + sgn : T[j];
-                    }
-                    t++;
-                    continue;
-                }
-                ls_store<W>(lane + (sp - 2) * 64 * W, U);
-#pragma unroll
-                for (int j = 0; j < W; j++) U[j] = T[j];
-                sp++;
-                T[0] = b;
-#pragma unroll
-                for (int j = 1; j < W; j++) T[j] = rk == j - 1 ? 1.0f : 0.0f;
-                continue;
-            }
-            if (m.cheap[h] & bit) {
-                // U op T for +, -, *
-                if (m.mul[h] & bit) {
-                    const float uv = U[0], tv = T[0];
-                    T[0] = U[0] * T[0];
-#pragma unroll
-                    for (int j = 1; j < W; j++) T[j] = tmul(U[j], tv) + tmul(T[j], uv);
-                } else {
-                    const float sgn = (m.sub[h] & bit) ? -1.0f : 1.0f;
-                    T[0] = __builtin_fmaf(sgn, T[0], U[0]);
-#pragma unroll
-                    for (int j = 1; j < W; j++) T[j] = __builtin_fmaf(sgn, T[j], U[j]);
-                }
-                ls_load<W>(lane + (sp - 3) * 64 * W, U);
-                sp--;
-                continue;
-            }
-            const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
-            if (op == GPR_OP_ONE) {
-                ls_store<W>(lane + (sp - 2) * 64 * W, U);
-#pragma unroll
-                for (int j = 0; j < W; j++) { U[j] = T[j]; T[j] = j == 0 ? 1.0f : 0.0f; }
-                sp++;
-            } else if (op == GPR_OP_DIV) {
-                const float a = U[0], b = T[0];
-                const float q = a / b;
-                const float ib = 1.0f / b, nq = -(a / (b * b));
-#pragma unroll
-                for (int j = 1; j < W; j++) T[j] = tmul(U[j], ib) + tmul(T[j], nq);
-                T[0] = q;
-                ls_load<W>(lane + (sp - 3) * 64 * W, U);
-                sp--;
-            } else if (op == GPR_OP_POW) {
-                const float a = U[0], b = T[0];
-                const float v = powf(a, b);
-                const float pa = b * powf(a, b - 1.0f), pb = v * logf(a);
-#pragma unroll
-                for (int j = 1; j < W; j++) T[j] = tmul(U[j], pa) + tmul(T[j], pb);
-                T[0] = v;
-                ls_load<W>(lane + (sp - 3) * 64 * W, U);
-                sp--;
-            } else if (op == GPR_OP_EXP) {
-                const float v = expf(T[0]);
-#pragma unroll
-                for (int j = 1; j < W; j++) T[j] = tmul(T[j], v);
-                T[0] = v;
-            } else if (op == GPR_OP_LOG) {
-                const float a = T[0];
-                const float ia = 1.0f / a;
-                T[0] = logf(a);
-#pragma unroll
-                for (int j = 1; j < W; j++) T[j] = tmul(T[j], ia);
-            } else {   // GPR_OP_NEG (dual_ok admits no other op)
-#pragma unroll
-                for (int j = 0; j < W; j++) T[j] = -T[j];
-            }
-        }
+        // tokens of the low half, then (two-register tapes) the high half
+        const int lo_end = HI ? min(p.len, 64) : p.len;
+        for (int t = 0; t < lo_end; t++) dual_token<0>(p, m, cm, nlo, xv, lane, T, U, sp, t);
+        if (HI)
+            for (int t = 64; t < p.len; t++) dual_token<1>(p, m, cm, nlo, xv, lane, T, U, sp, t);
         const int i = ch * 64 + l;
         const bool valid = i < d.n;
         const float r = T[0] - c.y[0];
