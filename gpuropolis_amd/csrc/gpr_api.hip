@@ -540,7 +540,7 @@ static hipError_t launch_bucket(gpr_ctx *c, const GprMHArgs &a, int b, hipStream
     ab.mala_gslab = c->d_gslab ? c->d_gslab + kb.gslab_off : nullptr;
     const int grid = std::max(1, std::min(kb.ntiles, c->cus * kb.wgpc));
     if (c->cfg.semantics == GPR_SEM_REFERENCE) return gpr_launch_mh_ref_wave(ab, kb.kc, grid, s);
-    return gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, c->cfg.proposal == GPR_PROP_MALA, grid, s);
+    return gpr_launch_mh_config_wave(ab, kb.kc, kb.hi, c->cfg.proposal == GPR_PROP_MALA, kb.cls == 0, grid, s);
 }
 
 // All size-class launches of one block.  They touch disjoint chains and have their own

@@ -730,6 +730,119 @@ __device__ __forceinline__ void dual_grad_lanes(const WaveProg &p, const uint64_
     ok = ballot((c0 && !__builtin_isfinite(g0)) || (c1 && !__builtin_isfinite(g1))) == 0;
 }
 
+// ---------------------------------------------------------------- speculative constant steps
+// Datasets of at most 32 points fill a fraction of a wave.  The random-walk kernel then
+// evaluates the proposals of G consecutive constant steps in one interpreter pass, one
+// per lane group (G = 4 groups of 16 lanes for n <= 16, 2 of 32 for n <= 32): lane l
+// takes point l mod (64 / G) of the proposal of step u0 + l / (64 / G).  All of them are
+// proposals from the same current state, which is the state each step sees as long as
+// the steps before it reject; the kernel uses them in step order and drops the rest
+// after an accept.  The dispatch of every token serves the G proposals.
+//
+// Proposal constants come from the draw batch: the constant of rank r of step u is lane
+// ks u + r of cb.  The value of each proposal is bit-identical to its own evaluation by
+// wave_loglik<1, false>: the same operations per point, a one-chunk Kahan sum (the term
+// itself), and a group sum whose FP64 additions are those of wave_sum_d over the lanes
+// it would occupy alone (rows of 16 lanes summed by the same DPP steps; a group of two
+// rows adds them as wave_sum_d adds rows 0 and 1, and adding the zero rows of a lone
+// evaluation changes nothing).  ls: the wave's value-stack region (KC = 1 layout).
+// out[g] receives the LL of group g's proposal, for g < gn (groups g >= gn repeat the
+// proposal of group gn - 1).
+__device__ __forceinline__ void wave_loglik_groups(const WaveProg &p, const GprDataset &d, const float *dx,
+                                                   const float *dy, const float *dw, float *ls, const OpMasks &m,
+                                                   uint64_t cm, float cb, int ks, int u0, int G, int gn,
+                                                   double (&out)[4]) {
+    const int l = lane_id();
+    const int lgs = G == 4 ? 4 : 5;                  // log2 of the group size
+    const int gl = l >> lgs;
+    const int i = l & ((1 << lgs) - 1);
+    const bool valid = i < d.n;
+    const float xv = valid ? dx[i] : 0.0f, yv = valid ? dy[i] : 0.0f, wv = valid ? dw[i] : 0.0f;
+    const int src = ks * (u0 + min(gl, gn - 1));    // lane of cb holding rank 0 of this group's step
+    float *lane = ls + 2 * 64 + l;                   // level -2 of the stack below T, U
+    float T = 0.0f, U = 0.0f;
+    int sp = 0;
+    for (int t = 0; t < p.len; t++) {
+        const uint64_t bit = 1ull << t;
+        if (m.leaf[0] & bit) {
+            const bool isx = (m.x[0] & bit) != 0;
+            float b = xv;
+            if (!isx) b = shfl_all(cb, src + __popcll(cm & (bit - 1)));
+            if (m.cheap[0] & (bit << 1)) {
+                // leaf followed by +, - or *: T = T op leaf, stack untouched
+                const uint64_t nb = bit << 1;
+                const bool mul = (m.mul[0] & nb) != 0;
+                const float sgn = (m.sub[0] & nb) ? -1.0f : 1.0f;
+                const float pm = __builtin_fmaf(sgn, b, T), pr = T * b;
+                T = mul ? pr : pm;
+                t++;
+                continue;
+            }
+            lane[(sp - 2) * 64] = U;
+            U = T;
+            sp++;
+            T = b;
+            continue;
+        }
+        if (m.cheap[0] & bit) {
+            const bool mul = (m.mul[0] & bit) != 0;
+            const float sgn = (m.sub[0] & bit) ? -1.0f : 1.0f;
+            const float pm = __builtin_fmaf(sgn, T, U), pr = U * T;
+            T = mul ? pr : pm;
+            U = lane[(sp - 3) * 64];
+            sp--;
+            continue;
+        }
+        const int op = rl_i(p.o0, t);
+        if (op == GPR_OP_ONE) {
+            lane[(sp - 2) * 64] = U;
+            U = T;
+            sp++;
+            T = 1.0f;
+        } else if (op <= GPR_OP_RPOW) {
+            switch (op) {
+            case GPR_OP_DIV: T = U / T; break;
+            case GPR_OP_RSUB: T = T - U; break;
+            case GPR_OP_RDIV: T = T / U; break;
+            case GPR_OP_POW: T = powf(U, T); break;
+            default: T = powf(T, U); break;
+            }
+            U = lane[(sp - 3) * 64];
+            sp--;
+        } else {
+            switch (op) {
+            case GPR_OP_NEG: T = -T; break;
+            case GPR_OP_EXP: T = expf(T); break;
+            case GPR_OP_LOG: T = logf(T); break;
+            case GPR_OP_INV: T = 1.0f / T; break;
+            case GPR_OP_SQRT: T = sqrtf(T); break;
+            case GPR_OP_SIN: T = sinf(T); break;
+            case GPR_OP_ASIN: T = asinf(T); break;
+            case GPR_OP_COS: T = cosf(T); break;
+            case GPR_OP_ACOS: T = acosf(T); break;
+            case GPR_OP_TAN: T = tanf(T); break;
+            case GPR_OP_ATAN: T = atanf(T); break;
+            case GPR_OP_ABS: T = fabsf(T); break;
+            default: T = __builtin_nanf(""); break;   // constant-operand ops never reach here
+            }
+        }
+    }
+    const float r = T - yv;
+    const float term = valid ? (r * r) * wv : 0.0f;
+    double v = (double)term;
+    v += dpp_d<DPP_QP_1032>(v);
+    v += dpp_d<DPP_QP_2301>(v);
+    v += dpp_d<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_d<DPP_ROW_MIRROR>(v);
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        double s;
+        if (G == 4) s = rl_d(v, 16 * g);
+        else s = g < 2 ? rl_d(v, 32 * g) + rl_d(v, 32 * g + 16) : 0.0;
+        out[g] = !__builtin_isfinite(s) ? -GPR_INF : d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
+    }
+}
+
 // Description-length prior plus Cauchy(0,1) on every constant token, summed in token
 // order in FP64 (the same order as the oracle, so the value is bit-reproducible).
 template <bool HI>

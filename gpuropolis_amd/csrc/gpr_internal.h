@@ -135,7 +135,9 @@ struct GprResampleArgs {
 };
 
 // launchers (gpr_kernels.hip)
-hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, bool mala, int grid, hipStream_t s);
+// small: every dataset of the launch has at most 64 points (the speculative build)
+hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, bool mala, bool small, int grid,
+                                     hipStream_t s);
 hipError_t gpr_launch_mh_config_lane(const GprMHArgs &a, int cap_max, hipStream_t s);
 hipError_t gpr_launch_mh_ref_lane(const GprMHArgs &a, hipStream_t s);
 hipError_t gpr_launch_mh_ref_wave(const GprMHArgs &a, int kc, int grid, hipStream_t s);

@@ -53,7 +53,10 @@ __global__ void k_fill_logint() {
 #ifndef GPR_MALA_MET_WAVES_PER_EU
 #define GPR_MALA_MET_WAVES_PER_EU 4
 #endif
-template <int KC, bool HI, bool MALA, bool MET>
+// SPEC: the launch's datasets have at most 64 points; those of at most 32 evaluate
+// speculative constant steps (wave_loglik_groups).  A separate build, so the kernels of
+// larger datasets keep their registers.
+template <int KC, bool HI, bool MALA, bool MET, bool SPEC = false>
 __global__ void __launch_bounds__(GPR_WG)
 __attribute__((amdgpu_waves_per_eu(MALA ? (MET ? GPR_MALA_MET_WAVES_PER_EU : GPR_MALA_WAVES_PER_EU) : (KC == 1 ? 6 : 1))))
 k_mh_config_wave(GprMHArgs a) {
@@ -70,9 +73,11 @@ k_mh_config_wave(GprMHArgs a) {
     // (the forward-mode interpreter's stack of value + tangents shares it too)
     constexpr int MFLOATS = MROWS * 64 > GPR_DUAL_LDS_FLOATS ? MROWS * 64 : GPR_DUAL_LDS_FLOATS;
     float *ls = gpr_lds_slab + (size_t)wave * (MALA ? MFLOATS : SROWS * 64 * KC);
-    // after the wpg value stacks: each wave's priors of the (up to 16) steps of its draw batch
+    // after the wpg value stacks: each wave's priors of the (up to 16) steps of its draw
+    // batch, then the LLs of its (up to 4) speculatively evaluated steps
     double *bpri = MALA ? nullptr
-                        : (double *)(gpr_lds_slab + (size_t)a.wpg * ((GPR_MAX_DEPTH + 2) * 64 * KC)) + wave * 16;
+                        : (double *)(gpr_lds_slab + (size_t)a.wpg * ((GPR_MAX_DEPTH + 2) * 64 * KC)) + wave * 20;
+    double *sll = MALA ? nullptr : bpri + 16;
     float *lvg = (MALA && a.mala_gslab) ? a.mala_gslab + ((size_t)blockIdx.x * a.wpg + wave) * (size_t)a.dmax_cap * 64
                                         : nullptr;
     const uint32_t step0 = a.clock ? (uint32_t)uni((int)a.clock[0]) : a.step0;
@@ -92,6 +97,12 @@ k_mh_config_wave(GprMHArgs a) {
         const int cap = d.cap;
         const float *dx = a.dx + d.off, *dy = a.dy + d.off, *dw = a.dw + d.off;
         const int nchunks = (d.n + 64 * KC - 1) / (64 * KC);
+        // speculative constant steps of small datasets: G proposals per interpreter pass
+        // (wave_loglik_groups); the window of the last pass starts at step spec_lo and
+        // holds spec_n cached LLs in sll, valid while the state does not change
+        const int G = (SPEC && !MALA && KC == 1) ? (d.n <= 16 ? 4 : (d.n <= 32 ? 2 : 1)) : 1;
+        uint32_t spec_lo = 0;
+        int spec_n = 0;
 
         WaveProg cur;
         cur.len = uni(a.len[chain]);
@@ -307,6 +318,33 @@ k_mh_config_wave(GprMHArgs a) {
                     if (k0) prop.c0 = cv;
                     bprior = true;
                     bu = u;
+                    if (G > 1) {
+                        if (spec_n > 0 && step >= spec_lo && step < spec_lo + (uint32_t)spec_n) {
+                            pll_m = rl_d(sll[step - spec_lo], 0);   // evaluated by the window's pass
+                            have_pll = true;
+                            mala_pts = d.n;
+                        } else {
+                            // the constant steps from this one to the next structure step, the
+                            // end of the block and the end of the draw batch
+                            int win = G;
+                            if (!a.fixed_structure) win = min(win, 1 + (int)((se - srem) % se));
+                            win = min(win, a.steps - i);
+                            win = min(win, (1 << lg) - u);
+                            if (win > 1) {
+                                double o[4];
+                                wave_loglik_groups(cur, d, dx, dy, dw, ls, cmask, mk0, cb, ks, u, G, win, o);
+                                if (l == 0) {
+#pragma unroll
+                                    for (int g = 0; g < 4; g++) sll[g] = o[g];
+                                }
+                                spec_lo = step;
+                                spec_n = win;
+                                pll_m = o[0];
+                                have_pll = true;
+                                mala_pts = d.n;
+                            }
+                        }
+                    }
                 } else {
                 if (BATCH || (step >> 2) != wblk) {
                     wblk = step >> 2;
@@ -493,6 +531,7 @@ k_mh_config_wave(GprMHArgs a) {
                 prior = pprior;
                 ll = pll;
                 changed = true;
+                spec_n = 0;
                 if (structural) {
                     c_sa++;
                     grad_ok = false;
@@ -550,25 +589,29 @@ k_mh_config_wave(GprMHArgs a) {
     if (l < GPR_NCNT && a.counters && cnt) atomicAdd(&a.counters[l], cnt);
 }
 
-template <int KC, bool HI, bool MALA, bool MET>
+template <int KC, bool HI, bool MALA, bool MET, bool SPEC = false>
 static hipError_t launch_mh_wave(const GprMHArgs &a, int grid, hipStream_t s) {
     const size_t mfloats = std::max((size_t)std::max(GPR_MALA_LDS_ROWS, GPR_MAX_DEPTH + 2) * 64,
                                     (size_t)GPR_DUAL_LDS_FLOATS);
     const size_t sh = MALA ? sizeof(float) * mfloats * a.wpg
-                           : (sizeof(float) * (GPR_MAX_DEPTH + 2) * 64 * KC + sizeof(double) * 16) * (size_t)a.wpg;
+                           : (sizeof(float) * (GPR_MAX_DEPTH + 2) * 64 * KC + sizeof(double) * 20) * (size_t)a.wpg;
     if (sh > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA, MET>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_wave<KC, HI, MALA, MET, SPEC>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_mh_config_wave<KC, HI, MALA, MET>), dim3(grid), dim3(64 * a.wpg), sh, s, a);
+    hipLaunchKernelGGL((k_mh_config_wave<KC, HI, MALA, MET, SPEC>), dim3(grid), dim3(64 * a.wpg), sh, s, a);
     return hipGetLastError();
 }
 
 template <bool MET>
-static hipError_t launch_mh_wave_met(const GprMHArgs &a, int kc, bool hi, bool mala, int grid, hipStream_t s) {
+static hipError_t launch_mh_wave_met(const GprMHArgs &a, int kc, bool hi, bool mala, bool small, int grid,
+                                     hipStream_t s) {
     // MALA evaluates one point per lane whatever the shape's kc
     if (mala) return hi ? launch_mh_wave<1, true, true, MET>(a, grid, s) : launch_mh_wave<1, false, true, MET>(a, grid, s);
+    if (kc == 1 && small)
+        return hi ? launch_mh_wave<1, true, false, MET, true>(a, grid, s)
+                  : launch_mh_wave<1, false, false, MET, true>(a, grid, s);
 #define GPR_L(K, H) return launch_mh_wave<K, H, false, MET>(a, grid, s)
     if (kc == 1) { if (hi) GPR_L(1, true); else GPR_L(1, false); }
     else if (kc == 2) { if (hi) GPR_L(2, true); else GPR_L(2, false); }
@@ -576,9 +619,10 @@ static hipError_t launch_mh_wave_met(const GprMHArgs &a, int kc, bool hi, bool m
 #undef GPR_L
 }
 
-hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, bool mala, int grid, hipStream_t s) {
-    return a.metric ? launch_mh_wave_met<true>(a, kc, hi, mala, grid, s)
-                    : launch_mh_wave_met<false>(a, kc, hi, mala, grid, s);
+hipError_t gpr_launch_mh_config_wave(const GprMHArgs &a, int kc, bool hi, bool mala, bool small, int grid,
+                                     hipStream_t s) {
+    return a.metric ? launch_mh_wave_met<true>(a, kc, hi, mala, small, grid, s)
+                    : launch_mh_wave_met<false>(a, kc, hi, mala, small, grid, s);
 }
 
 // ---------------------------------------------------------------- config MH, lane schedule
