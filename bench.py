@@ -44,13 +44,20 @@ HBM_PEAK = 8.0e12
 # HBM bytes per MH launch (FETCH_SIZE + WRITE_SIZE of rocprofv3 --pmc, scaled by the
 # same counters on a plain copy of known size), keyed by (workload, chains, MH steps per
 # block, seed); the profiles/ summaries hold the runs.  Valid for the kernel build they
-# were measured on.
-MEASURED_TRAFFIC = {}
+# were measured on.  For the batched workload a "launch" is one block's six concurrent
+# size-class launches.
+MEASURED_TRAFFIC = {
+    ("large", 131072, 1000, 0): 269_317_600,
+    ("batched", 262144, 1000, 0): 978_668_064,
+}
 # VALU and SALU instructions per chain-step of the same launches (rocprofv3 --pmc
 # SQ_INSTS_VALU, SQ_INSTS_SALU over chains x steps).  With the live launch time they give
 # the share of issue slots each unit is busy: one wave64 instruction per SIMD every 4
 # cycles, 1024 SIMDs at 2.4 GHz.
-MEASURED_ISSUE = {}
+MEASURED_ISSUE = {
+    ("large", 131072, 1000, 0): (297.86, 299.57),
+    ("batched", 262144, 1000, 0): (329.50, 301.71),
+}
 
 METRIC = "MH steps/sec + node×datapoint evals/sec (% VALU peak); ESS/sec MALA vs RW"
 
@@ -338,11 +345,16 @@ def batched(G, seed=0, blocks=3, mh_steps=1000):
     wall = time.perf_counter() - t0
     st = ctx.stats()
     ctx.close()
+    nch = 4096 * len(dsets)
+    # the block's six size-class launches run concurrently: the roofline takes the graph
+    # time of the block (HIP events around the replay on the context stream) as the launch
     return {"workload": "batched: 64 datasets, n log-uniform [6, 250], 4096 chains each, caps 32/64/128, "
                         f"blocks of {mh_steps} MH steps + MAP-100 resample as hipGraph replays",
             "blocks": blocks, "mh_steps_per_s": st.steps / wall, "graph_ms_per_block": ms / blocks,
             "alg_tflops": st.alg_flops / (ms / 1e3) / 1e12,
-            "frac_valu_peak": st.alg_flops / (ms / 1e3) / VALU_LANE_OP_PEAK, "launch_shape": table}
+            "frac_valu_peak": st.alg_flops / (ms / 1e3) / VALU_LANE_OP_PEAK, "launch_shape": table,
+            "roofline": roofline("k_mh_config_wave (six size-class launches per block)", st.alg_flops / blocks,
+                                 ms / 1e3 / blocks, ("batched", nch, mh_steps, seed))}
 
 
 NIST_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "nist")
@@ -459,7 +471,7 @@ def large(G, args):
                        f"blocks of {args.mh_steps} MH steps + MAP-100 resample",
            "blocks": args.large_blocks, "mh_steps_per_s": st.steps / wall, "node_evals_per_s": st.node_evals / wall,
            "launch_shape": table,
-           "roofline": roofline("k_mh_config_wave<1,false,false,false>", st.alg_flops / args.large_blocks, launch_s,
+           "roofline": roofline("k_mh_config_wave<1,false,false,false,false>", st.alg_flops / args.large_blocks, launch_s,
                                 ("large", args.chains, args.mh_steps, args.seed))}
     if args.lane_blocks > 0:
         o_st, o_wall, o_ms, _ = time_blocks(G, ds, args, G.SCHED_LANE, args.lane_blocks, 1)
@@ -545,7 +557,7 @@ def main():
         "launch_shape": table,
         "acceptance": {"structure": st.struct_accepted / max(1, st.struct_proposed),
                        "constants": st.const_accepted / max(1, st.const_proposed)},
-        "roofline": roofline("k_mh_config_wave<1,true,true,false>", st.alg_flops / args.steps, launch_s,
+        "roofline": roofline("k_mh_config_wave<1,true,true,false,false>", st.alg_flops / args.steps, launch_s,
                              ("full", nchains, args.mh_steps, args.seed)),
     }
     if not args.only_headline:

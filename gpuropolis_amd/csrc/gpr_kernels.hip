@@ -40,15 +40,17 @@ __global__ void k_fill_logint() {
 // The plain one-point-per-lane kernel is latency bound: six waves per SIMD (80 VGPRs) run
 // it faster than the four that fit unspilled, although the compiler then spills per-tile
 // values to scratch at every tile start and reloads some of them in the step loop.
-// The MALA kernel likewise: unconstrained it takes about 160 VGPRs (three waves per SIMD);
-// held to five waves (96 VGPRs, per-tile values spilled) it runs the full form's
-// constant steps 15 % and its structure steps 21 % faster.  The metric build runs at four
-// waves (128 VGPRs): unconstrained it takes 138 and three waves, 20 % slower.
+// The MALA kernel: unconstrained it takes about 160 VGPRs (three waves per SIMD), 15 %
+// slower on the full form than at four waves (128 VGPRs, per-tile values spilled at tile
+// start only).  Five waves (96 VGPRs) ran 2 % faster still, but spilled inside the step
+// loop: 18 scratch stores per MH step, 2.4 TB written per 1000-step launch of the full
+// form against 1.1 GB at four waves.  The metric build runs at four waves as well:
+// unconstrained it takes 138 VGPRs and three waves, 20 % slower.
 // MET: some dataset of the launch has a proposal metric.  The metric path holds many
 // wave-uniform values (matrix rows, pointers), so launches without a metric use a
 // build without it, whose scalar registers then fit with far fewer spills.
 #ifndef GPR_MALA_WAVES_PER_EU
-#define GPR_MALA_WAVES_PER_EU 5
+#define GPR_MALA_WAVES_PER_EU 4
 #endif
 #ifndef GPR_MALA_MET_WAVES_PER_EU
 #define GPR_MALA_MET_WAVES_PER_EU 4
