@@ -519,7 +519,8 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // The MALA proposals of short programs with few constants get their gradient in the same
 // interpreter pass as their likelihood: every stack entry carries the value and its
 // tangents with respect to the program's constant tokens (constant of rank j: tangent j),
-// W = 1 + GPR_DUAL_NT floats, pushed to the wave's LDS stack as one 16-byte store.  The
+// W = 1 + NT floats, pushed to the wave's LDS stack as one 16-byte store (NT =
+// GPR_DUAL_NT) or, for programs of a single constant, one 8-byte store (NT = 1).  The
 // values take the same operations on the same operands as wave_eval_ls, so the LL is
 // bit-identical to it (and early rejection stops a pass at the same chunk); each lane
 // adds the per-point gradient (-2 r w) f' of its points, and one wave sum per constant
@@ -550,12 +551,12 @@ __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
 // One token (two for a leaf fused with the next +, - or *) of the forward-mode
 // interpreter, tokens of half H of the tape (t < 64 for H = 0): the half is a
 // compile-time constant, so the token-class masks and the tape words are read without
-// per-token selects.
-template <int H>
+// per-token selects.  NT tangents (1 for programs of one constant, else GPR_DUAL_NT).
+template <int H, int NT>
 __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, const uint64_t (&cm)[2], int nlo,
-                                           float xv, float *lane, float (&T)[GPR_DUAL_W], float (&U)[GPR_DUAL_W],
+                                           float xv, float *lane, float (&T)[1 + NT], float (&U)[1 + NT],
                                            int &sp, int &t) {
-    constexpr int W = GPR_DUAL_W;
+    constexpr int W = 1 + NT;
     const int li = t & 63;
     constexpr int h = H;
     const uint64_t bit = 1ull << li;
@@ -626,7 +627,13 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
     } else if (op == GPR_OP_POW) {
         const float a = U[0], b = T[0];
         const float v = powf(a, b);
-        const float pa = b * powf(a, b - 1.0f), pb = v * logf(a);
+        // d/da a^b = b a^(b-1) = b (a^b / a) where a is finite and non-zero and a^b finite
+        float pa;
+        if (ballot(a == 0.0f || !__builtin_isfinite(a) || !__builtin_isfinite(v)))
+            pa = b * powf(a, b - 1.0f);
+        else
+            pa = b * (v / a);
+        const float pb = v * logf(a);
 #pragma unroll
         for (int j = 1; j < W; j++) T[j] = tmul(U[j], pa) + tmul(T[j], pb);
         T[0] = v;
@@ -652,18 +659,20 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 // LL of the wave's program (as wave_loglik<1, HI, true>) and, when the pass completes,
 // gs[j] = sum over points of (-2 r w) df/dc_j for the constant token of rank j (token
 // order; cm = the constant-token masks).  ls: the wave's dual stack region.
-template <bool HI>
+// NT: tangents carried, at least the program's constant count (1 or GPR_DUAL_NT).
+template <bool HI, int NT>
 __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
                                    const float *dw, const Chunk<1> &c0, int nchunks, float *ls, const OpMasks &m,
                                    const uint64_t (&cm)[2], double reject_below, int *points,
                                    float (&gs)[GPR_DUAL_NT]) {
-    constexpr int W = GPR_DUAL_W;
+    static_assert(NT == 1 || NT == GPR_DUAL_NT, "tangent count");
+    constexpr int W = 1 + NT;
     const int l = lane_id();
     float *lane = ls + 2 * 64 * W + l * W;   // level -2 of the stack below T, U
     float acc = 0.0f, comp = 0.0f;
-    float ga[GPR_DUAL_NT];
+    float ga[NT];
 #pragma unroll
-    for (int j = 0; j < GPR_DUAL_NT; j++) ga[j] = 0.0f;
+    for (int j = 0; j < NT; j++) ga[j] = 0.0f;
     const bool folded = d.wconst > 0.0;
     const float wf = (float)d.wconst;
     const int nlo = __popcll(cm[0]);
@@ -678,9 +687,9 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
         int sp = 0;
         // tokens of the low half, then (two-register tapes) the high half
         const int lo_end = HI ? min(p.len, 64) : p.len;
-        for (int t = 0; t < lo_end; t++) dual_token<0>(p, m, cm, nlo, xv, lane, T, U, sp, t);
+        for (int t = 0; t < lo_end; t++) dual_token<0, NT>(p, m, cm, nlo, xv, lane, T, U, sp, t);
         if (HI)
-            for (int t = 64; t < p.len; t++) dual_token<1>(p, m, cm, nlo, xv, lane, T, U, sp, t);
+            for (int t = 64; t < p.len; t++) dual_token<1, NT>(p, m, cm, nlo, xv, lane, T, U, sp, t);
         const int i = ch * 64 + l;
         const bool valid = i < d.n;
         const float r = T[0] - c.y[0];
@@ -691,7 +700,7 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
         acc = tv;
         const float G = (-2.0f * r) * (folded ? wf : c.w[0]);
 #pragma unroll
-        for (int j = 0; j < GPR_DUAL_NT; j++) ga[j] = valid ? __builtin_fmaf(G, T[j + 1], ga[j]) : ga[j];
+        for (int j = 0; j < NT; j++) ga[j] = valid ? __builtin_fmaf(G, T[j + 1], ga[j]) : ga[j];
         if (ch + 1 < nchunks && reject_below > -GPR_INF) {
             const double lb = (double)wave_sum_f(acc - comp) * (1.0 - 1e-4);
             const double ub = d.lconst - (d.wconst > 0.0 ? d.wconst * lb : lb);
@@ -702,7 +711,7 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
         }
     }
 #pragma unroll
-    for (int j = 0; j < GPR_DUAL_NT; j++) gs[j] = j < nlo + (HI ? __popcll(cm[1]) : 0) ? wave_sum_f(ga[j]) : 0.0f;
+    for (int j = 0; j < GPR_DUAL_NT; j++) gs[j] = (j < NT && j < nlo + (HI ? __popcll(cm[1]) : 0)) ? wave_sum_f(ga[j < NT ? j : 0]) : 0.0f;
     const double s = wave_sum_d((double)acc - (double)comp);
     if (!__builtin_isfinite(s)) return -GPR_INF;
     return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
@@ -1246,23 +1255,30 @@ __device__ __forceinline__ double wave_loglik_grad(const WaveProg &p, const Grad
     return d.lconst - (folded ? d.wconst * s : s);
 }
 
-// FP64 sum of v over the lanes where m holds, in token order (lanes of the low half,
-// then the high half), as a serial loop over tokens would add them.
+// FP64 sum of the Gaussian log-density terms -f^2 / s2 over the lanes where m holds, in
+// token order (lanes of the low half, then the high half), as a serial loop over tokens
+// would add them.  A half without such lanes computes no terms.
 template <bool HI>
-__device__ __forceinline__ double wave_ordered_sum_d(double v0, bool m0, double v1, bool m1) {
+__device__ __forceinline__ double wave_gauss_sum_d(double f0, bool m0, double f1, bool m1, double s2) {
     double s = 0.0;
     uint64_t b0 = ballot(m0);
-    while (b0) {
-        const int b = __builtin_ctzll(b0);
-        b0 &= b0 - 1;
-        s += rl_d(v0, b);
+    if (b0) {
+        const double v0 = -f0 * f0 / s2;
+        while (b0) {
+            const int b = __builtin_ctzll(b0);
+            b0 &= b0 - 1;
+            s += rl_d(v0, b);
+        }
     }
     if (HI) {
         uint64_t b1 = ballot(m1);
-        while (b1) {
-            const int b = __builtin_ctzll(b1);
-            b1 &= b1 - 1;
-            s += rl_d(v1, b);
+        if (b1) {
+            const double v1 = -f1 * f1 / s2;
+            while (b1) {
+                const int b = __builtin_ctzll(b1);
+                b1 &= b1 - 1;
+                s += rl_d(v1, b);
+            }
         }
     }
     return s;

@@ -13,8 +13,10 @@
 //         stack in VGPRs indexed by the uniform stack pointer.  Per-point terms are
 //         Kahan-summed per lane and tree-reduced across the wave.
 //   lane  one lane per chain, as the reference.  Every lane interprets its own tape;
-//         the tape and the value stack live in LDS ([slot][lane], conflict-free), data
-//         points are broadcast from LDS.
+//         the proposal tape and the value stack live in LDS ([slot][lane],
+//         conflict-free).  When the workgroup's chains share one dataset its points are
+//         staged through LDS 64 at a time and read as broadcasts; the evaluation stops
+//         early, per lane, once a proposal's partial sum proves its rejection.
 #include "gpr_device.h"
 
 // ---------------------------------------------------------------- config MH, wave schedule
@@ -167,7 +169,8 @@ k_mh_config_wave(GprMHArgs a) {
         // random-walk moves can be batched (see BATCH below)
         // MALA: whether the current program's gradient comes from the forward-mode
         // interpreter (dual_ok), with its constant-token masks
-        bool cur_dual = false;
+        // (cur_nt1: a single constant token, whose pass carries one tangent)
+        bool cur_dual = false, cur_nt1 = false;
         uint64_t ccm[2] = {0, 0};
         if (MALA) {
             int D0, D1, dm;
@@ -175,6 +178,7 @@ k_mh_config_wave(GprMHArgs a) {
             cur_dual = !MET && mk == 0 && dual_ok<HI>(cur, dm);
             ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
             ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
+            cur_nt1 = __popcll(ccm[0]) + __popcll(ccm[1]) == 1;
         }
         int cur_opq, cur_nc;
         prog_counts<HI>(cur, cur_opq, cur_nc);
@@ -388,8 +392,12 @@ k_mh_config_wave(GprMHArgs a) {
                         if (cur_dual) {
                             float gs[GPR_DUAL_NT];
                             int np_;
-                            const double cl = wave_loglik_dual<HI>(cur, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks,
-                                                                   ls, cmask, ccm, -GPR_INF, &np_, gs);
+                            const double cl =
+                                cur_nt1 ? wave_loglik_dual<HI, 1>(cur, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks,
+                                                                  ls, cmask, ccm, -GPR_INF, &np_, gs)
+                                        : wave_loglik_dual<HI, GPR_DUAL_NT>(cur, d, dx, dy, dw,
+                                                                            *(const Chunk<1> *)&c0, nchunks, ls, cmask,
+                                                                            ccm, -GPR_INF, &np_, gs);
                             dual_grad_lanes<HI>(cur, ccm, gs, h0, h1, gok);
                             gok = gok && __builtin_isfinite(cl);
                         } else {
@@ -408,7 +416,7 @@ k_mh_config_wave(GprMHArgs a) {
                     if (i0) prop.c0 = gpr_axpy(e, z0, m0);
                     if (HI && i1) prop.c1 = gpr_axpy(e, z1, m1);
                     const double f0 = (double)prop.c0 - (double)m0, f1 = (double)prop.c1 - (double)m1;
-                    double fq = wave_ordered_sum_d<HI>(-f0 * f0 / inv2e2, i0, -f1 * f1 / inv2e2, i1);
+                    double fq = wave_gauss_sum_d<HI>(f0, i0, f1, i1, inv2e2);
                     float cn = 0.0f;
                     if (mk) {
                         // c' = c + h M g + e L z over the metric rows
@@ -425,56 +433,64 @@ k_mh_config_wave(GprMHArgs a) {
                     // The reverse proposal's log density bq is <= 0, so a = pprior + pll + bq -
                     // fq - post is at most pprior + pll - fq - post: a proposal whose LL lies
                     // below post - pprior + fq - 18 gives a < -18 whatever its gradient and is
-                    // rejected without a draw (cfg_accept).  The interpreter evaluates the LL
-                    // with that bound and stops such proposals early; only the others (and
-                    // every proposal from an invalid current state, which always moves) get
-                    // the adjoint sweeps.  Their LL from the sweeps equals the interpreter's
-                    // bit for bit (same operations, same per-lane order).
+                    // rejected without a draw (cfg_accept).  The pass that first evaluates the
+                    // LL checks that bound after each 64-point chunk and stops such proposals
+                    // early (not those from an invalid current state, which always move).
+                    // Programs that qualify get the LL and the gradient from one forward-mode
+                    // pass of the interpreter.  The others are pre-checked by the plain
+                    // interpreter when the data spans several chunks (one chunk cannot stop
+                    // early), and only the survivors get the adjoint sweeps, whose LL equals
+                    // the interpreter's bit for bit (same operations, same per-lane order).
+                    // (A chunk of both sweeps costs more than an interpreter chunk: on the full
+                    // form, sweeps that check the bound themselves instead ran 9 % slower, and
+                    // a pre-check of the first chunk only, before the sweeps or the
+                    // forward-mode pass, 7 % slower.)
                     pre_prior = wave_prior<HI>(prop, a.lnsym);
                     have_prior = true;
                     double rb = -GPR_INF;
                     if (__builtin_isfinite(post) && __builtin_isfinite(pre_prior) && __builtin_isfinite(fq))
                         rb = post - pre_prior + fq - 18.0 -
                              1e-12 * (__builtin_fabs(post) + __builtin_fabs(pre_prior) + __builtin_fabs(fq));
-                    // programs that qualify get the LL and the gradient from one forward-mode
-                    // pass of the interpreter (bound included)
-                    // The others are pre-checked by the plain interpreter when the data
-                    // spans several chunks (one chunk cannot stop early), then swept.
                     float dgs[GPR_DUAL_NT];
+                    double h0 = 0.0, h1 = 0.0;
+                    bool pok = false;
                     const bool precheck = !MET && !cur_dual && nchunks > 1;
-                    if (cur_dual)
-                        pll_m = wave_loglik_dual<HI>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls, cmask,
-                                                     ccm, rb, &mala_pts, dgs);
+                    if (cur_dual && cur_nt1)
+                        pll_m = wave_loglik_dual<HI, 1>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls,
+                                                        cmask, ccm, rb, &mala_pts, dgs);
+                    else if (cur_dual)
+                        pll_m = wave_loglik_dual<HI, GPR_DUAL_NT>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks,
+                                                                  ls, cmask, ccm, rb, &mala_pts, dgs);
                     else if (precheck && HI && prop.len <= 64)
                         pll_m = wave_loglik<KC, false, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &cmask, rb,
                                                              &mala_pts);
                     else if (precheck)
                         pll_m = wave_loglik<KC, HI, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &cmask, rb,
                                                           &mala_pts);
+                    bool rejected = (cur_dual || precheck) && !__builtin_isfinite(pll_m) && __builtin_isfinite(post);
+                    if (!cur_dual && !rejected) {
+                        const double gl = wave_loglik_grad<HI>(prop, gplan, d, dx, dy, dw,
+                                                               prop.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1, pok);
+                        if (!precheck) {
+                            pll_m = gl;
+                            mala_pts = d.n;
+                            rejected = !__builtin_isfinite(pll_m) && __builtin_isfinite(post);
+                        }
+                    }
                     have_pll = true;
-                    if ((cur_dual || precheck) && !__builtin_isfinite(pll_m) && __builtin_isfinite(post)) {
+                    if (rejected) {
                         extra = -GPR_INF;      // rejected: invalid, or below the bound
                     } else {
-                        double h0, h1;
-                        bool pok;
                         if (cur_dual) {
                             dual_grad_lanes<HI>(prop, ccm, dgs, h0, h1, pok);
                             pok = pok && __builtin_isfinite(pll_m);
-                        } else {
-                            const double gl = wave_loglik_grad<HI>(prop, gplan, d, dx, dy, dw,
-                                                                   prop.len <= GPR_MALA_LDS_ROWS ? ls : lvg, h0, h1,
-                                                                   pok);
-                            if (!precheck) {   // the sweeps' LL is the interpreter's, bit for bit
-                                pll_m = gl;
-                                mala_pts = d.n;
-                            }
                         }
-                        c_fl += (unsigned long long)(cur_opq - 2 + cur_nc) * (unsigned long long)d.n;
+                        c_fl += (unsigned long long)(cur_opq - 2 + cur_nc) * (unsigned long long)mala_pts;
                         pg0 = pok ? (float)h0 : 0.0f;
                         pg1 = pok ? (float)h1 : 0.0f;
                         const float b0 = gpr_axpy(h, pg0, prop.c0), b1 = gpr_axpy(h, pg1, prop.c1);
                         const double r0 = (double)cur.c0 - (double)b0, r1 = (double)cur.c1 - (double)b1;
-                        double bq = wave_ordered_sum_d<HI>(-r0 * r0 / inv2e2, i0, -r1 * r1 / inv2e2, i1);
+                        double bq = wave_gauss_sum_d<HI>(r0, i0, r1, i1, inv2e2);
                         if (mk) {
                             pmg = metric_Mv(mM, mk, gather_f(pg0, pg1, mrow, HI));
                             const float mb = gpr_axpy(h, pmg, cn);
@@ -549,6 +565,7 @@ k_mh_config_wave(GprMHArgs a) {
                         cur_dual = !MET && mk == 0 && dual_ok<HI>(cur, dm);
                         ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
                         ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
+                        cur_nt1 = __popcll(ccm[0]) + __popcll(ccm[1]) == 1;
                     }
                     cur_b16 = cur_bok && const_count<HI>(cur) <= 4;
                 } else {
@@ -655,34 +672,76 @@ __device__ __forceinline__ float lane_op(int o, float a, float b, float c) {
 }
 
 // Sampler log-likelihood of one lane's tape (reference lnormalpdf numerics, as the
-// wave kernel: FP32 residual and square, compensated sum, FP64 weight).
+// wave kernel: FP32 residual and square, compensated sum, FP64 weight), over chunks of
+// 64 points.  stage: LDS [3][64] for x, y, w, used when every lane of the (one-wave)
+// workgroup evaluates the same dataset (the wave stages each chunk once and the lanes
+// read it as broadcasts); null: each lane reads its points from global memory.
+// reject_below: the caller rejects the proposal whenever the LL lies below it.  After
+// each chunk but the last the partial sum bounds the LL from above (every term is >= 0;
+// a relative margin of 1e-4 covers the FP32 sum's rounding, as in the wave kernel), and a
+// lane whose bound is below reject_below stops and returns -inf: the proposal is
+// rejected either way.  *points receives the number of points evaluated.  With stage set
+// every lane of the wave must make the call (active false: the lane has no proposal to
+// evaluate this step).
 __device__ double lane_loglik(LaneTape &tp, int len, float *stk, const GprDataset &d, const float *dx,
-                              const float *dy, const float *dw) {
+                              const float *dy, const float *dw, float *stage, double reject_below, int *points,
+                              bool active = true) {
     float acc = 0.0f, comp = 0.0f;
-    for (int i = 0; i < d.n; i++) {
-        const float x = dx[i];
-        float T = 0.0f;
-        int sp = 0;
-        for (int t = 0; t < len; t++) {
-            const int o = tp.op(t);
-            if (o <= GPR_OP_ONE) {
-                stk[min(sp, GPR_MAX_DEPTH - 1) * GPR_CL_WG] = T;
-                sp = min(sp + 1, GPR_MAX_DEPTH);
-                T = o == GPR_OP_X ? x : (o == GPR_OP_CONST ? tp.cv(t) : 1.0f);
-            } else if (o <= GPR_OP_RPOW) {
-                sp = max(sp - 1, 0);
-                T = lane_op(o, stk[min(sp, GPR_MAX_DEPTH - 1) * GPR_CL_WG], T, 0.0f);
-            } else {
-                T = lane_op(o, T, 0.0f, tp.cv(t));
+    bool live = active;   // an inactive lane only helps stage the chunks
+    *points = active ? d.n : 0;
+    for (int base = 0; base < d.n; base += 64) {
+        const int m = min(64, d.n - base);
+        if (stage) {
+            // the whole wave stages the chunk (a wave-uniform loop: every lane has this dataset)
+            const int lane = (int)threadIdx.x;
+            stage[lane] = lane < m ? dx[base + lane] : 0.0f;
+            stage[64 + lane] = lane < m ? dy[base + lane] : 0.0f;
+            stage[128 + lane] = lane < m ? dw[base + lane] : 0.0f;
+            __syncthreads();
+        }
+        if (live) {
+            for (int k = 0; k < m; k++) {
+                const int i = base + k;
+                const float x = stage ? stage[k] : dx[i];
+                float T = 0.0f;
+                int sp = 0;
+                for (int t = 0; t < len; t++) {
+                    const int o = tp.op(t);
+                    if (o <= GPR_OP_ONE) {
+                        stk[min(sp, GPR_MAX_DEPTH - 1) * GPR_CL_WG] = T;
+                        sp = min(sp + 1, GPR_MAX_DEPTH);
+                        T = o == GPR_OP_X ? x : (o == GPR_OP_CONST ? tp.cv(t) : 1.0f);
+                    } else if (o <= GPR_OP_RPOW) {
+                        sp = max(sp - 1, 0);
+                        T = lane_op(o, stk[min(sp, GPR_MAX_DEPTH - 1) * GPR_CL_WG], T, 0.0f);
+                    } else {
+                        T = lane_op(o, T, 0.0f, tp.cv(t));
+                    }
+                }
+                const float r = T - (stage ? stage[64 + k] : dy[i]);
+                const float term = (r * r) * (stage ? stage[128 + k] : dw[i]);
+                const float yv = term - comp;
+                const float tv = acc + yv;
+                comp = (tv - acc) - yv;
+                acc = tv;
+            }
+            if (base + 64 < d.n && reject_below > -GPR_INF) {
+                const double lb = (double)(acc - comp) * (1.0 - 1e-4);
+                const double ub = d.lconst - (d.wconst > 0.0 ? d.wconst * lb : lb);
+                if (!(ub >= reject_below)) {
+                    live = false;
+                    *points = base + 64;
+                }
             }
         }
-        const float r = T - dy[i];
-        const float term = (r * r) * dw[i];
-        const float yv = term - comp;
-        const float tv = acc + yv;
-        comp = (tv - acc) - yv;
-        acc = tv;
+        if (stage) {
+            __syncthreads();            // the chunk is read before the next one is staged
+            if (!__any(live)) break;     // wave-uniform
+        } else if (!live) {
+            break;
+        }
     }
+    if (!live) return -GPR_INF;
     const double s = (double)acc - (double)comp;
     if (!__builtin_isfinite(s)) return -GPR_INF;
     return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
@@ -708,10 +767,14 @@ __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
     const int capmax = a.dmax_cap;
     float *s_stack = (float *)smem;                                  // [16][WG]
     float *s_c = s_stack + GPR_MAX_DEPTH * GPR_CL_WG;                // [cap][WG]
-    uint8_t *s_o = (uint8_t *)(s_c + capmax * GPR_CL_WG);            // [cap][WG]
+    float *s_stage = s_c + capmax * GPR_CL_WG;                       // [3][64]: x, y, w of a chunk
+    uint8_t *s_o = (uint8_t *)(s_stage + 3 * 64);                    // [cap][WG]
+    const bool full = (int)(blockIdx.x + 1) * GPR_CL_WG <= a.nchains;
     if (chain >= a.nchains) return;
     int di = 0;
     while (a.ds[di].chain0 + a.ds[di].nchains <= chain) di++;
+    // chains of one dataset in every lane: its points are staged through LDS
+    float *stage = (full && __all(di == __shfl(di, 0))) ? s_stage : nullptr;
     const GprDataset d = a.ds[di];
     const float *dx = a.dx + d.off, *dy = a.dy + d.off, *dw = a.dw + d.off;
     const int N = a.nchains, cap = d.cap;
@@ -720,16 +783,22 @@ __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
     uint8_t *gops = a.ops + chain;
     float *gcs = a.cs + chain;
     int len = a.len[chain];
-    // proposal = current; evaluate the current state
+    // the LDS tape holds the current program at the start of every step; a rejected
+    // proposal restores only what it changed
     for (int t = 0; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
     double prior = lane_prior(tp, len, a.lnsym);
-    double ll = lane_loglik(tp, len, stk, d, dx, dy, dw);
+    int pts0;
+    double ll = lane_loglik(tp, len, stk, d, dx, dy, dw, stage, -GPR_INF, &pts0);
     unsigned long long n_sprop = 0, n_sacc = 0, n_cprop = 0, n_cacc = 0, n_evals = 0, n_flops = 0;
     for (int i = 0; i < a.steps; i++) {
         const uint32_t step = step0 + (uint32_t)i;
         const double post = prior + ll;
         double extra = 0.0;
         int plen = len;
+        // lowest token the proposal rewrote (a structure move), or -1: constants only
+        int dirty = -1;
+        bool skip = false;   // no proposal this step (every lane still joins the evaluation)
+        int s0 = 0;
         const bool structural = !a.fixed_structure && (step % (uint32_t)a.struct_every) == 0;
         if (structural) {
             n_sprop++;
@@ -739,14 +808,23 @@ __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
             const uint32_t klo = (uint32_t)a.seed, khi = (uint32_t)(a.seed >> 32);
             const uint32_t r = gpr_philox((uint32_t)chain, step, sc2, 0u, klo, khi).z;
             const int j = __builtin_isfinite(post) ? (int)(((uint64_t)r * (uint64_t)len) >> 32) : len - 1;
-            int need = 1, s0 = j;
+            int need = 1;
+            s0 = j;
             while (true) {
-                need += gpr_arity(gops[(size_t)s0 * N]) - 1;
+                need += gpr_arity(tp.op(s0)) - 1;
                 if (need == 0) break;
                 s0--;
             }
             const int oldsize = j - s0 + 1;
             const int maxnew = cap - (len - oldsize);
+            // the old subtree's generation terms, before the tape changes
+            double lq_old = 0.0;
+            for (int t = s0; t <= j; t++) {
+                const int o = tp.op(t);
+                double term = gpr_gen_logp(o);
+                if (o == GPR_OP_CONST) term += gpr_lphi(tp.cv(t));
+                lq_old += term;
+            }
             // generate in right-child-first prefix order into [cap-m, cap): that run is the postfix
             int m = 0;
             need = 1;
@@ -765,60 +843,77 @@ __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
                 m++;
                 need += gpr_arity(sy) - 1;
             }
+            dirty = min(s0, cap - m);
             if (fail) {
-                for (int t = 0; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
-                continue;
+                skip = true;
+            } else {
+                plen = len - oldsize + m;
+                for (int t = 0; t < m; t++) { tp.op(s0 + t) = tp.op(cap - m + t); tp.cv(s0 + t) = tp.cv(cap - m + t); }
+                // the tail after the old subtree, from the current program in global memory
+                // (its LDS copy may be overwritten by the shift)
+                for (int t = j + 1; t < len; t++) {
+                    tp.op(t - oldsize + m) = gops[(size_t)t * N];
+                    tp.cv(t - oldsize + m) = gcs[(size_t)t * N];
+                }
+                int depth = 0, dmx = 0;
+                for (int t = 0; t < plen; t++) { depth += 1 - gpr_arity(tp.op(t)); dmx = max(dmx, depth); }
+                if (dmx > a.dmax) skip = true;
+                extra = (-(double)gpr_logf((float)plen) + lq_old) - (-(double)gpr_logf((float)len) + lq_new);
             }
-            plen = len - oldsize + m;
-            for (int t = 0; t < m; t++) { tp.op(s0 + t) = tp.op(cap - m + t); tp.cv(s0 + t) = tp.cv(cap - m + t); }
-            for (int t = j + 1; t < len; t++) {
-                tp.op(t - oldsize + m) = gops[(size_t)t * N];
-                tp.cv(t - oldsize + m) = gcs[(size_t)t * N];
-            }
-            for (int t = 0; t < s0; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
-            int depth = 0, dmx = 0;
-            for (int t = 0; t < plen; t++) { depth += 1 - gpr_arity(tp.op(t)); dmx = max(dmx, depth); }
-            double lq_old = 0.0;
-            for (int t = s0; t <= j; t++) {
-                const int o = gops[(size_t)t * N];
-                double term = gpr_gen_logp(o);
-                if (o == GPR_OP_CONST) term += gpr_lphi(gcs[(size_t)t * N]);
-                lq_old += term;
-            }
-            if (dmx > a.dmax) {
-                for (int t = 0; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
-                continue;
-            }
-            extra = (-(double)gpr_logf((float)plen) + lq_old) - (-(double)gpr_logf((float)len) + lq_new);
         } else {
             int nconst = 0;
             for (int t = 0; t < len; t++) nconst += tp.op(t) == GPR_OP_CONST;
-            if (nconst == 0) continue;
-            n_cprop++;
-            for (int t = 0; t < len; t++) {
-                if (tp.op(t) != GPR_OP_CONST) continue;
-                const gpr_u4 w = gpr_step_block(a.seed, (uint32_t)chain, step, GPR_P_CPROP, (uint32_t)(t & 63),
-                                                (uint32_t)(t >> 6));
-                tp.cv(t) = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(w, (int)(step & 3)))), gcs[(size_t)t * N]);
+            if (nconst == 0) {
+                skip = true;
+            } else {
+                n_cprop++;
+                for (int t = 0; t < len; t++) {
+                    if (tp.op(t) != GPR_OP_CONST) continue;
+                    const gpr_u4 w = gpr_step_block(a.seed, (uint32_t)chain, step, GPR_P_CPROP, (uint32_t)(t & 63),
+                                                    (uint32_t)(t >> 6));
+                    tp.cv(t) = gpr_axpy(a.sigma, gpr_normal(gpr_u01(gpr_u4_word(w, (int)(step & 3)))), tp.cv(t));
+                }
             }
         }
-        const double pp = lane_prior(tp, plen, a.lnsym);
-        const double pl = lane_loglik(tp, plen, stk, d, dx, dy, dw);
-        n_evals += (unsigned long long)plen * (unsigned long long)d.n;
-        {
+        double pp = 0.0;
+        double reject_below = -GPR_INF;
+        if (!skip) {
+            pp = lane_prior(tp, plen, a.lnsym);
+            // a = pp + pl - post + extra < -17 rejects without a draw (cfg_accept); an LL
+            // below this bound gives a < -18 after any FP64 rounding of a
+            if (__builtin_isfinite(post) && __builtin_isfinite(pp) && __builtin_isfinite(extra))
+                reject_below = post - pp - extra - 18.0 -
+                               1e-12 * (__builtin_fabs(post) + __builtin_fabs(pp) + __builtin_fabs(extra));
+        }
+        int pts;
+        const double pl = lane_loglik(tp, plen, stk, d, dx, dy, dw, stage, reject_below, &pts, !skip);
+        bool acc = false;
+        if (!skip) {
+            n_evals += (unsigned long long)plen * (unsigned long long)pts;
             int nleaf = 0, nconst = 0;
             for (int t = 0; t < plen; t++) { nleaf += tp.op(t) <= GPR_OP_ONE; nconst += gpr_has_const(tp.op(t)); }
-            n_flops += (unsigned long long)(plen - nleaf + 4) * (unsigned long long)d.n + (unsigned long long)nconst;
+            n_flops += (unsigned long long)(plen - nleaf + 4) * (unsigned long long)pts + (unsigned long long)nconst;
+            acc = cfg_accept(a.seed, (uint32_t)chain, step, post, pp + pl, (pp + pl) - post + extra);
         }
-        if (cfg_accept(a.seed, (uint32_t)chain, step, post, pp + pl, (pp + pl) - post + extra)) {
-            for (int t = 0; t < plen; t++) { gops[(size_t)t * N] = tp.op(t); gcs[(size_t)t * N] = tp.cv(t); }
-            for (int t = plen; t < len; t++) { gops[(size_t)t * N] = 0; gcs[(size_t)t * N] = 0.0f; }
+        if (acc) {
+            // write back what changed: the rewritten rows, or the constants
+            if (dirty >= 0) {
+                for (int t = s0; t < plen; t++) { gops[(size_t)t * N] = tp.op(t); gcs[(size_t)t * N] = tp.cv(t); }
+                for (int t = max(plen, s0); t < len; t++) { gops[(size_t)t * N] = 0; gcs[(size_t)t * N] = 0.0f; }
+            } else {
+                for (int t = 0; t < plen; t++)
+                    if (tp.op(t) == GPR_OP_CONST) gcs[(size_t)t * N] = tp.cv(t);
+            }
             len = plen;
             prior = pp;
             ll = pl;
             if (structural) n_sacc++; else n_cacc++;
-        } else {
-            for (int t = 0; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
+        } else if (dirty >= 0) {
+            // a structure proposal rewrote [dirty, cap): restore the current program's rows there
+            for (int t = dirty; t < len; t++) { tp.op(t) = gops[(size_t)t * N]; tp.cv(t) = gcs[(size_t)t * N]; }
+        } else if (!skip) {
+            for (int t = 0; t < len; t++)
+                if (tp.op(t) == GPR_OP_CONST) tp.cv(t) = gcs[(size_t)t * N];
         }
     }
     a.len[chain] = len;
@@ -837,7 +932,8 @@ __global__ void __launch_bounds__(GPR_CL_WG) k_mh_config_lane(GprMHArgs a) {
 
 hipError_t gpr_launch_mh_config_lane(const GprMHArgs &a, int cap_max, hipStream_t s) {
     const int grid = (a.nchains + GPR_CL_WG - 1) / GPR_CL_WG;
-    const size_t sh = sizeof(float) * (GPR_MAX_DEPTH + (size_t)cap_max) * GPR_CL_WG + (size_t)cap_max * GPR_CL_WG;
+    const size_t sh = sizeof(float) * ((GPR_MAX_DEPTH + (size_t)cap_max) * GPR_CL_WG + 3 * 64) +
+                      (size_t)cap_max * GPR_CL_WG;
     if (sh > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void *)k_mh_config_lane, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         if (e != hipSuccess) return e;

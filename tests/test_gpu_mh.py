@@ -35,12 +35,19 @@ def test_init_matches_oracle(oracle, gpr):
 
 
 @pytest.mark.parametrize("cap,n,steps,sched", [(32, 32, 300, "wave"), (64, 250, 150, "wave"), (128, 70, 150, "wave"),
-                                               (32, 32, 300, "lane"), (128, 70, 100, "lane")])
+                                               (32, 32, 300, "lane"), (128, 70, 100, "lane"),
+                                               (64, "L", 150, "wave"), (64, "L", 100, "lane")])
 def test_mh_trajectories_match_oracle(oracle, gpr, cap, n, steps, sched):
-    rng = np.random.default_rng(cap + n)
-    if n == 32:
+    """n = "L": the plain large workload's own dataset (n = 250, x on [-5, 5], a random
+    target of at most 12 nodes plus 1% noise) at its cap of 64, on both schedules."""
+    if n == "L":
+        from gpuropolis_amd import workloads as W
+        x, y, sd = W.dataset_L(0)[0]
+        n = len(x)
+    elif n == 32:
         x, y, sd = s_data()
     else:
+        rng = np.random.default_rng(cap + n)
         x = rng.uniform(-5, 5, n).astype(np.float32)
         y = (1.5 * np.exp(-0.4 * x) + rng.normal(0, 0.05, n)).astype(np.float32)
         sd = np.full(n, 0.05, dtype=np.float32)
