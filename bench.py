@@ -47,6 +47,7 @@ HBM_PEAK = 8.0e12
 # were measured on.  For the batched workload a "launch" is one block's six concurrent
 # size-class launches.
 MEASURED_TRAFFIC = {
+    ("full", 1048576, 1000, 0): 1_970_964_768,
     ("large", 131072, 1000, 0): 269_317_600,
     ("batched", 262144, 1000, 0): 978_668_064,
 }
@@ -55,6 +56,7 @@ MEASURED_TRAFFIC = {
 # the share of issue slots each unit is busy: one wave64 instruction per SIMD every 4
 # cycles, 1024 SIMDs at 2.4 GHz.
 MEASURED_ISSUE = {
+    ("full", 1048576, 1000, 0): (1171.97, 1025.99),
     ("large", 131072, 1000, 0): (297.86, 299.57),
     ("batched", 262144, 1000, 0): (329.50, 301.71),
 }
