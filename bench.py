@@ -64,6 +64,11 @@ MEASURED_ISSUE = {
 METRIC = "MH steps/sec + node×datapoint evals/sec (% VALU peak); ESS/sec MALA vs RW"
 
 
+def progress(name, value):
+    """One line on stderr as each part of the run ends (the JSON line goes to stdout)."""
+    print(f"bench.py: {name} done ({value:.4g})", file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -562,17 +567,23 @@ def main():
         "roofline": roofline("k_mh_config_wave<1,true,true,false,false>", st.alg_flops / args.steps, launch_s,
                              ("full", nchains, args.mh_steps, args.seed)),
     }
+    progress("full form", out["value"])
     if not args.only_headline:
         if args.large_blocks > 0:
             out["large"] = large(G, args)
+            progress("large", out["large"]["mh_steps_per_s"])
         if args.batched_blocks > 0:
             out["batched"] = batched(G, args.seed, args.batched_blocks, args.mh_steps)
+            progress("batched", out["batched"]["mh_steps_per_s"])
         if args.nist_blocks > 0:
             out["nist"] = nist_batch(G, args.seed, args.nist_blocks, args.mh_steps)
+            progress("nist", out["nist"]["config"]["mh_steps_per_s"])
         if args.ess:
             out["ess"] = ess_rational(G, args.seed)
+            progress("ess", out["ess"]["mala_over_rw_ess_per_s"])
         if not args.no_cpu_baseline:
             out["cpu_baseline"] = cpu_baseline(args.seed, args.cpu_scale)
+            progress("cpu_baseline", out["cpu_baseline"]["value"])
     print(json.dumps(out), flush=True)
 
 
