@@ -1120,3 +1120,11 @@ extern "C" int gpr_test_draws(gpr_ctx *c, const uint32_t *d_words, int n, uint64
     HIP_TRY(hipStreamSynchronize(c->stream));
     return GPR_OK;
 }
+
+extern "C" int gpr_test_powf(gpr_ctx *c, const float *d_a, const float *d_b, int n, float *d_out) {
+    if (!c || !d_a || !d_b || !d_out || n < 1) return fail(GPR_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(c->cfg.device));
+    HIP_TRY(gpr_launch_test_powf(d_a, d_b, n, d_out, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return GPR_OK;
+}

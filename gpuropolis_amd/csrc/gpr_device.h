@@ -112,10 +112,10 @@ __device__ __forceinline__ float op_bin(int op, float a, float b) {
     case GPR_OP_SUB: return a - b;
     case GPR_OP_MUL: return a * b;
     case GPR_OP_DIV: return a / b;
-    case GPR_OP_POW: return powf(a, b);
+    case GPR_OP_POW: return gpr_powf(a, b);
     case GPR_OP_RSUB: return b - a;
     case GPR_OP_RDIV: return b / a;
-    default: return powf(b, a);  // RPOW
+    default: return gpr_powf(b, a);  // RPOW
     }
 }
 __device__ __forceinline__ float op_un(int op, float a, float c) {
@@ -134,8 +134,8 @@ __device__ __forceinline__ float op_un(int op, float a, float c) {
     case GPR_OP_ABS: return fabsf(a);
     case GPR_OP_CADD: return a + c;
     case GPR_OP_CMUL: return c * a;
-    case GPR_OP_CPOW: return powf(a, c);
-    case GPR_OP_CRPOW: return powf(c, a);
+    case GPR_OP_CPOW: return gpr_powf(a, c);
+    case GPR_OP_CRPOW: return gpr_powf(c, a);
     default: return __builtin_nanf("");
     }
 }
@@ -211,8 +211,8 @@ __device__ __forceinline__ void wave_eval(const WaveProg &p, const float (&xs)[K
             case GPR_OP_DIV: GPR_KLOOP(S.get(k, sp) / T[k]); break;
             case GPR_OP_RSUB: GPR_KLOOP(T[k] - S.get(k, sp)); break;
             case GPR_OP_RDIV: GPR_KLOOP(T[k] / S.get(k, sp)); break;
-            case GPR_OP_POW: GPR_KLOOP(powf(S.get(k, sp), T[k])); break;
-            default: GPR_KLOOP(powf(T[k], S.get(k, sp))); break;
+            case GPR_OP_POW: GPR_KLOOP(gpr_powf(S.get(k, sp), T[k])); break;
+            default: GPR_KLOOP(gpr_powf(T[k], S.get(k, sp))); break;
             }
         } else {
             const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 0.0f;
@@ -231,8 +231,8 @@ __device__ __forceinline__ void wave_eval(const WaveProg &p, const float (&xs)[K
             case GPR_OP_ABS: GPR_KLOOP(fabsf(T[k])); break;
             case GPR_OP_CADD: GPR_KLOOP(T[k] + c); break;
             case GPR_OP_CMUL: GPR_KLOOP(c * T[k]); break;
-            case GPR_OP_CPOW: GPR_KLOOP(powf(T[k], c)); break;
-            case GPR_OP_CRPOW: GPR_KLOOP(powf(c, T[k])); break;
+            case GPR_OP_CPOW: GPR_KLOOP(gpr_powf(T[k], c)); break;
+            case GPR_OP_CRPOW: GPR_KLOOP(gpr_powf(c, T[k])); break;
             default: GPR_KLOOP(__builtin_nanf("")); break;
             }
         }
@@ -361,8 +361,8 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
                 case GPR_OP_DIV: GPR_KLOOP(U[k] / T[k]); break;
                 case GPR_OP_RSUB: GPR_KLOOP(T[k] - U[k]); break;
                 case GPR_OP_RDIV: GPR_KLOOP(T[k] / U[k]); break;
-                case GPR_OP_POW: GPR_KLOOP(powf(U[k], T[k])); break;
-                default: GPR_KLOOP(powf(T[k], U[k])); break;
+                case GPR_OP_POW: GPR_KLOOP(gpr_powf(U[k], T[k])); break;
+                default: GPR_KLOOP(gpr_powf(T[k], U[k])); break;
                 }
             }
             ls_load<KC>(lane + (sp - 3) * 64 * KC, U);   // the new second entry (level sp-3)
@@ -384,8 +384,8 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
             case GPR_OP_ABS: GPR_KLOOP(fabsf(T[k])); break;
             case GPR_OP_CADD: GPR_KLOOP(T[k] + c); break;
             case GPR_OP_CMUL: GPR_KLOOP(c * T[k]); break;
-            case GPR_OP_CPOW: GPR_KLOOP(powf(T[k], c)); break;
-            case GPR_OP_CRPOW: GPR_KLOOP(powf(c, T[k])); break;
+            case GPR_OP_CPOW: GPR_KLOOP(gpr_powf(T[k], c)); break;
+            case GPR_OP_CRPOW: GPR_KLOOP(gpr_powf(c, T[k])); break;
             default: GPR_KLOOP(__builtin_nanf("")); break;
             }
         }
@@ -533,6 +533,14 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // floats of the dual stack of one wave: GPR_DUAL_DEPTH + 2 levels of 64 lanes x W
 #define GPR_DUAL_LDS_FLOATS ((GPR_DUAL_DEPTH + 2) * 64 * GPR_DUAL_W)
 
+// a^(b-1), the partial derivative of a^b in a over b, given v = a^b: v / a where a is
+// finite and non-zero and v finite on every lane of the wave (a division instead of a
+// second pow), else the pow itself.
+__device__ __forceinline__ float pow_bm1(float a, float b, float v) {
+    if (ballot(a == 0.0f || !__builtin_isfinite(a) || !__builtin_isfinite(v))) return gpr_powf(a, b - 1.0f);
+    return v / a;
+}
+
 // A zero tangent contributes nothing, whatever the partial derivative (an infinite or NaN
 // partial of a constant-free operand must not reach the constants, as in the adjoint).
 __device__ __forceinline__ float tmul(float t, float partial) { return t != 0.0f ? t * partial : 0.0f; }
@@ -626,14 +634,8 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
         sp--;
     } else if (op == GPR_OP_POW) {
         const float a = U[0], b = T[0];
-        const float v = powf(a, b);
-        // d/da a^b = b a^(b-1) = b (a^b / a) where a is finite and non-zero and a^b finite
-        float pa;
-        if (ballot(a == 0.0f || !__builtin_isfinite(a) || !__builtin_isfinite(v)))
-            pa = b * powf(a, b - 1.0f);
-        else
-            pa = b * (v / a);
-        const float pb = v * logf(a);
+        const float v = gpr_powf(a, b);
+        const float pa = b * pow_bm1(a, b, v), pb = v * logf(a);
 #pragma unroll
         for (int j = 1; j < W; j++) T[j] = tmul(U[j], pa) + tmul(T[j], pb);
         T[0] = v;
@@ -813,8 +815,8 @@ __device__ __forceinline__ void wave_loglik_groups(const WaveProg &p, const GprD
             case GPR_OP_DIV: T = U / T; break;
             case GPR_OP_RSUB: T = T - U; break;
             case GPR_OP_RDIV: T = T / U; break;
-            case GPR_OP_POW: T = powf(U, T); break;
-            default: T = powf(T, U); break;
+            case GPR_OP_POW: T = gpr_powf(U, T); break;
+            default: T = gpr_powf(T, U); break;
             }
             U = lane[(sp - 3) * 64];
             sp--;
@@ -1145,8 +1147,8 @@ __device__ __forceinline__ void grad_reverse(const WaveProg &p, const GradPlan &
                 case GPR_OP_DIV: ga = G / b; gb = -(G * a) / (b * b); break;
                 case GPR_OP_RSUB: ga = -G; gb = G; break;
                 case GPR_OP_RDIV: gb = G / a; ga = -(G * b) / (a * a); break;
-                case GPR_OP_POW: ga = G * b * powf(a, b - 1.0f); gb = G * v * logf(a); break;
-                default: gb = G * a * powf(b, a - 1.0f); ga = G * v * logf(b); break;  // RPOW: b^a
+                case GPR_OP_POW: ga = G * b * pow_bm1(a, b, v); gb = G * v * logf(a); break;
+                default: gb = G * a * pow_bm1(b, a, v); ga = G * v * logf(b); break;  // RPOW: b^a
                 }
             } else {
                 const float a = tok_val<HI>(p, g, t - 1, x, lvl);
@@ -1167,8 +1169,8 @@ __device__ __forceinline__ void grad_reverse(const WaveProg &p, const GradPlan &
                 case GPR_OP_ABS: ga = a < 0.0f ? -G : G; break;
                 case GPR_OP_CADD: ga = G; gc = G; break;
                 case GPR_OP_CMUL: ga = G * c; gc = G * a; break;
-                case GPR_OP_CPOW: ga = G * c * powf(a, c - 1.0f); gc = G * v * logf(a); break;
-                case GPR_OP_CRPOW: ga = G * v * logf(c); gc = G * a * powf(c, a - 1.0f); break;
+                case GPR_OP_CPOW: ga = G * c * pow_bm1(a, c, v); gc = G * v * logf(a); break;
+                case GPR_OP_CRPOW: ga = G * v * logf(c); gc = G * a * pow_bm1(c, a, v); break;
                 default: ga = __builtin_nanf(""); break;
                 }
                 if (op >= GPR_OP_CADD) lvl[t * 64] = gc;

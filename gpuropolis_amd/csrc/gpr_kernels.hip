@@ -21,14 +21,16 @@
 
 // ---------------------------------------------------------------- config MH, wave schedule
 // MALA: constant moves are Langevin proposals c' = c + h g(c) + eps z (h = eps^2/2, g the
-// gradient of the log posterior from the adjoint sweep) with the Hastings ratio of the
-// two Gaussian proposal densities.  Under MALA the interpreter evaluates every
-// likelihood at one point per lane (KC = 1), with early rejection; the gradients of the
-// proposals that survive it come from the stack-free adjoint sweeps, whose token values
-// sit in the wave's LDS region for programs of at most GPR_MALA_LDS_ROWS tokens and in
-// the wave's slab in global memory for longer ones (LDS rows for every token of a
-// 128-token cap, 32 KB per wave, would leave room for five waves per CU).  The current
-// structure's sweep plan stays in registers.
+// gradient of the log posterior) with the Hastings ratio of the two Gaussian proposal
+// densities.  Under MALA every likelihood is evaluated at one point per lane (KC = 1),
+// with early rejection.  Programs of at most three constants and stack depth 6 get the
+// proposal's LL and gradient from one forward-mode interpreter pass (value and tangents
+// per stack entry); the others are pre-checked by the plain interpreter, and the
+// survivors get the stack-free adjoint sweeps, whose token values sit in the wave's LDS
+// region for programs of at most GPR_MALA_LDS_ROWS tokens and in the wave's slab in
+// global memory for longer ones (LDS rows for every token of a 128-token cap, 32 KB per
+// wave, would leave room for five waves per CU).  The current structure's sweep plan
+// stays in registers.
 extern __shared__ float gpr_lds_slab[];
 
 // gpr_logf of the integers 0 .. GPR_MAX_CAP (program lengths), filled once per device by
@@ -663,10 +665,10 @@ __device__ __forceinline__ float lane_op(int o, float a, float b, float c) {
     case GPR_OP_SUB: return a - b;
     case GPR_OP_MUL: return a * b;
     case GPR_OP_DIV: return a / b;
-    case GPR_OP_POW: return powf(a, b);
+    case GPR_OP_POW: return gpr_powf(a, b);
     case GPR_OP_RSUB: return b - a;
     case GPR_OP_RDIV: return b / a;
-    case GPR_OP_RPOW: return powf(b, a);
+    case GPR_OP_RPOW: return gpr_powf(b, a);
     default: return op_un(o, a, c);
     }
 }

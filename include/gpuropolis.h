@@ -274,6 +274,10 @@ int gpr_ref_op_set(int set, uint8_t *codes, int max_codes, const char **program_
 int gpr_test_ref_decode(gpr_ctx *ctx, const uint8_t *d_hop, const float *d_hc, int nheaps, uint8_t *d_ops,
                         float *d_cs, int32_t *d_meta);
 int gpr_test_draws(gpr_ctx *ctx, const uint32_t *d_words, int n, uint64_t seed, float *d_out);
+/* gpr_test_powf writes a[i]^b[i] as the evaluators compute it (gpr_powf of
+ * gpuropolis_amd/csrc/gpr_pow.h, which a host build of that header reproduces bit for
+ * bit). */
+int gpr_test_powf(gpr_ctx *ctx, const float *d_a, const float *d_b, int n, float *d_out);
 
 #ifdef __cplusplus
 }
