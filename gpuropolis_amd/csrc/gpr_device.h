@@ -545,6 +545,30 @@ __device__ __forceinline__ float pow_bm1(float a, float b, float v) {
 // partial of a constant-free operand must not reach the constants, as in the adjoint).
 __device__ __forceinline__ float tmul(float t, float partial) { return t != 0.0f ? t * partial : 0.0f; }
 
+// The tangents times a partial derivative (tmul), and U's plus T's times two partials.
+// When the partials are finite on every lane of the wave, plain products give the same
+// values (up to the sign of a zero), without the per-tangent test.
+template <int W>
+__device__ __forceinline__ void tscale(float (&T)[W], float pt) {
+    if (ballot(!__builtin_isfinite(pt))) {
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(T[j], pt);
+    } else {
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = T[j] * pt;
+    }
+}
+template <int W>
+__device__ __forceinline__ void tcombine(float (&T)[W], const float (&U)[W], float pu, float pt) {
+    if (ballot(!__builtin_isfinite(pu) || !__builtin_isfinite(pt))) {
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(U[j], pu) + tmul(T[j], pt);
+    } else {
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = U[j] * pu + T[j] * pt;
+    }
+}
+
 template <bool HI>
 __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
     constexpr uint32_t GRAMMAR = (1u << GPR_OP_X) | (1u << GPR_OP_CONST) | (1u << GPR_OP_ONE) | (1u << GPR_OP_ADD) |
@@ -580,8 +604,9 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
             if (m.mul[h] & nb) {
                 const float tv = T[0];
                 T[0] = T[0] * b;
+                tscale<W>(T, b);
 #pragma unroll
-                for (int j = 1; j < W; j++) T[j] = tmul(T[j], b) + (rk == j - 1 ? tv : 0.0f);
+                for (int j = 1; j < W; j++) T[j] = T[j] + (rk == j - 1 ? tv : 0.0f);
             } else {
                 const float sgn = (m.sub[h] & nb) ? -1.0f : 1.0f;
                 T[0] = __builtin_fmaf(sgn, b, T[0]);
@@ -605,8 +630,7 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
         if (m.mul[h] & bit) {
             const float uv = U[0], tv = T[0];
             T[0] = U[0] * T[0];
-#pragma unroll
-            for (int j = 1; j < W; j++) T[j] = tmul(U[j], tv) + tmul(T[j], uv);
+            tcombine<W>(T, U, tv, uv);
         } else {
             const float sgn = (m.sub[h] & bit) ? -1.0f : 1.0f;
             T[0] = __builtin_fmaf(sgn, T[0], U[0]);
@@ -627,8 +651,7 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
         const float a = U[0], b = T[0];
         const float q = a / b;
         const float ib = 1.0f / b, nq = -(a / (b * b));
-#pragma unroll
-        for (int j = 1; j < W; j++) T[j] = tmul(U[j], ib) + tmul(T[j], nq);
+        tcombine<W>(T, U, ib, nq);
         T[0] = q;
         ls_load<W>(lane + (sp - 3) * 64 * W, U);
         sp--;
@@ -636,22 +659,19 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
         const float a = U[0], b = T[0];
         const float v = gpr_powf(a, b);
         const float pa = b * pow_bm1(a, b, v), pb = v * logf(a);
-#pragma unroll
-        for (int j = 1; j < W; j++) T[j] = tmul(U[j], pa) + tmul(T[j], pb);
+        tcombine<W>(T, U, pa, pb);
         T[0] = v;
         ls_load<W>(lane + (sp - 3) * 64 * W, U);
         sp--;
     } else if (op == GPR_OP_EXP) {
         const float v = expf(T[0]);
-#pragma unroll
-        for (int j = 1; j < W; j++) T[j] = tmul(T[j], v);
+        tscale<W>(T, v);
         T[0] = v;
     } else if (op == GPR_OP_LOG) {
         const float a = T[0];
         const float ia = 1.0f / a;
         T[0] = logf(a);
-#pragma unroll
-        for (int j = 1; j < W; j++) T[j] = tmul(T[j], ia);
+        tscale<W>(T, ia);
     } else {   // GPR_OP_NEG (dual_ok admits no other op)
 #pragma unroll
         for (int j = 0; j < W; j++) T[j] = -T[j];
