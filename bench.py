@@ -236,7 +236,7 @@ def ess_multichain(x):
     w = acov[0] * n / (n - 1)
     var_plus = (n - 1) / n * w + x.mean(axis=1).var(ddof=1)
     if not var_plus > 0:
-        return float(m * n)
+        return 0.0                       # no chain moved: no information
     rho = 1.0 - (w - acov * n / (n - 1)) / var_plus
     rho[0] = 1.0
     tau, prev = -1.0, np.inf
@@ -328,6 +328,124 @@ def ess_rational(G, seed=0, nch=16384, thin=2, blocks=160):
                   "ess_per_step": r["ess_per_step"], "mh_steps_per_s": r["mh_steps_per_s"],
                   "ess_per_s": r["ess_per_s"]}
     res["mala_over_rw_ess_per_s"] = out["mala"]["ess_per_s"] / out["rw"]["ess_per_s"]
+    return res
+
+
+def ess_batched(G, seed=0, nch=512, thin=2, blocks=120):
+    """MALA against random walk on the batched workload's datasets, each with its own
+    target's structure fixed: the datasets whose target has 1 to 16 constants, nch chains
+    each, all in one context (one persistent launch per size class).  Warm-up adapts a
+    proposal metric per dataset (Cholesky factor of the chains' covariance of its
+    constants, five rounds); each sampler then runs with each dataset at the step size
+    that gave it the most ESS per step among five pilots.
+    Per dataset, ESS = the smallest over its constants (all its chains pooled); ESS per
+    second = ESS per chain-step x the MH steps per second of the whole batched launch
+    (one 1000-step block).  Reported: the ratio of the summed ESS per second, and the
+    median and geometric mean of the per-dataset ratios."""
+    import numpy as np
+    from gpuropolis_amd import workloads as W
+    dsets, _, targets = W.datasets_B(seed)
+    cap, CONST = 64, G.OP["CONST"]
+    use = [d for d, (o, _) in enumerate(targets) if 1 <= int(np.sum(o == CONST)) <= 16 and len(o) <= cap]
+    nd = len(use)
+    N = nd * nch
+    O = np.zeros((cap, N), dtype=np.uint8)
+    V = np.zeros((cap, N), dtype=np.float32)
+    L = np.zeros(N, dtype=np.int32)
+    rng = np.random.default_rng(seed + 11)
+    rows = []
+    for k, d in enumerate(use):
+        o, c = targets[d]
+        r = np.flatnonzero(o == CONST)
+        rows.append(r)
+        sl = slice(k * nch, (k + 1) * nch)
+        O[:len(o), sl] = o[:, None]
+        V[:len(o), sl] = c[:, None]
+        V[r, sl] += (1e-3 * (1 + np.abs(c[r]))[:, None] * rng.normal(size=(len(r), nch))).astype(np.float32)
+        L[sl] = len(o)
+    data = [dsets[d] for d in use]
+    nrows = 1 + max(int(r.max()) for r in rows)           # the constants' rows (targets of at most 16 tokens)
+
+    def context(prop, step, cs, chols, scale=None):
+        # scale[k]: dataset k's step relative to the global one (its metric factor scaled;
+        # for MALA the drift's metric scales with its square, as a step of eps * scale[k])
+        kw = dict(eps=step) if prop == G.PROP_MALA else dict(sigma=step)
+        ctx = G.Context(cap=cap, seed=seed + 7, proposal=prop, fixed_structure=1, resample=G.RS_NONE,
+                        schedule=G.SCHED_WAVE, **kw)
+        ctx.set_data(data, [nch] * nd)
+        if chols is not None:
+            for k in range(nd):
+                ctx.set_metric(k, rows[k], chols[k] * (1.0 if scale is None else scale[k]))
+        ctx.write_state(O, cs, L)
+        return ctx
+
+    def metric(cs):
+        out = []
+        for k in range(nd):
+            x = cs[rows[k]][:, k * nch:(k + 1) * nch].astype(np.float64)
+            cov = np.atleast_2d(np.cov(x))
+            cov = cov + (1e-12 * np.trace(cov) + 1e-30) * np.eye(len(rows[k]))
+            out.append(np.linalg.cholesky(cov))
+        return out
+
+    cs, chols = V.copy(), None
+    for step, nsteps in ((1e-3, 1000), (1.0, 500), (1.0, 500), (1.0, 500), (1.0, 500)):
+        ctx = context(G.PROP_MALA, step, cs, chols)
+        ctx.run_block(nsteps)
+        cs = ctx.read_state()[1]
+        ctx.close()
+        chols = metric(cs)
+
+    def run(prop, step, nblocks, burn, scale=None):
+        ctx = context(prop, step, cs, chols, scale)
+        draws = []
+        for i in range(nblocks):
+            ctx.run_block(thin)
+            if i >= burn:
+                draws.append(ctx.read_state()[1][:nrows].copy())
+        ll = ctx.read_state()[4]
+        ctx.reset_stats()
+        ctx.run_block(1000)
+        rate = ctx.stats().steps / (ctx.stats().last_block_ms / 1e3)
+        ctx.close()
+        D = np.array(draws)                                   # [draws, nrows, N]
+        chain_steps = (nblocks - burn) * thin * nch
+        ess = np.zeros(nd)
+        for k in range(nd):
+            sl = slice(k * nch, (k + 1) * nch)
+            ess[k] = min(ess_multichain(D[:, j, sl].astype(np.float64).T) for j in rows[k])
+        valid = np.array([np.all(np.isfinite(ll[k * nch:(k + 1) * nch])) for k in range(nd)])
+        return dict(step=step, ess_per_step=ess / chain_steps, rate=rate, valid=valid)
+
+    out = {}
+    for name, prop, steps in (("rw", G.PROP_RW, [0.4, 0.8, 1.6, 2.4, 3.2]),
+                              ("mala", G.PROP_MALA, [0.6, 1.0, 1.4, 1.8, 2.4])):
+        # per dataset the pilot step with the most ESS per step (one launch serves all, so
+        # the cost per step is shared), then one run with each dataset at its own step
+        pilots = [run(prop, h, 50, 15) for h in steps]
+        eps = np.array([r["ess_per_step"] for r in pilots])
+        scale = np.array(steps)[np.argmax(eps, axis=0)]
+        out[name] = run(prop, 1.0, blocks, blocks // 4, scale)
+        out[name]["scale"] = scale
+    ok = out["rw"]["valid"] & out["mala"]["valid"]
+    e_rw = out["rw"]["ess_per_step"][ok] * out["rw"]["rate"]
+    e_ma = out["mala"]["ess_per_step"][ok] * out["mala"]["rate"]
+    ratio = e_ma / e_rw
+    nconst = np.array([len(r) for r in rows])[ok]
+    res = {"workload": f"batched, fixed structures: the batched workload's {nd} datasets whose target has 1 to 16 "
+                       f"constants, each target's structure fixed, {nch} chains per dataset in one context, proposal "
+                       f"metric per dataset adapted in warm-up, {blocks} draws thinned by {thin} MH steps, first quarter "
+                       f"discarded", "datasets_used": int(ok.sum()),
+           "mala_over_rw_ess_per_s": float(e_ma.sum() / e_rw.sum()),
+           "ratio_median": float(np.median(ratio)), "ratio_geomean": float(np.exp(np.mean(np.log(ratio))))}
+    for name in ("rw", "mala"):
+        r = out[name]
+        res[name] = {"step_median": float(np.median(r["scale"][ok])), "mh_steps_per_s": r["rate"],
+                     "ess_per_step_median": float(np.median(r["ess_per_step"][ok]))}
+    for lo, hi in ((1, 2), (3, 4), (5, 16)):
+        m = (nconst >= lo) & (nconst <= hi)
+        if m.any():
+            res[f"ratio_median_{lo}_to_{hi}_constants"] = float(np.median(ratio[m]))
     return res
 
 
@@ -509,7 +627,7 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=20,
                     help="autotune the wave launch shapes on this many MH steps after warm-up (0: defaults)")
     ap.add_argument("--only-headline", action="store_true", help="time the full form only (profiling runs)")
-    ap.add_argument("--only", choices=["large", "batched", "nist", "ess", "cpu"], default=None,
+    ap.add_argument("--only", choices=["large", "batched", "nist", "ess", "ess_batched", "cpu"], default=None,
                     help="run one sub-record alone and print it (profiling runs)")
     ap.add_argument("--step-split", choices=["large", "full"], default=None,
                     help="only time one block of constant steps and one of structure steps on a workload's "
@@ -535,6 +653,7 @@ def main():
                "batched": lambda: batched(G, args.seed, args.batched_blocks, args.mh_steps),
                "nist": lambda: nist_batch(G, args.seed, args.nist_blocks, args.mh_steps),
                "ess": lambda: ess_rational(G, args.seed),
+               "ess_batched": lambda: ess_batched(G, args.seed),
                "cpu": lambda: cpu_baseline(args.seed, args.cpu_scale)}[args.only]
         print(json.dumps({args.only: run()}), flush=True)
         return
@@ -581,6 +700,8 @@ def main():
         if args.ess:
             out["ess"] = ess_rational(G, args.seed)
             progress("ess", out["ess"]["mala_over_rw_ess_per_s"])
+            out["ess_batched"] = ess_batched(G, args.seed)
+            progress("ess_batched", out["ess_batched"]["mala_over_rw_ess_per_s"])
         if not args.no_cpu_baseline:
             out["cpu_baseline"] = cpu_baseline(args.seed, args.cpu_scale)
             progress("cpu_baseline", out["cpu_baseline"]["value"])
