@@ -47,16 +47,16 @@ HBM_PEAK = 8.0e12
 # were measured on.  For the batched workload a "launch" is one block's six concurrent
 # size-class launches.
 MEASURED_TRAFFIC = {
-    ("full", 1048576, 1000, 0): 1_970_964_768,
-    ("large", 131072, 1000, 0): 269_317_600,
-    ("batched", 262144, 1000, 0): 978_668_064,
+    ("full", 1048576, 1000, 0): 3_701_221_536,
+    ("large", 131072, 1000, 0): 268_676_832,
+    ("batched", 262144, 1000, 0): 982_742_432,
 }
 # VALU and SALU instructions per chain-step of the same launches (rocprofv3 --pmc
 # SQ_INSTS_VALU, SQ_INSTS_SALU over chains x steps).  With the live launch time they give
 # the share of issue slots each unit is busy: one wave64 instruction per SIMD every 4
 # cycles, 1024 SIMDs at 2.4 GHz.
 MEASURED_ISSUE = {
-    ("full", 1048576, 1000, 0): (1171.97, 1025.99),
+    ("full", 1048576, 1000, 0): (1167.41, 1033.11),
     ("large", 131072, 1000, 0): (297.86, 299.57),
     ("batched", 262144, 1000, 0): (329.50, 301.71),
 }

@@ -13,10 +13,6 @@
 
 #define GPR_HD __host__ __device__ __forceinline__
 
-// pow of the evaluators (computed in double, rounded once)
-#define GPR_POW_FN GPR_HD
-#include "gpr_pow.h"
-
 // ---------------------------------------------------------------- op table
 // Postfix op codes (GPR_OP_* in gpuropolis.h).  Leaves push, binary ops pop rhs
 // then lhs, unary ops rewrite the top of the stack.

@@ -112,10 +112,10 @@ __device__ __forceinline__ float op_bin(int op, float a, float b) {
     case GPR_OP_SUB: return a - b;
     case GPR_OP_MUL: return a * b;
     case GPR_OP_DIV: return a / b;
-    case GPR_OP_POW: return gpr_powf(a, b);
+    case GPR_OP_POW: return powf(a, b);
     case GPR_OP_RSUB: return b - a;
     case GPR_OP_RDIV: return b / a;
-    default: return gpr_powf(b, a);  // RPOW
+    default: return powf(b, a);  // RPOW
     }
 }
 __device__ __forceinline__ float op_un(int op, float a, float c) {
@@ -134,8 +134,8 @@ __device__ __forceinline__ float op_un(int op, float a, float c) {
     case GPR_OP_ABS: return fabsf(a);
     case GPR_OP_CADD: return a + c;
     case GPR_OP_CMUL: return c * a;
-    case GPR_OP_CPOW: return gpr_powf(a, c);
-    case GPR_OP_CRPOW: return gpr_powf(c, a);
+    case GPR_OP_CPOW: return powf(a, c);
+    case GPR_OP_CRPOW: return powf(c, a);
     default: return __builtin_nanf("");
     }
 }
@@ -211,8 +211,8 @@ __device__ __forceinline__ void wave_eval(const WaveProg &p, const float (&xs)[K
             case GPR_OP_DIV: GPR_KLOOP(S.get(k, sp) / T[k]); break;
             case GPR_OP_RSUB: GPR_KLOOP(T[k] - S.get(k, sp)); break;
             case GPR_OP_RDIV: GPR_KLOOP(T[k] / S.get(k, sp)); break;
-            case GPR_OP_POW: GPR_KLOOP(gpr_powf(S.get(k, sp), T[k])); break;
-            default: GPR_KLOOP(gpr_powf(T[k], S.get(k, sp))); break;
+            case GPR_OP_POW: GPR_KLOOP(powf(S.get(k, sp), T[k])); break;
+            default: GPR_KLOOP(powf(T[k], S.get(k, sp))); break;
             }
         } else {
             const float c = op >= GPR_OP_CADD ? (h ? rl_f(p.c1, l) : rl_f(p.c0, l)) : 0.0f;
@@ -231,8 +231,8 @@ __device__ __forceinline__ void wave_eval(const WaveProg &p, const float (&xs)[K
             case GPR_OP_ABS: GPR_KLOOP(fabsf(T[k])); break;
             case GPR_OP_CADD: GPR_KLOOP(T[k] + c); break;
             case GPR_OP_CMUL: GPR_KLOOP(c * T[k]); break;
-            case GPR_OP_CPOW: GPR_KLOOP(gpr_powf(T[k], c)); break;
-            case GPR_OP_CRPOW: GPR_KLOOP(gpr_powf(c, T[k])); break;
+            case GPR_OP_CPOW: GPR_KLOOP(powf(T[k], c)); break;
+            case GPR_OP_CRPOW: GPR_KLOOP(powf(c, T[k])); break;
             default: GPR_KLOOP(__builtin_nanf("")); break;
             }
         }
@@ -361,8 +361,8 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
                 case GPR_OP_DIV: GPR_KLOOP(U[k] / T[k]); break;
                 case GPR_OP_RSUB: GPR_KLOOP(T[k] - U[k]); break;
                 case GPR_OP_RDIV: GPR_KLOOP(T[k] / U[k]); break;
-                case GPR_OP_POW: GPR_KLOOP(gpr_powf(U[k], T[k])); break;
-                default: GPR_KLOOP(gpr_powf(T[k], U[k])); break;
+                case GPR_OP_POW: GPR_KLOOP(powf(U[k], T[k])); break;
+                default: GPR_KLOOP(powf(T[k], U[k])); break;
                 }
             }
             ls_load<KC>(lane + (sp - 3) * 64 * KC, U);   // the new second entry (level sp-3)
@@ -384,8 +384,8 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
             case GPR_OP_ABS: GPR_KLOOP(fabsf(T[k])); break;
             case GPR_OP_CADD: GPR_KLOOP(T[k] + c); break;
             case GPR_OP_CMUL: GPR_KLOOP(c * T[k]); break;
-            case GPR_OP_CPOW: GPR_KLOOP(gpr_powf(T[k], c)); break;
-            case GPR_OP_CRPOW: GPR_KLOOP(gpr_powf(c, T[k])); break;
+            case GPR_OP_CPOW: GPR_KLOOP(powf(T[k], c)); break;
+            case GPR_OP_CRPOW: GPR_KLOOP(powf(c, T[k])); break;
             default: GPR_KLOOP(__builtin_nanf("")); break;
             }
         }
@@ -537,37 +537,13 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // finite and non-zero and v finite on every lane of the wave (a division instead of a
 // second pow), else the pow itself.
 __device__ __forceinline__ float pow_bm1(float a, float b, float v) {
-    if (ballot(a == 0.0f || !__builtin_isfinite(a) || !__builtin_isfinite(v))) return gpr_powf(a, b - 1.0f);
+    if (ballot(a == 0.0f || !__builtin_isfinite(a) || !__builtin_isfinite(v))) return powf(a, b - 1.0f);
     return v / a;
 }
 
 // A zero tangent contributes nothing, whatever the partial derivative (an infinite or NaN
 // partial of a constant-free operand must not reach the constants, as in the adjoint).
 __device__ __forceinline__ float tmul(float t, float partial) { return t != 0.0f ? t * partial : 0.0f; }
-
-// The tangents times a partial derivative (tmul), and U's plus T's times two partials.
-// When the partials are finite on every lane of the wave, plain products give the same
-// values (up to the sign of a zero), without the per-tangent test.
-template <int W>
-__device__ __forceinline__ void tscale(float (&T)[W], float pt) {
-    if (ballot(!__builtin_isfinite(pt))) {
-#pragma unroll
-        for (int j = 1; j < W; j++) T[j] = tmul(T[j], pt);
-    } else {
-#pragma unroll
-        for (int j = 1; j < W; j++) T[j] = T[j] * pt;
-    }
-}
-template <int W>
-__device__ __forceinline__ void tcombine(float (&T)[W], const float (&U)[W], float pu, float pt) {
-    if (ballot(!__builtin_isfinite(pu) || !__builtin_isfinite(pt))) {
-#pragma unroll
-        for (int j = 1; j < W; j++) T[j] = tmul(U[j], pu) + tmul(T[j], pt);
-    } else {
-#pragma unroll
-        for (int j = 1; j < W; j++) T[j] = U[j] * pu + T[j] * pt;
-    }
-}
 
 template <bool HI>
 __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
@@ -604,9 +580,8 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
             if (m.mul[h] & nb) {
                 const float tv = T[0];
                 T[0] = T[0] * b;
-                tscale<W>(T, b);
 #pragma unroll
-                for (int j = 1; j < W; j++) T[j] = T[j] + (rk == j - 1 ? tv : 0.0f);
+                for (int j = 1; j < W; j++) T[j] = tmul(T[j], b) + (rk == j - 1 ? tv : 0.0f);
             } else {
                 const float sgn = (m.sub[h] & nb) ? -1.0f : 1.0f;
                 T[0] = __builtin_fmaf(sgn, b, T[0]);
@@ -630,7 +605,8 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
         if (m.mul[h] & bit) {
             const float uv = U[0], tv = T[0];
             T[0] = U[0] * T[0];
-            tcombine<W>(T, U, tv, uv);
+#pragma unroll
+            for (int j = 1; j < W; j++) T[j] = tmul(U[j], tv) + tmul(T[j], uv);
         } else {
             const float sgn = (m.sub[h] & bit) ? -1.0f : 1.0f;
             T[0] = __builtin_fmaf(sgn, T[0], U[0]);
@@ -651,27 +627,31 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
         const float a = U[0], b = T[0];
         const float q = a / b;
         const float ib = 1.0f / b, nq = -(a / (b * b));
-        tcombine<W>(T, U, ib, nq);
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(U[j], ib) + tmul(T[j], nq);
         T[0] = q;
         ls_load<W>(lane + (sp - 3) * 64 * W, U);
         sp--;
     } else if (op == GPR_OP_POW) {
         const float a = U[0], b = T[0];
-        const float v = gpr_powf(a, b);
+        const float v = powf(a, b);
         const float pa = b * pow_bm1(a, b, v), pb = v * logf(a);
-        tcombine<W>(T, U, pa, pb);
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(U[j], pa) + tmul(T[j], pb);
         T[0] = v;
         ls_load<W>(lane + (sp - 3) * 64 * W, U);
         sp--;
     } else if (op == GPR_OP_EXP) {
         const float v = expf(T[0]);
-        tscale<W>(T, v);
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(T[j], v);
         T[0] = v;
     } else if (op == GPR_OP_LOG) {
         const float a = T[0];
         const float ia = 1.0f / a;
         T[0] = logf(a);
-        tscale<W>(T, ia);
+#pragma unroll
+        for (int j = 1; j < W; j++) T[j] = tmul(T[j], ia);
     } else {   // GPR_OP_NEG (dual_ok admits no other op)
 #pragma unroll
         for (int j = 0; j < W; j++) T[j] = -T[j];
@@ -835,8 +815,8 @@ __device__ __forceinline__ void wave_loglik_groups(const WaveProg &p, const GprD
             case GPR_OP_DIV: T = U / T; break;
             case GPR_OP_RSUB: T = T - U; break;
             case GPR_OP_RDIV: T = T / U; break;
-            case GPR_OP_POW: T = gpr_powf(U, T); break;
-            default: T = gpr_powf(T, U); break;
+            case GPR_OP_POW: T = powf(U, T); break;
+            default: T = powf(T, U); break;
             }
             U = lane[(sp - 3) * 64];
             sp--;

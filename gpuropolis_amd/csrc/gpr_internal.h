@@ -159,4 +159,3 @@ hipError_t gpr_launch_clock_advance(uint32_t *clock, uint32_t steps, hipStream_t
 hipError_t gpr_launch_test_ref_decode(const uint8_t *hop, const float *hc, int nheaps, uint8_t *ops, float *cs,
                                       int32_t *meta, hipStream_t s);
 hipError_t gpr_launch_test_draws(const uint32_t *words, int n, uint64_t seed, float *out, hipStream_t s);
-hipError_t gpr_launch_test_powf(const float *a, const float *b, int n, float *out, hipStream_t s);

@@ -665,10 +665,10 @@ __device__ __forceinline__ float lane_op(int o, float a, float b, float c) {
     case GPR_OP_SUB: return a - b;
     case GPR_OP_MUL: return a * b;
     case GPR_OP_DIV: return a / b;
-    case GPR_OP_POW: return gpr_powf(a, b);
+    case GPR_OP_POW: return powf(a, b);
     case GPR_OP_RSUB: return b - a;
     case GPR_OP_RDIV: return b / a;
-    case GPR_OP_RPOW: return gpr_powf(b, a);
+    case GPR_OP_RPOW: return powf(b, a);
     default: return op_un(o, a, c);
     }
 }

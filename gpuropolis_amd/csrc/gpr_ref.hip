@@ -224,10 +224,10 @@ __device__ __forceinline__ float r_dispatch_eval(int o, float a, float b, float 
     case GPR_REF_TAN: return tanf(a);
     case GPR_REF_ATAN: return atanf(a);
     case GPR_REF_EXP: return expf(a);
-    case GPR_REF_POW: return gpr_powf(a, b);
-    case GPR_REF_CPOW: return gpr_powf(a, C);
-    case GPR_REF_RPOW: return gpr_powf(b, a);
-    case GPR_REF_CRPOW: return gpr_powf(C, a);
+    case GPR_REF_POW: return powf(a, b);
+    case GPR_REF_CPOW: return powf(a, C);
+    case GPR_REF_RPOW: return powf(b, a);
+    case GPR_REF_CRPOW: return powf(C, a);
     case GPR_REF_ABS: return fabsf(a);
     case GPR_REF_HALF: return (float)((double)a / 2.0);
     default: return gpr_ref_is_physical(o) ? gpr_ref_physical_value(o) : __builtin_nanf("");
@@ -805,16 +805,5 @@ __global__ void k_test_draws(const uint32_t *words, int n, uint64_t seed, float 
 
 hipError_t gpr_launch_test_draws(const uint32_t *words, int n, uint64_t seed, float *out, hipStream_t s) {
     hipLaunchKernelGGL(k_test_draws, dim3((n + 255) / 256), dim3(256), 0, s, words, n, seed, out);
-    return hipGetLastError();
-}
-
-// The evaluators' pow on the device, one pair per thread.
-__global__ void k_test_powf(const float *a, const float *b, int n, float *out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = gpr_powf(a[i], b[i]);
-}
-
-hipError_t gpr_launch_test_powf(const float *a, const float *b, int n, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_test_powf, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n, out);
     return hipGetLastError();
 }

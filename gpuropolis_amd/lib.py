@@ -33,7 +33,7 @@ MAX_CAP, MAX_DEPTH = 128, 16
 EXPORTS = ["gpr_config_default", "gpr_create", "gpr_destroy", "gpr_last_error", "gpr_version", "gpr_set_data",
            "gpr_init_chains", "gpr_write_state", "gpr_read_state", "gpr_run_block", "gpr_run_block_async",
            "gpr_resample", "gpr_run_blocks_graph", "gpr_get_stats", "gpr_reset_stats", "gpr_sync", "gpr_stream",
-           "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws", "gpr_test_powf",
+           "gpr_eval_loglik", "gpr_grad_logpost", "gpr_ref_decode", "gpr_test_ref_decode", "gpr_test_draws",
            "gpr_autotune", "gpr_set_tuning", "gpr_get_tuning", "gpr_set_metric", "gpr_ref_op_set",
            "gpr_get_clock", "gpr_set_clock", "gpr_eval_loglik_noalloc_async", "gpr_write_heap_i32",
            "gpr_read_heap_i32"]
@@ -109,7 +109,6 @@ def load():
         "gpr_ref_op_set": (C.c_int, [C.c_int, vp, C.c_int, C.POINTER(C.c_char_p)]),
         "gpr_test_ref_decode": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, vp]),
         "gpr_test_draws": (C.c_int, [vp, vp, C.c_int, C.c_uint64, vp]),
-        "gpr_test_powf": (C.c_int, [vp, vp, vp, C.c_int, vp]),
         "gpr_write_heap_i32": (C.c_int, [vp, vp, vp]),
         "gpr_read_heap_i32": (C.c_int, [vp, vp, vp, vp, vp]),
         "gpr_eval_loglik_noalloc_async": (C.c_int, [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int,
@@ -264,9 +263,6 @@ class Context:
 
     def test_draws_dev(self, d_words, n, seed, d_out):
         check(load().gpr_test_draws(self.h, d_words, n, seed, d_out))
-
-    def test_powf_dev(self, d_a, d_b, n, d_out):
-        check(load().gpr_test_powf(self.h, d_a, d_b, n, d_out))
 
     def autotune(self, steps=50):
         """Tune the launch shape of every size class on the current chains; returns the

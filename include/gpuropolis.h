@@ -196,7 +196,9 @@ int gpr_set_metric(gpr_ctx *ctx, int dataset, int k, const int32_t *rows, const 
  * points per lane per chunk (the data tile, 64*kc points), waves = chains per
  * workgroup, and workgroups per CU.  gpr_autotune times every candidate shape of each
  * class present on the context's own chains (steps MH steps each, from the same
- * state, which is restored afterwards) and keeps the fastest; with GPR_SCHED_AUTO it
+ * state, which is restored afterwards) and keeps the fastest, where a candidate must be
+ * 1 % faster than an earlier one (fewer points per lane, smaller workgroups, fewer
+ * workgroups per CU) to replace it; with GPR_SCHED_AUTO it
  * also picks the lane or wave schedule.  gpr_set_tuning applies a saved table. */
 typedef struct {
     int n_class;        /* 1: n <= 64, 2: n <= 128, 4: n > 128 */
@@ -274,10 +276,6 @@ int gpr_ref_op_set(int set, uint8_t *codes, int max_codes, const char **program_
 int gpr_test_ref_decode(gpr_ctx *ctx, const uint8_t *d_hop, const float *d_hc, int nheaps, uint8_t *d_ops,
                         float *d_cs, int32_t *d_meta);
 int gpr_test_draws(gpr_ctx *ctx, const uint32_t *d_words, int n, uint64_t seed, float *d_out);
-/* gpr_test_powf writes a[i]^b[i] as the evaluators compute it (gpr_powf of
- * gpuropolis_amd/csrc/gpr_pow.h, which a host build of that header reproduces bit for
- * bit). */
-int gpr_test_powf(gpr_ctx *ctx, const float *d_a, const float *d_b, int n, float *d_out);
 
 #ifdef __cplusplus
 }
