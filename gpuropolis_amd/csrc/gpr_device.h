@@ -22,6 +22,31 @@ __device__ __forceinline__ double rl_d(double v, int l) {
 __device__ __forceinline__ int wl_i(int old, int v, int l) { return lane_id() == l ? v : old; }
 __device__ __forceinline__ float wl_f(float old, float v, int l) { return lane_id() == l ? v : old; }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni_d(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long hi = (uint32_t)uni((int)(b >> 32)), lo = (uint32_t)uni((int)(uint32_t)b);
+    return __longlong_as_double((long long)((hi << 32) | lo));
+}
+// A dataset record with every field wave-uniform (scalar registers): a record loaded by
+// the whole wave otherwise lands in vector registers, where values used at every step
+// take registers from the step loop.
+__device__ __forceinline__ GprDataset uni_dataset(const GprDataset &g) {
+    GprDataset d;
+    d.n = uni(g.n);
+    d.off = uni(g.off);
+    d.chain0 = uni(g.chain0);
+    d.nchains = uni(g.nchains);
+    d.cap = uni(g.cap);
+    d.kc = uni(g.kc);
+    d.lconst = uni_d(g.lconst);
+    d.wconst = uni_d(g.wconst);
+    return d;
+}
+// A copy of a wave-uniform value that the compiler cannot prove equal to the original:
+// an address computed from it at a late use is computed there, instead of being kept
+// live (and spilled) from an earlier use of the same address.
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v) { __asm__ volatile("" : "+s"(v)); return v; }
+__device__ __forceinline__ int opaque_lane_i32(int v) { __asm__ volatile("" : "+v"(v)); return v; }
 
 // Lane gathers that stay in wave-uniform control flow: lane l receives v of lane
 // src mod 64 (ds_bpermute reads the byte address' lane bits, so no index masking is
@@ -763,7 +788,7 @@ __device__ __forceinline__ void wave_loglik_groups(const WaveProg &p, const GprD
                                                    const float *dy, const float *dw, float *ls, const OpMasks &m,
                                                    uint64_t cm, float cb, int ks, int u0, int G, int gn,
                                                    double (&out)[4]) {
-    const int l = lane_id();
+    const int l = opaque_lane_i32(lane_id());       // what follows is computed here, not kept live
     const int lgs = G == 4 ? 4 : 5;                  // log2 of the group size
     const int gl = l >> lgs;
     const int i = l & ((1 << lgs) - 1);

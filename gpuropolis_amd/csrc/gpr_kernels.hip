@@ -87,8 +87,11 @@ k_mh_config_wave(GprMHArgs a) {
     float *lvg = (MALA && a.mala_gslab) ? a.mala_gslab + ((size_t)blockIdx.x * a.wpg + wave) * (size_t)a.dmax_cap * 64
                                         : nullptr;
     const uint32_t step0 = a.clock ? (uint32_t)uni((int)a.clock[0]) : a.step0;
-    // launch counters, lane k holds counter k (folded in once per tile)
-    unsigned long long cnt = 0;
+    // launch counters: lane k adds counter k into its wave's LDS row once per chain (a
+    // register would live across the chain loop and spill to memory); the rows are folded
+    // into the global counters at the end
+    __shared__ unsigned long long s_cnt[GPR_WG / 64][GPR_NCNT];
+    if (l < GPR_NCNT) s_cnt[wave][l] = 0;
     for (;;) {
         __shared__ int s_tile;
         if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1);
@@ -98,7 +101,7 @@ k_mh_config_wave(GprMHArgs a) {
         if (tile_idx >= a.ntiles) break;
         const GprTile tile = a.tiles[tile_idx];
         if (wave >= tile.count) continue;
-        const GprDataset d = a.ds[tile.dataset];
+        const GprDataset d = uni_dataset(a.ds[tile.dataset]);
         const uint32_t chain = (uint32_t)uni(tile.chain0 + wave);
         const int cap = d.cap;
         const float *dx = a.dx + d.off, *dy = a.dy + d.off, *dw = a.dw + d.off;
@@ -591,7 +594,7 @@ k_mh_config_wave(GprMHArgs a) {
             v = l == GPR_CNT_EVALS ? c_ev : v;
             v = l == GPR_CNT_STEPS ? (unsigned long long)a.steps : v;
             v = l == GPR_CNT_FLOPS ? c_fl + c_nc : v;
-            cnt += v;
+            if (l < GPR_NCNT) s_cnt[wave][l] += v;
         }
         // Write back only a changed program, and only its rows below the longer of the old
         // and new lengths: the state holds zeros past a program's length (init, write-back),
@@ -599,15 +602,19 @@ k_mh_config_wave(GprMHArgs a) {
         // chain cost a memory transaction each, so skipping them saves most of the traffic.
         if (changed) {
             const int wlen = max(len0, cur.len);
-            if (l < wlen) { a.ops[(size_t)l * N + chain] = (uint8_t)cur.o0; a.cs[(size_t)l * N + chain] = cur.c0; }
+            const uint32_t cw = opaque_u32(chain);   // the load's addresses are not kept live
+            if (l < wlen) { a.ops[(size_t)l * N + cw] = (uint8_t)cur.o0; a.cs[(size_t)l * N + cw] = cur.c0; }
             if (HI && l + 64 < wlen) {
-                a.ops[(size_t)(l + 64) * N + chain] = (uint8_t)cur.o1;
-                a.cs[(size_t)(l + 64) * N + chain] = cur.c1;
+                a.ops[(size_t)(l + 64) * N + cw] = (uint8_t)cur.o1;
+                a.cs[(size_t)(l + 64) * N + cw] = cur.c1;
             }
         }
         if (l == 0) { a.len[chain] = cur.len; a.prior[chain] = prior; a.ll[chain] = ll; }
     }
-    if (l < GPR_NCNT && a.counters && cnt) atomicAdd(&a.counters[l], cnt);
+    if (l < GPR_NCNT && a.counters) {
+        const unsigned long long t = s_cnt[wave][l];
+        if (t) atomicAdd(&a.counters[l], t);
+    }
 }
 
 template <int KC, bool HI, bool MALA, bool MET, bool SPEC = false>
