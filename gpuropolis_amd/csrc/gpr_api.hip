@@ -859,7 +859,7 @@ extern "C" int gpr_autotune(gpr_ctx *c, int steps, gpr_tune *out, int max_out, i
         best.ms = 1e30;
         for (int kc : {1, 2, 4})
             for (int wpg : {1, 2, 4})
-                for (int wpc : {8, 16, 32}) {
+                for (int wpc : {32, 16, 8}) {
                     if (c->cfg.semantics == GPR_SEM_REFERENCE && hi) continue;
                     if (mala && kc != 1) continue;      // MALA evaluates one point per lane
                     const ClassTune cand{kc, wpg, wpc / wpg, 0.0};
@@ -877,11 +877,11 @@ extern "C" int gpr_autotune(gpr_ctx *c, int steps, gpr_tune *out, int max_out, i
                     TUNE_TRY(hipEventSynchronize(c->ev1));
                     float ms = 0.0f;
                     TUNE_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-                    // A later candidate (more points per lane, larger workgroups, more
+                    // A later candidate (more points per lane, larger workgroups, fewer
                     // workgroups per CU) must be 1 % faster to replace the best: timings
-                    // of a short run spread by about that much, and on a tie the smaller
-                    // footprint keeps less per-wave scratch and slab memory live in the
-                    // caches.
+                    // of a short run spread by about that much.  On a tie the class keeps
+                    // more workgroups, its larger share of the CUs when the size classes
+                    // of a block run concurrently (each is tuned alone).
                     if (ms < best.ms * 0.99) { best = cand; best.ms = ms; }
                 }
         c->ct[k][hi] = best;

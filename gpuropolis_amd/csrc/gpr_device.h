@@ -1319,7 +1319,8 @@ __device__ __forceinline__ double wave_gauss_sum_d(double f0, bool m0, double f1
 
 // lane l's row of a 16-wide float matrix, columns below k (lanes past k read row k-1)
 __device__ __forceinline__ void metric_row(const float *A, int k, float (&r)[GPR_METRIC_MAX]) {
-    const float4 *q = (const float4 *)(A + min(lane_id(), k - 1) * GPR_METRIC_MAX);
+    // (the row address is computed here, not kept live across the steps)
+    const float4 *q = (const float4 *)(A + min(opaque_lane_i32(lane_id()), k - 1) * GPR_METRIC_MAX);
 #pragma unroll
     for (int j = 0; j < GPR_METRIC_MAX; j += 4) {
         float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1364,7 +1365,7 @@ __device__ __forceinline__ float metric_Mv(const float *M, int k, float g) {
 __device__ __forceinline__ double metric_logq(const double *Li, int k, double r, double two_e2) {
 #pragma clang fp contract(off)
     const int l = lane_id();
-    const double2 *q = (const double2 *)(Li + min(l, k - 1) * GPR_METRIC_MAX);
+    const double2 *q = (const double2 *)(Li + min(opaque_lane_i32(l), k - 1) * GPR_METRIC_MAX);
     double w = 0.0;
 #pragma unroll
     for (int half = 0; half < 2; half++) {

@@ -101,7 +101,8 @@ k_mh_config_wave(GprMHArgs a) {
         if (tile_idx >= a.ntiles) break;
         const GprTile tile = a.tiles[tile_idx];
         if (wave >= tile.count) continue;
-        const GprDataset d = uni_dataset(a.ds[tile.dataset]);
+        const int tds = uni(tile.dataset);
+        const GprDataset d = uni_dataset(a.ds[tds]);
         const uint32_t chain = (uint32_t)uni(tile.chain0 + wave);
         const int cap = d.cap;
         const float *dx = a.dx + d.off, *dy = a.dy + d.off, *dw = a.dw + d.off;
@@ -127,7 +128,7 @@ k_mh_config_wave(GprMHArgs a) {
 
         // proposal metric of this dataset; a chain whose metric rows are not all constant
         // tokens moves every constant with the identity
-        const GprMetric *mt = (MET && a.metric) ? a.metric + tile.dataset : nullptr;
+        const GprMetric *mt = (MET && a.metric) ? a.metric + tds : nullptr;
         int mk = mt ? uni(mt->k) : 0;
         const int mrow = (l < mk) ? mt->rows[l] : 0;
         if (mk) {

@@ -197,7 +197,7 @@ int gpr_set_metric(gpr_ctx *ctx, int dataset, int k, const int32_t *rows, const 
  * workgroup, and workgroups per CU.  gpr_autotune times every candidate shape of each
  * class present on the context's own chains (steps MH steps each, from the same
  * state, which is restored afterwards) and keeps the fastest, where a candidate must be
- * 1 % faster than an earlier one (fewer points per lane, smaller workgroups, fewer
+ * 1 % faster than an earlier one (fewer points per lane, smaller workgroups, more
  * workgroups per CU) to replace it; with GPR_SCHED_AUTO it
  * also picks the lane or wave schedule.  gpr_set_tuning applies a saved table. */
 typedef struct {
