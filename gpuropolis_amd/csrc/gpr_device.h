@@ -1282,15 +1282,15 @@ __device__ __forceinline__ double wave_loglik_grad(const WaveProg &p, const Grad
     return d.lconst - (folded ? d.wconst * s : s);
 }
 
-// FP64 sum of the Gaussian log-density terms -f^2 / s2 over the lanes where m holds, in
+// FP64 sum of the Gaussian log-density terms -f^2 r2 (r2 = 1 / (2 e^2)) over the lanes where m holds, in
 // token order (lanes of the low half, then the high half), as a serial loop over tokens
 // would add them.  A half without such lanes computes no terms.
 template <bool HI>
-__device__ __forceinline__ double wave_gauss_sum_d(double f0, bool m0, double f1, bool m1, double s2) {
+__device__ __forceinline__ double wave_gauss_sum_d(double f0, bool m0, double f1, bool m1, double r2) {
     double s = 0.0;
     uint64_t b0 = ballot(m0);
     if (b0) {
-        const double v0 = -f0 * f0 / s2;
+        const double v0 = -f0 * f0 * r2;
         while (b0) {
             const int b = __builtin_ctzll(b0);
             b0 &= b0 - 1;
@@ -1300,7 +1300,7 @@ __device__ __forceinline__ double wave_gauss_sum_d(double f0, bool m0, double f1
     if (HI) {
         uint64_t b1 = ballot(m1);
         if (b1) {
-            const double v1 = -f1 * f1 / s2;
+            const double v1 = -f1 * f1 * r2;
             while (b1) {
                 const int b = __builtin_ctzll(b1);
                 b1 &= b1 - 1;

@@ -146,6 +146,9 @@ k_mh_config_wave(GprMHArgs a) {
         // per-lane constant-proposal block, reused by four consecutive steps
         gpr_u4 wlo = {0, 0, 0, 0}, whi = {0, 0, 0, 0};
         uint32_t wblk = 0xffffffffu, whblk = 0xffffffffu;
+        // MALA: the Langevin noise batch (key: step >> lg, and whether it spans 16 steps)
+        float zbm = 0.0f;
+        uint32_t zmblk = 0xffffffffu;
         // Batched random-walk moves (BATCH: no MALA; per program: tape within the low half,
         // no metric, at most 16 constant tokens and no constant-operand token).  The draw of
         // step s for a constant is word s & 3 of the Philox block s >> 2 of its token, so a
@@ -358,21 +361,67 @@ k_mh_config_wave(GprMHArgs a) {
                         }
                     }
                 } else {
-                if (BATCH || (step >> 2) != wblk) {
-                    wblk = step >> 2;
-                    wlo = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 0);
-                }
-                // the high half's block only for programs that reach it
-                const bool hi_live = HI && cur.len > 64;
-                if (hi_live && (BATCH || (step >> 2) != whblk)) {
-                    whblk = step >> 2;
-                    whi = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 1);
-                }
-                const int wi = (int)(step & 3);
                 // identity-metric constants; metric rows move together below
                 const bool i0 = k0 && slot0 < 0, i1 = k1 && slot1 < 0;
-                const float z0 = gpr_normal(gpr_u01(gpr_u4_word(wlo, wi)));
-                const float z1 = hi_live ? gpr_normal(gpr_u01(gpr_u4_word(whi, wi))) : 0.0f;
+                float z0, z1 = 0.0f;
+                const int nk0 = __popcll(mk0), nk = nk0 + (HI ? __popcll(mk1) : 0);
+                if (MALA && !MET && nk <= 16) {
+                    // Langevin noise from a draw batch in the random-walk batch's layout
+                    // (lane ks u + r: step base + u, constant of rank r in token order), one
+                    // normal per lane for 4 steps (16 for at most 4 constants).  The noise
+                    // depends on the steps and the constant tokens only, so the batch holds
+                    // across accepted constant moves, until the structure changes (zmblk is
+                    // reset there).
+                    const bool b16 = nk <= 4;
+                    const int lg = b16 ? 4 : 2, ks = b16 ? 4 : 16;
+                    const uint32_t key = ((step >> lg) << 1) | (b16 ? 1u : 0u);
+                    if (key != zmblk) {
+                        zmblk = key;
+                        const int r = l & (ks - 1);
+                        int tr = 0, q = 0;
+                        for (uint64_t m = mk0; m; q++) {
+                            const int b = __builtin_ctzll(m);
+                            m &= m - 1;
+                            tr = r == q ? b : tr;
+                        }
+                        if (HI) {
+                            for (uint64_t m = mk1; m; q++) {
+                                const int b = __builtin_ctzll(m);
+                                m &= m - 1;
+                                tr = r == q ? 64 + b : tr;
+                            }
+                        }
+                        const uint32_t sj = ((step >> lg) << lg) + (uint32_t)(l >> (6 - lg));
+                        const gpr_u4 wb = gpr_step_block(a.seed, chain, sj, GPR_P_CPROP, (uint32_t)(tr & 63),
+                                                         (uint32_t)(tr >> 6));
+                        zbm = gpr_normal(gpr_u01(gpr_u4_word(wb, (int)(sj & 3))));
+                    }
+                    const int base = ks * (int)(step & ((1u << lg) - 1));
+                    const int rank0 = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk0 >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mk0, 0u));
+                    z0 = shfl_all(zbm, base + rank0);
+                    if (HI && mk1) {
+                        const int rank1 = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mk1 >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mk1, 0u));
+                        z1 = shfl_all(zbm, base + nk0 + rank1);
+                    }
+                } else {
+                    // per-step draws: each lane's block of the four steps (kept across them
+                    // only with a proposal metric; the batch above serves the common case)
+                    if (BATCH || (MALA && !MET) || (step >> 2) != wblk) {
+                        wblk = step >> 2;
+                        wlo = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 0);
+                    }
+                    // the high half's block only for programs with constants there
+                    const bool hi_live = HI && mk1 != 0;
+                    if (hi_live && (BATCH || (MALA && !MET) || (step >> 2) != whblk)) {
+                        whblk = step >> 2;
+                        whi = gpr_step_block(a.seed, chain, step, GPR_P_CPROP, (uint32_t)l, 1);
+                    }
+                    const int wi = (int)(step & 3);
+                    z0 = gpr_normal(gpr_u01(gpr_u4_word(wlo, wi)));
+                    if (hi_live) z1 = gpr_normal(gpr_u01(gpr_u4_word(whi, wi)));
+                }
                 float zr = 0.0f, cr = 0.0f, dr = 0.0f;
                 if (mk) {
                     zr = gather_f(z0, z1, mrow, HI);
@@ -417,12 +466,12 @@ k_mh_config_wave(GprMHArgs a) {
                         grad_ok = true;
                     }
                     const float e = a.eps, h = 0.5f * e * e;
-                    const double inv2e2 = 2.0 * (double)e * (double)e;
+                    const double inv2e2 = 2.0 * (double)e * (double)e, r2 = 1.0 / inv2e2;
                     const float m0 = gpr_axpy(h, cg0, cur.c0), m1 = gpr_axpy(h, cg1, cur.c1);
                     if (i0) prop.c0 = gpr_axpy(e, z0, m0);
                     if (HI && i1) prop.c1 = gpr_axpy(e, z1, m1);
                     const double f0 = (double)prop.c0 - (double)m0, f1 = (double)prop.c1 - (double)m1;
-                    double fq = wave_gauss_sum_d<HI>(f0, i0, f1, i1, inv2e2);
+                    double fq = wave_gauss_sum_d<HI>(f0, i0, f1, i1, r2);
                     float cn = 0.0f;
                     if (mk) {
                         // c' = c + h M g + e L z over the metric rows
@@ -496,7 +545,7 @@ k_mh_config_wave(GprMHArgs a) {
                         pg1 = pok ? (float)h1 : 0.0f;
                         const float b0 = gpr_axpy(h, pg0, prop.c0), b1 = gpr_axpy(h, pg1, prop.c1);
                         const double r0 = (double)cur.c0 - (double)b0, r1 = (double)cur.c1 - (double)b1;
-                        double bq = wave_gauss_sum_d<HI>(r0, i0, r1, i1, inv2e2);
+                        double bq = wave_gauss_sum_d<HI>(r0, i0, r1, i1, r2);
                         if (mk) {
                             pmg = metric_Mv(mM, mk, gather_f(pg0, pg1, mrow, HI));
                             const float mb = gpr_axpy(h, pmg, cn);
@@ -560,6 +609,7 @@ k_mh_config_wave(GprMHArgs a) {
                     c_sa++;
                     grad_ok = false;
                     zblk = 0xffffffffu;
+                    zmblk = 0xffffffffu;
                     cmask = pmask;
                     if (MALA) gplan = grad_plan<HI>(cur);
                     cur_opq = p_opq;

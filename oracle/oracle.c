@@ -979,6 +979,7 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
                 }
                 const float e = cfg->eps, h = 0.5f * e * e;
                 const double two_e2 = 2.0 * (double)e * (double)e;
+                const double r2 = 1.0 / two_e2;     /* the identity rows' terms: -d^2 r2 */
                 double fq = 0.0;
                 for (int t = 0; t < len; t++) {
                     if (ops[t] != GPO_CONST || metric_slot(cfg, K, t) >= 0) continue;
@@ -986,7 +987,7 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
                     float mean = cs[t] + h * (float)grad[t];
                     pcs[t] = mean + e * z;
                     double d = (double)pcs[t] - (double)mean;
-                    fq += -d * d / two_e2;
+                    fq += -d * d * r2;
                 }
                 float mg[16], mv[16], mmean[16];
                 double r[16];
@@ -1006,7 +1007,7 @@ void gpo_cfg_run_chain(const gpo_cfg *cfg, uint32_t chain, uint64_t step0, int s
                     if (ops[t] != GPO_CONST || metric_slot(cfg, K, t) >= 0) continue;
                     float mean = pcs[t] + h * (float)(pg_ok ? pgrad[t] : 0.0);
                     double d = (double)cs[t] - (double)mean;
-                    bq += -d * d / two_e2;
+                    bq += -d * d * r2;
                 }
                 if (K > 0) {
                     for (int i = 0; i < K; i++) mg[i] = (float)(pg_ok ? pgrad[cfg->mrows[i]] : 0.0);
