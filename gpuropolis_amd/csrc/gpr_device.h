@@ -423,7 +423,7 @@ template <int KC>
 struct Chunk {
     float x[KC], y[KC], w[KC];
     __device__ __forceinline__ void load(const float *dx, const float *dy, const float *dw, int n, int base) {
-        const int l = lane_id();
+        const int l = opaque_lane_i32(lane_id());   // the point's offset is computed here, not kept live
 #pragma unroll
         for (int k = 0; k < KC; k++) {
             const int i = base + k * 64 + l;
@@ -545,18 +545,52 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // interpreter pass as their likelihood: every stack entry carries the value and its
 // tangents with respect to the program's constant tokens (constant of rank j: tangent j),
 // W = 1 + NT floats, pushed to the wave's LDS stack as one 16-byte store (NT =
-// GPR_DUAL_NT) or, for programs of a single constant, one 8-byte store (NT = 1).  The
+// GPR_DUAL_NT = 3), one 8-byte store (NT = 1, programs of a single constant), or for
+// programs of four constants (NT = GPR_DUAL_NTMAX) a 16-byte store of the value and the
+// first three tangents and a 4-byte store of the fourth into a second plane.  The
 // values take the same operations on the same operands as wave_eval_ls, so the LL is
 // bit-identical to it (and early rejection stops a pass at the same chunk); each lane
 // adds the per-point gradient (-2 r w) f' of its points, and one wave sum per constant
 // ends the pass.  Programs qualify when they use only the config grammar (x, constant,
-// one, + - * / pow exp log neg), hold at most GPR_DUAL_NT constant tokens and no
+// one, + - * / pow exp log neg), hold at most GPR_DUAL_NTMAX constant tokens and no
 // constant-operand token, and stay within GPR_DUAL_DEPTH stack entries.
 #define GPR_DUAL_NT 3
+#define GPR_DUAL_NTMAX 4
 #define GPR_DUAL_DEPTH 6
 #define GPR_DUAL_W (1 + GPR_DUAL_NT)
-// floats of the dual stack of one wave: GPR_DUAL_DEPTH + 2 levels of 64 lanes x W
-#define GPR_DUAL_LDS_FLOATS ((GPR_DUAL_DEPTH + 2) * 64 * GPR_DUAL_W)
+// Stack levels: a push at stack size sp stores U at level sp - 2 and a binary op at sp
+// loads level sp - 3, so a program of depth at most GPR_DUAL_DEPTH uses levels -2 ..
+// GPR_DUAL_DEPTH - 3, stored from offset 0 (level -2).  The packed layouts reserve
+// GPR_DUAL_DEPTH + 2 levels, the two-plane layout of NT = 4 GPR_DUAL_SPLIT_LEVELS.
+#define GPR_DUAL_SPLIT_LEVELS (GPR_DUAL_DEPTH + 1)
+#define GPR_DUAL_PACKED_FLOATS ((GPR_DUAL_DEPTH + 2) * 64 * GPR_DUAL_W)
+#define GPR_DUAL_SPLIT_FLOATS (GPR_DUAL_SPLIT_LEVELS * 64 * (GPR_DUAL_NTMAX + 1))
+// floats of the dual stack of one wave
+#define GPR_DUAL_LDS_FLOATS \
+    (GPR_DUAL_PACKED_FLOATS > GPR_DUAL_SPLIT_FLOATS ? GPR_DUAL_PACKED_FLOATS : GPR_DUAL_SPLIT_FLOATS)
+
+// One stack entry of the dual stack at level lv >= -2 (lane, laneb: this lane's level 0):
+// packed W floats per lane, or for NT = 4 four floats per lane in the first plane and the
+// last tangent in the second (laneb).
+template <int NT>
+__device__ __forceinline__ void dual_st(float *lane, float *laneb, int lv, const float (&v)[1 + NT]) {
+    if constexpr (NT == GPR_DUAL_NTMAX) {
+        *(float4 *)(lane + lv * 256) = make_float4(v[0], v[1], v[2], v[3]);
+        laneb[lv * 64] = v[NT];
+    } else {
+        ls_store<1 + NT>(lane + lv * 64 * (1 + NT), v);
+    }
+}
+template <int NT>
+__device__ __forceinline__ void dual_ld(const float *lane, const float *laneb, int lv, float (&v)[1 + NT]) {
+    if constexpr (NT == GPR_DUAL_NTMAX) {
+        const float4 f = *(const float4 *)(lane + lv * 256);
+        v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+        v[NT] = laneb[lv * 64];
+    } else {
+        ls_load<1 + NT>(lane + lv * 64 * (1 + NT), v);
+    }
+}
 
 // a^(b-1), the partial derivative of a^b in a over b, given v = a^b: v / a where a is
 // finite and non-zero and v finite on every lane of the wave (a division instead of a
@@ -570,6 +604,9 @@ __device__ __forceinline__ float pow_bm1(float a, float b, float v) {
 // partial of a constant-free operand must not reach the constants, as in the adjoint).
 __device__ __forceinline__ float tmul(float t, float partial) { return t != 0.0f ? t * partial : 0.0f; }
 
+// tangents of the forward-mode pass of a program of k constant tokens
+__device__ __forceinline__ int dual_nt(int k) { return k <= 1 ? 1 : (k <= GPR_DUAL_NT ? GPR_DUAL_NT : GPR_DUAL_NTMAX); }
+
 template <bool HI>
 __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
     constexpr uint32_t GRAMMAR = (1u << GPR_OP_X) | (1u << GPR_OP_CONST) | (1u << GPR_OP_ONE) | (1u << GPR_OP_ADD) |
@@ -578,17 +615,18 @@ __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
     const int l = lane_id();
     const bool v0 = l < p.len, v1 = HI && l + 64 < p.len;
     const bool bad = (v0 && !((GRAMMAR >> (p.o0 & 31)) & 1u)) || (v1 && !((GRAMMAR >> (p.o1 & 31)) & 1u));
-    return ballot(bad) == 0 && const_count<HI>(p) <= GPR_DUAL_NT && dmax <= GPR_DUAL_DEPTH;
+    return ballot(bad) == 0 && const_count<HI>(p) <= GPR_DUAL_NTMAX && dmax <= GPR_DUAL_DEPTH;
 }
 
 // One token (two for a leaf fused with the next +, - or *) of the forward-mode
 // interpreter, tokens of half H of the tape (t < 64 for H = 0): the half is a
 // compile-time constant, so the token-class masks and the tape words are read without
-// per-token selects.  NT tangents (1 for programs of one constant, else GPR_DUAL_NT).
+// per-token selects.  NT tangents (1, GPR_DUAL_NT or GPR_DUAL_NTMAX: at least the
+// program's constant count).  lane, laneb: this lane's level 0 of the stack (dual_st).
 template <int H, int NT>
 __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, const uint64_t (&cm)[2], int nlo,
-                                           float xv, float *lane, float (&T)[1 + NT], float (&U)[1 + NT],
-                                           int &sp, int &t) {
+                                           float xv, float *lane, float *laneb, float (&T)[1 + NT],
+                                           float (&U)[1 + NT], int &sp, int &t) {
     constexpr int W = 1 + NT;
     const int li = t & 63;
     constexpr int h = H;
@@ -616,7 +654,7 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
             t++;
             return;
         }
-        ls_store<W>(lane + (sp - 2) * 64 * W, U);
+        dual_st<NT>(lane, laneb, sp - 2, U);
 #pragma unroll
         for (int j = 0; j < W; j++) U[j] = T[j];
         sp++;
@@ -638,13 +676,13 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 #pragma unroll
             for (int j = 1; j < W; j++) T[j] = __builtin_fmaf(sgn, T[j], U[j]);
         }
-        ls_load<W>(lane + (sp - 3) * 64 * W, U);
+        dual_ld<NT>(lane, laneb, sp - 3, U);
         sp--;
         return;
     }
     const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
     if (op == GPR_OP_ONE) {
-        ls_store<W>(lane + (sp - 2) * 64 * W, U);
+        dual_st<NT>(lane, laneb, sp - 2, U);
 #pragma unroll
         for (int j = 0; j < W; j++) { U[j] = T[j]; T[j] = j == 0 ? 1.0f : 0.0f; }
         sp++;
@@ -655,7 +693,7 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 #pragma unroll
         for (int j = 1; j < W; j++) T[j] = tmul(U[j], ib) + tmul(T[j], nq);
         T[0] = q;
-        ls_load<W>(lane + (sp - 3) * 64 * W, U);
+        dual_ld<NT>(lane, laneb, sp - 3, U);
         sp--;
     } else if (op == GPR_OP_POW) {
         const float a = U[0], b = T[0];
@@ -664,7 +702,7 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 #pragma unroll
         for (int j = 1; j < W; j++) T[j] = tmul(U[j], pa) + tmul(T[j], pb);
         T[0] = v;
-        ls_load<W>(lane + (sp - 3) * 64 * W, U);
+        dual_ld<NT>(lane, laneb, sp - 3, U);
         sp--;
     } else if (op == GPR_OP_EXP) {
         const float v = expf(T[0]);
@@ -686,16 +724,19 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 // LL of the wave's program (as wave_loglik<1, HI, true>) and, when the pass completes,
 // gs[j] = sum over points of (-2 r w) df/dc_j for the constant token of rank j (token
 // order; cm = the constant-token masks).  ls: the wave's dual stack region.
-// NT: tangents carried, at least the program's constant count (1 or GPR_DUAL_NT).
+// NT: tangents carried, at least the program's constant count (1, GPR_DUAL_NT or
+// GPR_DUAL_NTMAX); gs[j] for j >= NT is 0.
 template <bool HI, int NT>
 __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
                                    const float *dw, const Chunk<1> &c0, int nchunks, float *ls, const OpMasks &m,
                                    const uint64_t (&cm)[2], double reject_below, int *points,
-                                   float (&gs)[GPR_DUAL_NT]) {
-    static_assert(NT == 1 || NT == GPR_DUAL_NT, "tangent count");
+                                   float (&gs)[GPR_DUAL_NTMAX]) {
+    static_assert(NT == 1 || NT == GPR_DUAL_NT || NT == GPR_DUAL_NTMAX, "tangent count");
     constexpr int W = 1 + NT;
     const int l = lane_id();
-    float *lane = ls + 2 * 64 * W + l * W;   // level -2 of the stack below T, U
+    // this lane's level 0 of the stack below T, U (dual_st); levels -2 and -1 lie below it
+    float *lane = NT == GPR_DUAL_NTMAX ? ls + 2 * 256 + l * 4 : ls + 2 * 64 * W + l * W;
+    float *laneb = ls + GPR_DUAL_SPLIT_LEVELS * 256 + 2 * 64 + l;
     float acc = 0.0f, comp = 0.0f;
     float ga[NT];
 #pragma unroll
@@ -714,9 +755,9 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
         int sp = 0;
         // tokens of the low half, then (two-register tapes) the high half
         const int lo_end = HI ? min(p.len, 64) : p.len;
-        for (int t = 0; t < lo_end; t++) dual_token<0, NT>(p, m, cm, nlo, xv, lane, T, U, sp, t);
+        for (int t = 0; t < lo_end; t++) dual_token<0, NT>(p, m, cm, nlo, xv, lane, laneb, T, U, sp, t);
         if (HI)
-            for (int t = 64; t < p.len; t++) dual_token<1, NT>(p, m, cm, nlo, xv, lane, T, U, sp, t);
+            for (int t = 64; t < p.len; t++) dual_token<1, NT>(p, m, cm, nlo, xv, lane, laneb, T, U, sp, t);
         const int i = ch * 64 + l;
         const bool valid = i < d.n;
         const float r = T[0] - c.y[0];
@@ -738,7 +779,7 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
         }
     }
 #pragma unroll
-    for (int j = 0; j < GPR_DUAL_NT; j++) gs[j] = (j < NT && j < nlo + (HI ? __popcll(cm[1]) : 0)) ? wave_sum_f(ga[j < NT ? j : 0]) : 0.0f;
+    for (int j = 0; j < GPR_DUAL_NTMAX; j++) gs[j] = (j < NT && j < nlo + (HI ? __popcll(cm[1]) : 0)) ? wave_sum_f(ga[j < NT ? j : 0]) : 0.0f;
     const double s = wave_sum_d((double)acc - (double)comp);
     if (!__builtin_isfinite(s)) return -GPR_INF;
     return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
@@ -749,7 +790,7 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
 // prior's -2c/(1+c^2); ok is false when one of them is not finite.
 template <bool HI>
 __device__ __forceinline__ void dual_grad_lanes(const WaveProg &p, const uint64_t (&cm)[2],
-                                                const float (&gs)[GPR_DUAL_NT], double &g0, double &g1, bool &ok) {
+                                                const float (&gs)[GPR_DUAL_NTMAX], double &g0, double &g1, bool &ok) {
     const int l = lane_id();
     const uint64_t below = (1ull << l) - 1ull;
     const int r0 = __popcll(cm[0] & below);
@@ -757,7 +798,7 @@ __device__ __forceinline__ void dual_grad_lanes(const WaveProg &p, const uint64_
     const bool c0 = (cm[0] >> l) & 1ull, c1 = HI && ((cm[1] >> l) & 1ull);
     float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
-    for (int j = 0; j < GPR_DUAL_NT; j++) {
+    for (int j = 0; j < GPR_DUAL_NTMAX; j++) {
         s0 = r0 == j ? gs[j] : s0;
         s1 = r1 == j ? gs[j] : s1;
     }

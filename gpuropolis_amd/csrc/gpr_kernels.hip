@@ -23,7 +23,7 @@
 // MALA: constant moves are Langevin proposals c' = c + h g(c) + eps z (h = eps^2/2, g the
 // gradient of the log posterior) with the Hastings ratio of the two Gaussian proposal
 // densities.  Under MALA every likelihood is evaluated at one point per lane (KC = 1),
-// with early rejection.  Programs of at most three constants and stack depth 6 get the
+// with early rejection.  Programs of at most four constants and stack depth 6 get the
 // proposal's LL and gradient from one forward-mode interpreter pass (value and tangents
 // per stack entry); the others are pre-checked by the plain interpreter, and the
 // survivors get the stack-free adjoint sweeps, whose token values sit in the wave's LDS
@@ -178,8 +178,9 @@ k_mh_config_wave(GprMHArgs a) {
         // random-walk moves can be batched (see BATCH below)
         // MALA: whether the current program's gradient comes from the forward-mode
         // interpreter (dual_ok), with its constant-token masks
-        // (cur_nt1: a single constant token, whose pass carries one tangent)
-        bool cur_dual = false, cur_nt1 = false;
+        // (cur_nt: tangents of its pass, 1, GPR_DUAL_NT or GPR_DUAL_NTMAX)
+        bool cur_dual = false;
+        int cur_nt = GPR_DUAL_NT;
         uint64_t ccm[2] = {0, 0};
         if (MALA) {
             int D0, D1, dm;
@@ -187,7 +188,7 @@ k_mh_config_wave(GprMHArgs a) {
             cur_dual = !MET && mk == 0 && dual_ok<HI>(cur, dm);
             ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
             ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
-            cur_nt1 = __popcll(ccm[0]) + __popcll(ccm[1]) == 1;
+            cur_nt = dual_nt(__popcll(ccm[0]) + __popcll(ccm[1]));
         }
         int cur_opq, cur_nc;
         prog_counts<HI>(cur, cur_opq, cur_nc);
@@ -445,14 +446,17 @@ k_mh_config_wave(GprMHArgs a) {
                         double h0, h1;
                         bool gok;
                         if (cur_dual) {
-                            float gs[GPR_DUAL_NT];
+                            float gs[GPR_DUAL_NTMAX];
                             int np_;
+                            const Chunk<1> &k0c = *(const Chunk<1> *)&c0;
                             const double cl =
-                                cur_nt1 ? wave_loglik_dual<HI, 1>(cur, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks,
-                                                                  ls, cmask, ccm, -GPR_INF, &np_, gs)
-                                        : wave_loglik_dual<HI, GPR_DUAL_NT>(cur, d, dx, dy, dw,
-                                                                            *(const Chunk<1> *)&c0, nchunks, ls, cmask,
-                                                                            ccm, -GPR_INF, &np_, gs);
+                                cur_nt == 1 ? wave_loglik_dual<HI, 1>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm,
+                                                                      -GPR_INF, &np_, gs)
+                                : cur_nt == GPR_DUAL_NT
+                                    ? wave_loglik_dual<HI, GPR_DUAL_NT>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm,
+                                                                        -GPR_INF, &np_, gs)
+                                    : wave_loglik_dual<HI, GPR_DUAL_NTMAX>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask,
+                                                                           ccm, -GPR_INF, &np_, gs);
                             dual_grad_lanes<HI>(cur, ccm, gs, h0, h1, gok);
                             gok = gok && __builtin_isfinite(cl);
                         } else {
@@ -506,16 +510,19 @@ k_mh_config_wave(GprMHArgs a) {
                     if (__builtin_isfinite(post) && __builtin_isfinite(pre_prior) && __builtin_isfinite(fq))
                         rb = post - pre_prior + fq - 18.0 -
                              1e-12 * (__builtin_fabs(post) + __builtin_fabs(pre_prior) + __builtin_fabs(fq));
-                    float dgs[GPR_DUAL_NT];
+                    float dgs[GPR_DUAL_NTMAX];
                     double h0 = 0.0, h1 = 0.0;
                     bool pok = false;
                     const bool precheck = !MET && !cur_dual && nchunks > 1;
-                    if (cur_dual && cur_nt1)
+                    if (cur_dual && cur_nt == 1)
                         pll_m = wave_loglik_dual<HI, 1>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls,
                                                         cmask, ccm, rb, &mala_pts, dgs);
-                    else if (cur_dual)
+                    else if (cur_dual && cur_nt == GPR_DUAL_NT)
                         pll_m = wave_loglik_dual<HI, GPR_DUAL_NT>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks,
                                                                   ls, cmask, ccm, rb, &mala_pts, dgs);
+                    else if (cur_dual)
+                        pll_m = wave_loglik_dual<HI, GPR_DUAL_NTMAX>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0,
+                                                                     nchunks, ls, cmask, ccm, rb, &mala_pts, dgs);
                     else if (precheck && HI && prop.len <= 64)
                         pll_m = wave_loglik<KC, false, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &cmask, rb,
                                                              &mala_pts);
@@ -621,7 +628,7 @@ k_mh_config_wave(GprMHArgs a) {
                         cur_dual = !MET && mk == 0 && dual_ok<HI>(cur, dm);
                         ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
                         ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
-                        cur_nt1 = __popcll(ccm[0]) + __popcll(ccm[1]) == 1;
+                        cur_nt = dual_nt(__popcll(ccm[0]) + __popcll(ccm[1]));
                     }
                     cur_b16 = cur_bok && const_count<HI>(cur) <= 4;
                 } else {
