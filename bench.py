@@ -47,18 +47,18 @@ HBM_PEAK = 8.0e12
 # were measured on.  For the batched workload a "launch" is one block's six concurrent
 # size-class launches.
 MEASURED_TRAFFIC = {
-    ("full", 1048576, 1000, 0): 2_135_408_128,
-    ("large", 131072, 1000, 0): 164_311_360,
-    ("batched", 262144, 1000, 0): 382_705_216,
+    ("full", 1048576, 1000, 0): 2_387_515_104,
+    ("large", 131072, 1000, 0): 164_723_072,
+    ("batched", 262144, 1000, 0): 384_571_328,
 }
 # VALU and SALU instructions per chain-step of the same launches (rocprofv3 --pmc
 # SQ_INSTS_VALU, SQ_INSTS_SALU over chains x steps).  With the live launch time they give
 # the share of issue slots each unit is busy: one wave64 instruction per SIMD every 4
 # cycles, 1024 SIMDs at 2.4 GHz.
 MEASURED_ISSUE = {
-    ("full", 1048576, 1000, 0): (1180.33, 1037.56),
-    ("large", 131072, 1000, 0): (295.28, 299.06),
-    ("batched", 262144, 1000, 0): (329.16, 304.07),
+    ("full", 1048576, 1000, 0): (1120.30, 939.61),
+    ("large", 131072, 1000, 0): (295.46, 299.06),
+    ("batched", 262144, 1000, 0): (329.79, 303.04),
 }
 
 METRIC = "MH steps/sec + node×datapoint evals/sec (% VALU peak); ESS/sec MALA vs RW"
