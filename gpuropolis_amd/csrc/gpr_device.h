@@ -545,36 +545,49 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // interpreter pass as their likelihood: every stack entry carries the value and its
 // tangents with respect to the program's constant tokens (constant of rank j: tangent j),
 // W = 1 + NT floats, pushed to the wave's LDS stack as one 16-byte store (NT =
-// GPR_DUAL_NT = 3), one 8-byte store (NT = 1, programs of a single constant), or for
-// programs of four constants (NT = GPR_DUAL_NTMAX) a 16-byte store of the value and the
-// first three tangents and a 4-byte store of the fourth into a second plane.  The
+// GPR_DUAL_NT = 3), one 8-byte store (NT = 1, programs of a single constant), for
+// programs of four constants (NT = GPR_DUAL_NT4) a 16-byte store of the value and the
+// first three tangents and a 4-byte store of the fourth into a second plane, and for five
+// or six (NT = GPR_DUAL_NTMAX, stack depth at most GPR_DUAL_DEPTH6) a 16-byte, an 8-byte
+// and a 4-byte store into three planes.  The
 // values take the same operations on the same operands as wave_eval_ls, so the LL is
 // bit-identical to it (and early rejection stops a pass at the same chunk); each lane
 // adds the per-point gradient (-2 r w) f' of its points, and one wave sum per constant
 // ends the pass.  Programs qualify when they use only the config grammar (x, constant,
-// one, + - * / pow exp log neg), hold at most GPR_DUAL_NTMAX constant tokens and no
-// constant-operand token, and stay within GPR_DUAL_DEPTH stack entries.
+// one, + - * / pow exp log neg), hold no constant-operand token, and hold at most
+// GPR_DUAL_NT4 constant tokens within GPR_DUAL_DEPTH stack entries or at most
+// GPR_DUAL_NTMAX within GPR_DUAL_DEPTH6.
 #define GPR_DUAL_NT 3
-#define GPR_DUAL_NTMAX 4
+#define GPR_DUAL_NT4 4
+#define GPR_DUAL_NTMAX 6
+#define GPR_DUAL_DEPTH6 5
 #define GPR_DUAL_DEPTH 6
 #define GPR_DUAL_W (1 + GPR_DUAL_NT)
 // Stack levels: a push at stack size sp stores U at level sp - 2 and a binary op at sp
 // loads level sp - 3, so a program of depth at most GPR_DUAL_DEPTH uses levels -2 ..
 // GPR_DUAL_DEPTH - 3, stored from offset 0 (level -2).  The packed layouts reserve
-// GPR_DUAL_DEPTH + 2 levels, the two-plane layout of NT = 4 GPR_DUAL_SPLIT_LEVELS.
+// GPR_DUAL_DEPTH + 2 levels, the two-plane layout of NT = 4 GPR_DUAL_SPLIT_LEVELS, the
+// three-plane layout of NT = 6 exactly the GPR_DUAL_DEPTH6 levels its programs use.
 #define GPR_DUAL_SPLIT_LEVELS (GPR_DUAL_DEPTH + 1)
 #define GPR_DUAL_PACKED_FLOATS ((GPR_DUAL_DEPTH + 2) * 64 * GPR_DUAL_W)
-#define GPR_DUAL_SPLIT_FLOATS (GPR_DUAL_SPLIT_LEVELS * 64 * (GPR_DUAL_NTMAX + 1))
+#define GPR_DUAL_SPLIT_FLOATS (GPR_DUAL_SPLIT_LEVELS * 64 * (GPR_DUAL_NT4 + 1))
+#define GPR_DUAL_SPLIT6_FLOATS (GPR_DUAL_DEPTH6 * 64 * (GPR_DUAL_NTMAX + 1))
 // floats of the dual stack of one wave
 #define GPR_DUAL_LDS_FLOATS \
-    (GPR_DUAL_PACKED_FLOATS > GPR_DUAL_SPLIT_FLOATS ? GPR_DUAL_PACKED_FLOATS : GPR_DUAL_SPLIT_FLOATS)
+    (GPR_DUAL_PACKED_FLOATS > GPR_DUAL_SPLIT_FLOATS                                      \
+         ? (GPR_DUAL_PACKED_FLOATS > GPR_DUAL_SPLIT6_FLOATS ? GPR_DUAL_PACKED_FLOATS : GPR_DUAL_SPLIT6_FLOATS) \
+         : (GPR_DUAL_SPLIT_FLOATS > GPR_DUAL_SPLIT6_FLOATS ? GPR_DUAL_SPLIT_FLOATS : GPR_DUAL_SPLIT6_FLOATS))
 
-// One stack entry of the dual stack at level lv >= -2 (lane, laneb: this lane's level 0):
-// packed W floats per lane, or for NT = 4 four floats per lane in the first plane and the
-// last tangent in the second (laneb).
+// One stack entry of the dual stack at level lv >= -2 (lane, laneb, lanec: this lane's
+// level 0 in each plane): packed W floats per lane; for NT = 4 four floats per lane in the
+// first plane and the last tangent in laneb's; for NT = 6 also two in lanec's.
 template <int NT>
-__device__ __forceinline__ void dual_st(float *lane, float *laneb, int lv, const float (&v)[1 + NT]) {
+__device__ __forceinline__ void dual_st(float *lane, float *laneb, float *lanec, int lv, const float (&v)[1 + NT]) {
     if constexpr (NT == GPR_DUAL_NTMAX) {
+        *(float4 *)(lane + lv * 256) = make_float4(v[0], v[1], v[2], v[3]);
+        *(float2 *)(lanec + lv * 128) = make_float2(v[4], v[5]);
+        laneb[lv * 64] = v[NT];
+    } else if constexpr (NT == GPR_DUAL_NT4) {
         *(float4 *)(lane + lv * 256) = make_float4(v[0], v[1], v[2], v[3]);
         laneb[lv * 64] = v[NT];
     } else {
@@ -582,8 +595,15 @@ __device__ __forceinline__ void dual_st(float *lane, float *laneb, int lv, const
     }
 }
 template <int NT>
-__device__ __forceinline__ void dual_ld(const float *lane, const float *laneb, int lv, float (&v)[1 + NT]) {
+__device__ __forceinline__ void dual_ld(const float *lane, const float *laneb, const float *lanec, int lv,
+                                        float (&v)[1 + NT]) {
     if constexpr (NT == GPR_DUAL_NTMAX) {
+        const float4 f = *(const float4 *)(lane + lv * 256);
+        const float2 e = *(const float2 *)(lanec + lv * 128);
+        v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+        v[4] = e.x; v[5] = e.y;
+        v[NT] = laneb[lv * 64];
+    } else if constexpr (NT == GPR_DUAL_NT4) {
         const float4 f = *(const float4 *)(lane + lv * 256);
         v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
         v[NT] = laneb[lv * 64];
@@ -605,7 +625,9 @@ __device__ __forceinline__ float pow_bm1(float a, float b, float v) {
 __device__ __forceinline__ float tmul(float t, float partial) { return t != 0.0f ? t * partial : 0.0f; }
 
 // tangents of the forward-mode pass of a program of k constant tokens
-__device__ __forceinline__ int dual_nt(int k) { return k <= 1 ? 1 : (k <= GPR_DUAL_NT ? GPR_DUAL_NT : GPR_DUAL_NTMAX); }
+__device__ __forceinline__ int dual_nt(int k) {
+    return k <= 1 ? 1 : (k <= GPR_DUAL_NT ? GPR_DUAL_NT : (k <= GPR_DUAL_NT4 ? GPR_DUAL_NT4 : GPR_DUAL_NTMAX));
+}
 
 template <bool HI>
 __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
@@ -615,17 +637,18 @@ __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
     const int l = lane_id();
     const bool v0 = l < p.len, v1 = HI && l + 64 < p.len;
     const bool bad = (v0 && !((GRAMMAR >> (p.o0 & 31)) & 1u)) || (v1 && !((GRAMMAR >> (p.o1 & 31)) & 1u));
-    return ballot(bad) == 0 && const_count<HI>(p) <= GPR_DUAL_NTMAX && dmax <= GPR_DUAL_DEPTH;
+    const int k = const_count<HI>(p);
+    return ballot(bad) == 0 && ((k <= GPR_DUAL_NT4 && dmax <= GPR_DUAL_DEPTH) || (k <= GPR_DUAL_NTMAX && dmax <= GPR_DUAL_DEPTH6));
 }
 
 // One token (two for a leaf fused with the next +, - or *) of the forward-mode
 // interpreter, tokens of half H of the tape (t < 64 for H = 0): the half is a
 // compile-time constant, so the token-class masks and the tape words are read without
-// per-token selects.  NT tangents (1, GPR_DUAL_NT or GPR_DUAL_NTMAX: at least the
+// per-token selects.  NT tangents (1, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NTMAX: at least the
 // program's constant count).  lane, laneb: this lane's level 0 of the stack (dual_st).
 template <int H, int NT>
 __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, const uint64_t (&cm)[2], int nlo,
-                                           float xv, float *lane, float *laneb, float (&T)[1 + NT],
+                                           float xv, float *lane, float *laneb, float *lanec, float (&T)[1 + NT],
                                            float (&U)[1 + NT], int &sp, int &t) {
     constexpr int W = 1 + NT;
     const int li = t & 63;
@@ -654,7 +677,7 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
             t++;
             return;
         }
-        dual_st<NT>(lane, laneb, sp - 2, U);
+        dual_st<NT>(lane, laneb, lanec, sp - 2, U);
 #pragma unroll
         for (int j = 0; j < W; j++) U[j] = T[j];
         sp++;
@@ -676,13 +699,13 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 #pragma unroll
             for (int j = 1; j < W; j++) T[j] = __builtin_fmaf(sgn, T[j], U[j]);
         }
-        dual_ld<NT>(lane, laneb, sp - 3, U);
+        dual_ld<NT>(lane, laneb, lanec, sp - 3, U);
         sp--;
         return;
     }
     const int op = h ? rl_i(p.o1, li) : rl_i(p.o0, li);
     if (op == GPR_OP_ONE) {
-        dual_st<NT>(lane, laneb, sp - 2, U);
+        dual_st<NT>(lane, laneb, lanec, sp - 2, U);
 #pragma unroll
         for (int j = 0; j < W; j++) { U[j] = T[j]; T[j] = j == 0 ? 1.0f : 0.0f; }
         sp++;
@@ -693,7 +716,7 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 #pragma unroll
         for (int j = 1; j < W; j++) T[j] = tmul(U[j], ib) + tmul(T[j], nq);
         T[0] = q;
-        dual_ld<NT>(lane, laneb, sp - 3, U);
+        dual_ld<NT>(lane, laneb, lanec, sp - 3, U);
         sp--;
     } else if (op == GPR_OP_POW) {
         const float a = U[0], b = T[0];
@@ -702,7 +725,7 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 #pragma unroll
         for (int j = 1; j < W; j++) T[j] = tmul(U[j], pa) + tmul(T[j], pb);
         T[0] = v;
-        dual_ld<NT>(lane, laneb, sp - 3, U);
+        dual_ld<NT>(lane, laneb, lanec, sp - 3, U);
         sp--;
     } else if (op == GPR_OP_EXP) {
         const float v = expf(T[0]);
@@ -724,19 +747,23 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 // LL of the wave's program (as wave_loglik<1, HI, true>) and, when the pass completes,
 // gs[j] = sum over points of (-2 r w) df/dc_j for the constant token of rank j (token
 // order; cm = the constant-token masks).  ls: the wave's dual stack region.
-// NT: tangents carried, at least the program's constant count (1, GPR_DUAL_NT or
-// GPR_DUAL_NTMAX); gs[j] for j >= NT is 0.
+// NT: tangents carried, at least the program's constant count (1, GPR_DUAL_NT,
+// GPR_DUAL_NT4 or GPR_DUAL_NTMAX); gs[j] for j >= NT is 0.
 template <bool HI, int NT>
 __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
                                    const float *dw, const Chunk<1> &c0, int nchunks, float *ls, const OpMasks &m,
                                    const uint64_t (&cm)[2], double reject_below, int *points,
                                    float (&gs)[GPR_DUAL_NTMAX]) {
-    static_assert(NT == 1 || NT == GPR_DUAL_NT || NT == GPR_DUAL_NTMAX, "tangent count");
+    static_assert(NT == 1 || NT == GPR_DUAL_NT || NT == GPR_DUAL_NT4 || NT == GPR_DUAL_NTMAX, "tangent count");
     constexpr int W = 1 + NT;
     const int l = lane_id();
     // this lane's level 0 of the stack below T, U (dual_st); levels -2 and -1 lie below it
-    float *lane = NT == GPR_DUAL_NTMAX ? ls + 2 * 256 + l * 4 : ls + 2 * 64 * W + l * W;
-    float *laneb = ls + GPR_DUAL_SPLIT_LEVELS * 256 + 2 * 64 + l;
+    // NT = 4: planes of 4 and 1 floats, GPR_DUAL_SPLIT_LEVELS levels each; NT = 6: planes of
+    // 4, 2 and 1 floats, GPR_DUAL_DEPTH6 levels each
+    constexpr int LV = NT == GPR_DUAL_NTMAX ? GPR_DUAL_DEPTH6 : GPR_DUAL_SPLIT_LEVELS;
+    float *lane = NT >= GPR_DUAL_NT4 ? ls + 2 * 256 + l * 4 : ls + 2 * 64 * W + l * W;
+    float *lanec = ls + LV * 256 + 2 * 128 + l * 2;
+    float *laneb = ls + LV * (NT == GPR_DUAL_NTMAX ? 384 : 256) + 2 * 64 + l;
     float acc = 0.0f, comp = 0.0f;
     float ga[NT];
 #pragma unroll
@@ -755,9 +782,9 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
         int sp = 0;
         // tokens of the low half, then (two-register tapes) the high half
         const int lo_end = HI ? min(p.len, 64) : p.len;
-        for (int t = 0; t < lo_end; t++) dual_token<0, NT>(p, m, cm, nlo, xv, lane, laneb, T, U, sp, t);
+        for (int t = 0; t < lo_end; t++) dual_token<0, NT>(p, m, cm, nlo, xv, lane, laneb, lanec, T, U, sp, t);
         if (HI)
-            for (int t = 64; t < p.len; t++) dual_token<1, NT>(p, m, cm, nlo, xv, lane, laneb, T, U, sp, t);
+            for (int t = 64; t < p.len; t++) dual_token<1, NT>(p, m, cm, nlo, xv, lane, laneb, lanec, T, U, sp, t);
         const int i = ch * 64 + l;
         const bool valid = i < d.n;
         const float r = T[0] - c.y[0];

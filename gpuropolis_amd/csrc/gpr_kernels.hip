@@ -23,9 +23,10 @@
 // MALA: constant moves are Langevin proposals c' = c + h g(c) + eps z (h = eps^2/2, g the
 // gradient of the log posterior) with the Hastings ratio of the two Gaussian proposal
 // densities.  Under MALA every likelihood is evaluated at one point per lane (KC = 1),
-// with early rejection.  Programs of at most four constants and stack depth 6 get the
-// proposal's LL and gradient from one forward-mode interpreter pass (value and tangents
-// per stack entry); the others are pre-checked by the plain interpreter, and the
+// with early rejection.  Programs of at most four constants and stack depth 6, or six
+// constants and depth 5, get the proposal's LL and gradient from one forward-mode
+// interpreter pass (value and tangents per stack entry); the others are pre-checked by
+// the plain interpreter, and the
 // survivors get the stack-free adjoint sweeps, whose token values sit in the wave's LDS
 // region for programs of at most GPR_MALA_LDS_ROWS tokens and in the wave's slab in
 // global memory for longer ones (LDS rows for every token of a 128-token cap, 32 KB per
@@ -178,7 +179,7 @@ k_mh_config_wave(GprMHArgs a) {
         // random-walk moves can be batched (see BATCH below)
         // MALA: whether the current program's gradient comes from the forward-mode
         // interpreter (dual_ok), with its constant-token masks
-        // (cur_nt: tangents of its pass, 1, GPR_DUAL_NT or GPR_DUAL_NTMAX)
+        // (cur_nt: tangents of its pass, 1, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NTMAX)
         bool cur_dual = false;
         int cur_nt = GPR_DUAL_NT;
         uint64_t ccm[2] = {0, 0};
@@ -455,6 +456,9 @@ k_mh_config_wave(GprMHArgs a) {
                                 : cur_nt == GPR_DUAL_NT
                                     ? wave_loglik_dual<HI, GPR_DUAL_NT>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm,
                                                                         -GPR_INF, &np_, gs)
+                                : cur_nt == GPR_DUAL_NT4
+                                    ? wave_loglik_dual<HI, GPR_DUAL_NT4>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask,
+                                                                         ccm, -GPR_INF, &np_, gs)
                                     : wave_loglik_dual<HI, GPR_DUAL_NTMAX>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask,
                                                                            ccm, -GPR_INF, &np_, gs);
                             dual_grad_lanes<HI>(cur, ccm, gs, h0, h1, gok);
@@ -520,6 +524,9 @@ k_mh_config_wave(GprMHArgs a) {
                     else if (cur_dual && cur_nt == GPR_DUAL_NT)
                         pll_m = wave_loglik_dual<HI, GPR_DUAL_NT>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks,
                                                                   ls, cmask, ccm, rb, &mala_pts, dgs);
+                    else if (cur_dual && cur_nt == GPR_DUAL_NT4)
+                        pll_m = wave_loglik_dual<HI, GPR_DUAL_NT4>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0,
+                                                                   nchunks, ls, cmask, ccm, rb, &mala_pts, dgs);
                     else if (cur_dual)
                         pll_m = wave_loglik_dual<HI, GPR_DUAL_NTMAX>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0,
                                                                      nchunks, ls, cmask, ccm, rb, &mala_pts, dgs);

@@ -229,34 +229,36 @@ def test_mala_long_programs_shadow_oracle(oracle, gpr):
 
 @pytest.mark.parametrize("cap", [64, 128])
 def test_mala_forward_mode_programs_shadow_oracle(oracle, gpr, cap):
-    """Programs of one to four constants within stack depth 6 take the forward-mode pass
-    (one tangent for one constant, three for two or three, four in the two-plane stack for
-    four); five constants take the adjoint sweeps.  Sums of c_j f_j(x) over 150 points
-    (three chunks), at cap 128 half of them padded past 64 tokens with (x - x) terms,
-    shadow the oracle's MALA chains in every group."""
+    """Programs of one to six constants take the forward-mode pass (one tangent for one
+    constant, three for two or three, four in the two-plane stack for four, six in the
+    three-plane stack for five or six within stack depth 5); seven constants take the
+    adjoint sweeps.  Sums of c_j f_j(x) over 150 points (three chunks), at cap 128 half of
+    them padded past 64 tokens with (x - x) terms, shadow the oracle's MALA chains in
+    every group."""
     rng = np.random.default_rng(29)
     n = 150
     x = rng.uniform(0.5, 2.0, n).astype(np.float32)
     y = (0.7 * x - 0.3 * x ** 2 + 0.5 / (x + 1) + 0.2 * np.exp(-x) + 0.1 * np.log(x)
-         + rng.normal(0.0, 0.05, n)).astype(np.float32)
+         + 0.05 * x ** 3 - 0.02 * np.exp(x) + rng.normal(0.0, 0.05, n)).astype(np.float32)
     sd = np.full(n, 0.05, dtype=np.float32)
-    terms = [["X"], ["X", "X", "MUL"], None, ["X", "NEG", "EXP"], ["X", "LOG"]]
-    nch, steps, eps = 60, 40, 2e-3
+    terms = [["X"], ["X", "X", "MUL"], None, ["X", "NEG", "EXP"], ["X", "LOG"], ["X", "X", "X", "MUL", "MUL"],
+             ["X", "EXP"]]
+    nch, steps, eps = 70, 40, 2e-3
     O = np.zeros((cap, nch), dtype=np.uint8)
     V = np.zeros((cap, nch), dtype=np.float32)
     L = np.zeros(nch, dtype=np.int32)
     K = np.zeros(nch, dtype=np.int32)
     for c in range(nch):
-        k = 1 + c % 5
+        k = 1 + c % 7
         toks = []
         for j in range(k):
-            cv = ("CONST", float(np.float32((0.7, -0.3, 0.5, 0.2, 0.1)[j] * (1 + 0.05 * rng.normal()))))
+            cv = ("CONST", float(np.float32((0.7, -0.3, 0.5, 0.2, 0.1, 0.05, -0.02)[j] * (1 + 0.05 * rng.normal()))))
             if j == 2:
                 toks += [cv, "X", "ONE", "ADD", "DIV"]          # c / (x + 1)
             else:
                 toks += [cv] + terms[j] + ["MUL"]
             toks += ["ADD"] if j else []
-        if cap == 128 and (c // 5) % 2 == 1:
+        if cap == 128 and (c // 7) % 2 == 1:
             while len(toks) < 72:
                 toks += ["X", "X", "SUB", "ADD"]
         o, v = oracle.tape_arrays(toks)
@@ -273,14 +275,14 @@ def test_mala_forward_mode_programs_shadow_oracle(oracle, gpr, cap):
                      resample=gpr.RS_NONE) as ctx:
         ctx.set_data([(x, y, sd)], [nch])
         ctx.write_state(O, V, L)
-        say(f"mala forward-mode programs: cap {cap}, {nch} chains, 1-5 constants, lengths "
+        say(f"mala forward-mode programs: cap {cap}, {nch} chains, 1-7 constants, lengths "
             f"{L.min()}..{L.max()}, {steps} steps")
         ctx.run_block(steps)
         g_cs = ctx.read_state()[1]
         st = ctx.stats()
     rel = np.max(np.abs(g_cs.astype(np.float64) - o_cs) / np.maximum(np.abs(o_cs.astype(np.float64)), 1e-3), axis=0)
     close = rel < 1e-3
-    for k in range(1, 6):
+    for k in range(1, 8):
         say(f"  {k} constants: {int(close[K == k].sum())} of {int((K == k).sum())} within 1e-3")
         assert close[K == k].sum() >= int(0.9 * (K == k).sum())
     say("acceptance gpu", st.const_accepted / st.const_proposed, "oracle", cnt[3] / max(cnt[2], 1))
