@@ -545,12 +545,12 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // interpreter pass as their likelihood: every stack entry carries the value and its
 // tangents with respect to the program's constant tokens (constant of rank j: tangent j),
 // W = 1 + NT floats, pushed to the wave's LDS stack as one 16-byte store (NT =
-// GPR_DUAL_NT = 3), one 8-byte store (NT = 1, programs of a single constant), for
-// programs of four constants (NT = GPR_DUAL_NT4) a 16-byte store of the value and the
-// first three tangents and a 4-byte store of the fourth into a second plane, and for five
-// or six (NT = GPR_DUAL_NTMAX, stack depth at most GPR_DUAL_DEPTH6) a 16-byte, an 8-byte
-// and a 4-byte store into three planes.  The
-// values take the same operations on the same operands as wave_eval_ls, so the LL is
+// GPR_DUAL_NT = 3, and NT = 2 in the same layout), one 8-byte store (NT = 1, programs of a
+// single constant), for programs of four constants (NT = GPR_DUAL_NT4) a 16-byte store of
+// the value and the first three tangents and a 4-byte store of the fourth into a second
+// plane, and for five or six (NT = GPR_DUAL_NTMAX, stack depth at most GPR_DUAL_DEPTH6) a
+// 16-byte, an 8-byte and a 4-byte store into three planes.  The values take the same
+// operations on the same operands as wave_eval_ls, so the LL is
 // bit-identical to it (and early rejection stops a pass at the same chunk); each lane
 // adds the per-point gradient (-2 r w) f' of its points, and one wave sum per constant
 // ends the pass.  Programs qualify when they use only the config grammar (x, constant,
@@ -590,6 +590,8 @@ __device__ __forceinline__ void dual_st(float *lane, float *laneb, float *lanec,
     } else if constexpr (NT == GPR_DUAL_NT4) {
         *(float4 *)(lane + lv * 256) = make_float4(v[0], v[1], v[2], v[3]);
         laneb[lv * 64] = v[NT];
+    } else if constexpr (NT == 2) {
+        *(float4 *)(lane + lv * 256) = make_float4(v[0], v[1], v[NT], 0.0f);   // the NT = 3 layout
     } else {
         ls_store<1 + NT>(lane + lv * 64 * (1 + NT), v);
     }
@@ -607,6 +609,9 @@ __device__ __forceinline__ void dual_ld(const float *lane, const float *laneb, c
         const float4 f = *(const float4 *)(lane + lv * 256);
         v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
         v[NT] = laneb[lv * 64];
+    } else if constexpr (NT == 2) {
+        const float4 f = *(const float4 *)(lane + lv * 256);
+        v[0] = f.x; v[1] = f.y; v[NT] = f.z;
     } else {
         ls_load<1 + NT>(lane + lv * 64 * (1 + NT), v);
     }
@@ -626,7 +631,7 @@ __device__ __forceinline__ float tmul(float t, float partial) { return t != 0.0f
 
 // tangents of the forward-mode pass of a program of k constant tokens
 __device__ __forceinline__ int dual_nt(int k) {
-    return k <= 1 ? 1 : (k <= GPR_DUAL_NT ? GPR_DUAL_NT : (k <= GPR_DUAL_NT4 ? GPR_DUAL_NT4 : GPR_DUAL_NTMAX));
+    return k <= 2 ? (k <= 1 ? 1 : 2) : (k <= GPR_DUAL_NT ? GPR_DUAL_NT : (k <= GPR_DUAL_NT4 ? GPR_DUAL_NT4 : GPR_DUAL_NTMAX));
 }
 
 template <bool HI>
@@ -644,7 +649,7 @@ __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
 // One token (two for a leaf fused with the next +, - or *) of the forward-mode
 // interpreter, tokens of half H of the tape (t < 64 for H = 0): the half is a
 // compile-time constant, so the token-class masks and the tape words are read without
-// per-token selects.  NT tangents (1, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NTMAX: at least the
+// per-token selects.  NT tangents (1, 2, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NTMAX: at least the
 // program's constant count).  lane, laneb: this lane's level 0 of the stack (dual_st).
 template <int H, int NT>
 __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, const uint64_t (&cm)[2], int nlo,
@@ -747,21 +752,21 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 // LL of the wave's program (as wave_loglik<1, HI, true>) and, when the pass completes,
 // gs[j] = sum over points of (-2 r w) df/dc_j for the constant token of rank j (token
 // order; cm = the constant-token masks).  ls: the wave's dual stack region.
-// NT: tangents carried, at least the program's constant count (1, GPR_DUAL_NT,
+// NT: tangents carried, at least the program's constant count (1, 2, GPR_DUAL_NT,
 // GPR_DUAL_NT4 or GPR_DUAL_NTMAX); gs[j] for j >= NT is 0.
 template <bool HI, int NT>
 __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
                                    const float *dw, const Chunk<1> &c0, int nchunks, float *ls, const OpMasks &m,
                                    const uint64_t (&cm)[2], double reject_below, int *points,
                                    float (&gs)[GPR_DUAL_NTMAX]) {
-    static_assert(NT == 1 || NT == GPR_DUAL_NT || NT == GPR_DUAL_NT4 || NT == GPR_DUAL_NTMAX, "tangent count");
+    static_assert(NT == 1 || NT == 2 || NT == GPR_DUAL_NT || NT == GPR_DUAL_NT4 || NT == GPR_DUAL_NTMAX, "tangent count");
     constexpr int W = 1 + NT;
     const int l = lane_id();
     // this lane's level 0 of the stack below T, U (dual_st); levels -2 and -1 lie below it
     // NT = 4: planes of 4 and 1 floats, GPR_DUAL_SPLIT_LEVELS levels each; NT = 6: planes of
     // 4, 2 and 1 floats, GPR_DUAL_DEPTH6 levels each
     constexpr int LV = NT == GPR_DUAL_NTMAX ? GPR_DUAL_DEPTH6 : GPR_DUAL_SPLIT_LEVELS;
-    float *lane = NT >= GPR_DUAL_NT4 ? ls + 2 * 256 + l * 4 : ls + 2 * 64 * W + l * W;
+    float *lane = (NT >= GPR_DUAL_NT4 || NT == 2) ? ls + 2 * 256 + l * 4 : ls + 2 * 64 * W + l * W;
     float *lanec = ls + LV * 256 + 2 * 128 + l * 2;
     float *laneb = ls + LV * (NT == GPR_DUAL_NTMAX ? 384 : 256) + 2 * 64 + l;
     float acc = 0.0f, comp = 0.0f;

@@ -179,7 +179,7 @@ k_mh_config_wave(GprMHArgs a) {
         // random-walk moves can be batched (see BATCH below)
         // MALA: whether the current program's gradient comes from the forward-mode
         // interpreter (dual_ok), with its constant-token masks
-        // (cur_nt: tangents of its pass, 1, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NTMAX)
+        // (cur_nt: tangents of its pass, 1, 2, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NTMAX)
         bool cur_dual = false;
         int cur_nt = GPR_DUAL_NT;
         uint64_t ccm[2] = {0, 0};
@@ -453,6 +453,8 @@ k_mh_config_wave(GprMHArgs a) {
                             const double cl =
                                 cur_nt == 1 ? wave_loglik_dual<HI, 1>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm,
                                                                       -GPR_INF, &np_, gs)
+                                : cur_nt == 2 ? wave_loglik_dual<HI, 2>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm,
+                                                                        -GPR_INF, &np_, gs)
                                 : cur_nt == GPR_DUAL_NT
                                     ? wave_loglik_dual<HI, GPR_DUAL_NT>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm,
                                                                         -GPR_INF, &np_, gs)
@@ -520,6 +522,9 @@ k_mh_config_wave(GprMHArgs a) {
                     const bool precheck = !MET && !cur_dual && nchunks > 1;
                     if (cur_dual && cur_nt == 1)
                         pll_m = wave_loglik_dual<HI, 1>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls,
+                                                        cmask, ccm, rb, &mala_pts, dgs);
+                    else if (cur_dual && cur_nt == 2)
+                        pll_m = wave_loglik_dual<HI, 2>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls,
                                                         cmask, ccm, rb, &mala_pts, dgs);
                     else if (cur_dual && cur_nt == GPR_DUAL_NT)
                         pll_m = wave_loglik_dual<HI, GPR_DUAL_NT>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks,
