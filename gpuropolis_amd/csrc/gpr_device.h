@@ -626,8 +626,14 @@ __device__ __forceinline__ float pow_bm1(float a, float b, float v) {
 }
 
 // A zero tangent contributes nothing, whatever the partial derivative (an infinite or NaN
-// partial of a constant-free operand must not reach the constants, as in the adjoint).
-__device__ __forceinline__ float tmul(float t, float partial) { return t != 0.0f ? t * partial : 0.0f; }
+// partial of a constant-free operand must not reach the constants, as in the adjoint):
+// the vector unit's legacy multiply, whose product is +0 when either factor is 0 and the
+// IEEE product otherwise, one instruction for the test, the product and the select.
+__device__ __forceinline__ float tmul(float t, float partial) {
+    float r;
+    __asm__("v_mul_legacy_f32 %0, %1, %2" : "=v"(r) : "v"(t), "v"(partial));
+    return r;
+}
 
 // tangents of the forward-mode pass of a program of k constant tokens
 __device__ __forceinline__ int dual_nt(int k) {

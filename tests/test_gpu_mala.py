@@ -288,3 +288,42 @@ def test_mala_forward_mode_programs_shadow_oracle(oracle, gpr, cap):
     say("acceptance gpu", st.const_accepted / st.const_proposed, "oracle", cnt[3] / max(cnt[2], 1))
     assert cnt[3] > 0.2 * cnt[2]
     assert abs(st.const_accepted - cnt[3]) <= 0.1 * cnt[3] + 5
+
+
+def test_mala_zero_tangent_meets_infinite_partial(oracle, gpr):
+    """c * x^(1/2) with x = 0 among the data: at x = 0 the partial of x^(1/2) in x is
+    infinite and its partial in the exponent is NaN, but neither operand carries a
+    tangent of c, so the forward-mode gradient must stay finite (a zero tangent times any
+    partial is zero) and the chains move as the oracle's do."""
+    rng = np.random.default_rng(41)
+    n = 80
+    x = np.concatenate([[0.0, 0.0], rng.uniform(0.0, 2.0, n - 2)]).astype(np.float32)
+    y = (1.7 * np.sqrt(x) + rng.normal(0.0, 0.05, n)).astype(np.float32)
+    sd = np.full(n, 0.05, dtype=np.float32)
+    nch, steps, eps, cap = 32, 40, 2e-3, 64
+    O = np.zeros((cap, nch), dtype=np.uint8)
+    V = np.zeros((cap, nch), dtype=np.float32)
+    L = np.zeros(nch, dtype=np.int32)
+    for c in range(nch):
+        o, v = oracle.tape_arrays([("CONST", float(np.float32(1.7 * (1 + 0.05 * rng.normal())))), "X", "ONE", "ONE",
+                                   "ONE", "ADD", "DIV", "POW", "MUL"])
+        O[:len(o), c] = o
+        V[:len(o), c] = v
+        L[c] = len(o)
+    cfg = oracle.make_cfg(cap=cap, proposal=1, eps=eps, fixed_structure=1, seed=43)
+    o_ops, o_cs, o_ln = O.copy(), V.copy(), L.copy()
+    _, _, cnt = oracle.cfg_run_block(cfg, o_ops, o_cs, o_ln, x, y, sd, 0, steps)
+    with gpr.Context(cap=cap, seed=43, proposal=gpr.PROP_MALA, eps=eps, fixed_structure=1,
+                     resample=gpr.RS_NONE) as ctx:
+        ctx.set_data([(x, y, sd)], [nch])
+        ctx.write_state(O, V, L)
+        say(f"mala zero tangent x infinite partial: {nch} chains, {steps} steps")
+        ctx.run_block(steps)
+        g_cs = ctx.read_state()[1]
+        st = ctx.stats()
+    rel = np.abs(g_cs[0].astype(np.float64) - o_cs[0]) / np.abs(o_cs[0].astype(np.float64))
+    say("chains within 1e-3 of the oracle's:", int((rel < 1e-3).sum()), "of", nch, "acceptance gpu",
+        st.const_accepted / st.const_proposed, "oracle", cnt[3] / max(cnt[2], 1))
+    assert cnt[3] > 0.3 * cnt[2]
+    assert (rel < 1e-3).sum() >= int(0.9 * nch)
+    assert abs(st.const_accepted - cnt[3]) <= 0.1 * cnt[3] + 5
