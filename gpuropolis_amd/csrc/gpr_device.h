@@ -325,16 +325,31 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
         // a leaf followed by +, - or * within the same half: push and pop cancel, so
         // T = T op leaf with the stack untouched (the same operation on the same operands)
         if (m.leaf[h] & (m.cheap[h] >> 1) & bit) {
-            const float cv = h ? rl_f(p.c1, l) : rl_f(p.c0, l);
-            const bool isx = (m.x[h] & bit) != 0;
-            const bool mul = ((m.mul[h] >> 1) & bit) != 0;
-            const float sgn = ((m.sub[h] >> 1) & bit) ? -1.0f : 1.0f;
+            // the leaf and the op are wave-uniform: scalar branches pick one vector
+            // instruction per point (the same operation on the same operands)
+            if (m.x[h] & bit) {
+                if ((m.mul[h] >> 1) & bit) {
 #pragma unroll
-            for (int k = 0; k < KC; k++) {
-                const float b = isx ? xs[k] : cv;
-                const float pm = __builtin_fmaf(sgn, b, T[k]);
-                const float pr = T[k] * b;
-                T[k] = mul ? pr : pm;
+                    for (int k = 0; k < KC; k++) T[k] = T[k] * xs[k];
+                } else if ((m.sub[h] >> 1) & bit) {
+#pragma unroll
+                    for (int k = 0; k < KC; k++) T[k] = __builtin_fmaf(-1.0f, xs[k], T[k]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < KC; k++) T[k] = __builtin_fmaf(1.0f, xs[k], T[k]);
+                }
+            } else {
+                const float cv = h ? rl_f(p.c1, l) : rl_f(p.c0, l);
+                if ((m.mul[h] >> 1) & bit) {
+#pragma unroll
+                    for (int k = 0; k < KC; k++) T[k] = T[k] * cv;
+                } else if ((m.sub[h] >> 1) & bit) {
+#pragma unroll
+                    for (int k = 0; k < KC; k++) T[k] = __builtin_fmaf(-1.0f, cv, T[k]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < KC; k++) T[k] = __builtin_fmaf(1.0f, cv, T[k]);
+                }
             }
             t++;
             continue;
@@ -349,13 +364,15 @@ __device__ __forceinline__ void wave_eval_ls(const WaveProg &p, const float (&xs
             const bool isx = (m.x[h] & bit) != 0;
             GPR_KLOOP(isx ? xs[k] : cv);
         } else if (m.cheap[h] & bit) {
-            const bool mul = (m.mul[h] & bit) != 0;
-            const float sgn = (m.sub[h] & bit) ? -1.0f : 1.0f;
+            if (m.mul[h] & bit) {
 #pragma unroll
-            for (int k = 0; k < KC; k++) {
-                const float pm = __builtin_fmaf(sgn, T[k], U[k]);
-                const float pr = U[k] * T[k];
-                T[k] = mul ? pr : pm;
+                for (int k = 0; k < KC; k++) T[k] = U[k] * T[k];
+            } else if (m.sub[h] & bit) {
+#pragma unroll
+                for (int k = 0; k < KC; k++) T[k] = __builtin_fmaf(-1.0f, T[k], U[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < KC; k++) T[k] = __builtin_fmaf(1.0f, T[k], U[k]);
             }
             ls_load<KC>(lane + (sp - 3) * 64 * KC, U);   // the new second entry (level sp-3)
             sp--;
