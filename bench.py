@@ -47,18 +47,18 @@ HBM_PEAK = 8.0e12
 # were measured on.  For the batched workload a "launch" is one block's six concurrent
 # size-class launches.
 MEASURED_TRAFFIC = {
-    ("full", 1048576, 1000, 0): 2_749_440_416,
-    ("large", 131072, 1000, 0): 164_840_032,
-    ("batched", 262144, 1000, 0): 380_469_248,
+    ("full", 1048576, 1000, 0): 2_760_194_240,
+    ("large", 131072, 1000, 0): 163_457_248,
+    ("batched", 262144, 1000, 0): 383_504_960,
 }
 # VALU and SALU instructions per chain-step of the same launches (rocprofv3 --pmc
 # SQ_INSTS_VALU, SQ_INSTS_SALU over chains x steps).  With the live launch time they give
 # the share of issue slots each unit is busy: one wave64 instruction per SIMD every 4
 # cycles, 1024 SIMDs at 2.4 GHz.
 MEASURED_ISSUE = {
-    ("full", 1048576, 1000, 0): (1174.28, 874.39),
-    ("large", 131072, 1000, 0): (295.46, 299.06),
-    ("batched", 262144, 1000, 0): (329.80, 303.04),
+    ("full", 1048576, 1000, 0): (1120.95, 872.42),
+    ("large", 131072, 1000, 0): (281.12, 288.25),
+    ("batched", 262144, 1000, 0): (319.92, 296.77),
 }
 
 METRIC = "MH steps/sec + node×datapoint evals/sec (% VALU peak); ESS/sec MALA vs RW"
