@@ -1,6 +1,6 @@
 """Benchmark: the MH sampler on one MI355X, prints ONE JSON line.
 
-Headline workload (BASELINE configuration "full form"): 1,048,576 chains over 256
+Headline workload, the full form: 1,048,576 chains over 256
 datasets of uneven size (n log-uniform in [16, 1024], x ~ U[-10, 10], y = a seeded random
 target expression of at most 24 nodes + noise of sd 1% of the y range), postfix programs
 of at most 128 tokens, MALA constant moves (eps 0.01) and subtree structure moves in the
@@ -643,7 +643,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(datasets and chains shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+              f"(datasets and chains shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
 
     from gpuropolis_amd import lib as G
