@@ -642,7 +642,7 @@ def main():
         step_split(args)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(datasets and chains shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
 
