@@ -168,12 +168,11 @@ k_mh_config_wave(GprMHArgs a) {
         float cg0 = 0.0f, cg1 = 0.0f, pg0 = 0.0f, pg1 = 0.0f, cmg = 0.0f, pmg = 0.0f;
         bool grad_ok = false;
         AcceptDraws adraw;
-        // token classes of the current program (MALA: with the sweeps' plan)
-        // (MALA: the interpreter evaluates every likelihood; the sweeps' plan serves the
-        // gradients, which only proposals that survive early rejection need)
+        // token classes of the current program (MALA: with the sweeps' plan for programs
+        // without the forward-mode pass; only proposals that survive early rejection need
+        // their gradient)
         GradPlan gplan{};
         OpMasks cmask = op_masks<HI>(cur);
-        if (MALA) gplan = grad_plan<HI>(cur);
         // per-program values that change only with the structure: (operator tokens + 4)
         // and constants of the current program for the work counters, and whether its
         // random-walk moves can be batched (see BATCH below)
@@ -190,6 +189,8 @@ k_mh_config_wave(GprMHArgs a) {
             ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
             ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
             cur_nt = dual_nt(__popcll(ccm[0]) + __popcll(ccm[1]));
+            // the sweeps' plan serves only programs without the forward-mode pass
+            if (!cur_dual) gplan = grad_plan<HI>(cur);
         }
         int cur_opq, cur_nc;
         prog_counts<HI>(cur, cur_opq, cur_nc);
@@ -630,7 +631,6 @@ k_mh_config_wave(GprMHArgs a) {
                     zblk = 0xffffffffu;
                     zmblk = 0xffffffffu;
                     cmask = pmask;
-                    if (MALA) gplan = grad_plan<HI>(cur);
                     cur_opq = p_opq;
                     cur_nc = p_nc;
                     cur_bok = batch_ok<BATCH, HI>(cur, mk);
@@ -641,6 +641,7 @@ k_mh_config_wave(GprMHArgs a) {
                         ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
                         ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
                         cur_nt = dual_nt(__popcll(ccm[0]) + __popcll(ccm[1]));
+                        if (!cur_dual) gplan = grad_plan<HI>(cur);
                     }
                     cur_b16 = cur_bok && const_count<HI>(cur) <= 4;
                 } else {
