@@ -47,16 +47,16 @@ HBM_PEAK = 8.0e12
 # were measured on.  For the batched workload a "launch" is one block's six concurrent
 # size-class launches.
 MEASURED_TRAFFIC = {
-    ("full", 1048576, 1000, 0): 2_760_194_240,
-    ("large", 131072, 1000, 0): 163_457_248,
-    ("batched", 262144, 1000, 0): 383_504_960,
+    ("full", 1048576, 1000, 0): 2_617_144_928,
+    ("large", 131072, 1000, 0): 163_308_256,
+    ("batched", 262144, 1000, 0): 384_401_216,
 }
 # VALU and SALU instructions per chain-step of the same launches (rocprofv3 --pmc
 # SQ_INSTS_VALU, SQ_INSTS_SALU over chains x steps).  With the live launch time they give
 # the share of issue slots each unit is busy: one wave64 instruction per SIMD every 4
 # cycles, 1024 SIMDs at 2.4 GHz.
 MEASURED_ISSUE = {
-    ("full", 1048576, 1000, 0): (1120.95, 872.42),
+    ("full", 1048576, 1000, 0): (1110.94, 852.87),
     ("large", 131072, 1000, 0): (281.12, 288.25),
     ("batched", 262144, 1000, 0): (319.92, 296.77),
 }
