@@ -565,30 +565,30 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // GPR_DUAL_NT = 3, and NT = 2 in the same layout), one 8-byte store (NT = 1, programs of a
 // single constant), for programs of four constants (NT = GPR_DUAL_NT4) a 16-byte store of
 // the value and the first three tangents and a 4-byte store of the fourth into a second
-// plane, and for five or six (NT = GPR_DUAL_NTMAX, stack depth at most GPR_DUAL_DEPTH6) a
-// 16-byte, an 8-byte and a 4-byte store into three planes.  The values take the same
+// plane, and for five or six (NT = GPR_DUAL_NTMAX) a 16-byte, an 8-byte and a 4-byte store
+// into three planes.  The values take the same
 // operations on the same operands as wave_eval_ls, so the LL is
 // bit-identical to it (and early rejection stops a pass at the same chunk); each lane
 // adds the per-point gradient (-2 r w) f' of its points, and one wave sum per constant
 // ends the pass.  Programs qualify when they use only the config grammar (x, constant,
 // one, + - * / pow exp log neg), hold no constant-operand token, and hold at most
-// GPR_DUAL_NT4 constant tokens within GPR_DUAL_DEPTH stack entries or at most
-// GPR_DUAL_NTMAX within GPR_DUAL_DEPTH6.
+// GPR_DUAL_NT constant tokens within stack depth GPR_DUAL_DEPTH, GPR_DUAL_NT4 within
+// GPR_DUAL_DEPTH4 or GPR_DUAL_NTMAX within GPR_DUAL_DEPTH6.
 #define GPR_DUAL_NT 3
 #define GPR_DUAL_NT4 4
 #define GPR_DUAL_NTMAX 6
-#define GPR_DUAL_DEPTH6 5
-#define GPR_DUAL_DEPTH 6
+#define GPR_DUAL_DEPTH 9
+#define GPR_DUAL_DEPTH4 8
+#define GPR_DUAL_DEPTH6 6
 #define GPR_DUAL_W (1 + GPR_DUAL_NT)
 // Stack levels: a push at stack size sp stores U at level sp - 2 and a binary op at sp
-// loads level sp - 3, so a program of depth at most GPR_DUAL_DEPTH uses levels -2 ..
-// GPR_DUAL_DEPTH - 3, stored from offset 0 (level -2).  The packed layouts reserve
-// GPR_DUAL_DEPTH + 2 levels, the two-plane layout of NT = 4 GPR_DUAL_SPLIT_LEVELS, the
-// three-plane layout of NT = 6 exactly the GPR_DUAL_DEPTH6 levels its programs use.
-#define GPR_DUAL_SPLIT_LEVELS (GPR_DUAL_DEPTH + 1)
-#define GPR_DUAL_PACKED_FLOATS ((GPR_DUAL_DEPTH + 2) * 64 * GPR_DUAL_W)
-#define GPR_DUAL_SPLIT_FLOATS (GPR_DUAL_SPLIT_LEVELS * 64 * (GPR_DUAL_NT4 + 1))
-#define GPR_DUAL_SPLIT6_FLOATS (GPR_DUAL_DEPTH6 * 64 * (GPR_DUAL_NTMAX + 1))
+// loads level sp - 3, so a program of depth at most D uses levels -1 .. D - 3 (D - 1 of
+// them), stored from offset 0 (level -1): token 0, always a leaf, starts the stack in T
+// without the store of level -2 (U holds no entry yet).  Levels -1 and below hold no
+// stack entry (the store at sp = 1 and the load at sp = 2 move the empty U).
+#define GPR_DUAL_PACKED_FLOATS ((GPR_DUAL_DEPTH - 1) * 64 * GPR_DUAL_W)
+#define GPR_DUAL_SPLIT_FLOATS ((GPR_DUAL_DEPTH4 - 1) * 64 * (GPR_DUAL_NT4 + 1))
+#define GPR_DUAL_SPLIT6_FLOATS ((GPR_DUAL_DEPTH6 - 1) * 64 * (GPR_DUAL_NTMAX + 1))
 // floats of the dual stack of one wave
 #define GPR_DUAL_LDS_FLOATS \
     (GPR_DUAL_PACKED_FLOATS > GPR_DUAL_SPLIT_FLOATS                                      \
@@ -666,7 +666,8 @@ __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
     const bool v0 = l < p.len, v1 = HI && l + 64 < p.len;
     const bool bad = (v0 && !((GRAMMAR >> (p.o0 & 31)) & 1u)) || (v1 && !((GRAMMAR >> (p.o1 & 31)) & 1u));
     const int k = const_count<HI>(p);
-    return ballot(bad) == 0 && ((k <= GPR_DUAL_NT4 && dmax <= GPR_DUAL_DEPTH) || (k <= GPR_DUAL_NTMAX && dmax <= GPR_DUAL_DEPTH6));
+    return ballot(bad) == 0 && ((k <= GPR_DUAL_NT && dmax <= GPR_DUAL_DEPTH) || (k <= GPR_DUAL_NT4 && dmax <= GPR_DUAL_DEPTH4) ||
+                                (k <= GPR_DUAL_NTMAX && dmax <= GPR_DUAL_DEPTH6));
 }
 
 // One token (two for a leaf fused with the next +, - or *) of the forward-mode
@@ -785,13 +786,13 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
     static_assert(NT == 1 || NT == 2 || NT == GPR_DUAL_NT || NT == GPR_DUAL_NT4 || NT == GPR_DUAL_NTMAX, "tangent count");
     constexpr int W = 1 + NT;
     const int l = lane_id();
-    // this lane's level 0 of the stack below T, U (dual_st); levels -2 and -1 lie below it
-    // NT = 4: planes of 4 and 1 floats, GPR_DUAL_SPLIT_LEVELS levels each; NT = 6: planes of
-    // 4, 2 and 1 floats, GPR_DUAL_DEPTH6 levels each
-    constexpr int LV = NT == GPR_DUAL_NTMAX ? GPR_DUAL_DEPTH6 : GPR_DUAL_SPLIT_LEVELS;
-    float *lane = (NT >= GPR_DUAL_NT4 || NT == 2) ? ls + 2 * 256 + l * 4 : ls + 2 * 64 * W + l * W;
-    float *lanec = ls + LV * 256 + 2 * 128 + l * 2;
-    float *laneb = ls + LV * (NT == GPR_DUAL_NTMAX ? 384 : 256) + 2 * 64 + l;
+    // this lane's level 0 of the stack below T, U (dual_st); level -1 lies below it
+    // NT = 4: planes of 4 and 1 floats, GPR_DUAL_DEPTH4 - 1 levels each; NT = 6: planes of
+    // 4, 2 and 1 floats, GPR_DUAL_DEPTH6 - 1 levels each
+    constexpr int LV = NT == GPR_DUAL_NTMAX ? GPR_DUAL_DEPTH6 - 1 : GPR_DUAL_DEPTH4 - 1;
+    float *lane = (NT >= GPR_DUAL_NT4 || NT == 2) ? ls + 256 + l * 4 : ls + 64 * W + l * W;
+    float *lanec = ls + LV * 256 + 128 + l * 2;
+    float *laneb = ls + LV * (NT == GPR_DUAL_NTMAX ? 384 : 256) + 64 + l;
     float acc = 0.0f, comp = 0.0f;
     float ga[NT];
 #pragma unroll
@@ -805,12 +806,17 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
         if (ch == 0) c = c0; else c.load(dx, dy, dw, d.n, ch * 64);
         const float xv = c.x[0];
         float T[W], U[W];
+        // token 0 (x, a constant of rank 0 or one) starts the stack in T
+        const bool t0x = (m.x[0] & 1ull) != 0, t0c = (cm[0] & 1ull) != 0;
+        T[0] = t0x ? xv : (t0c ? rl_f(p.c0, 0) : 1.0f);
 #pragma unroll
-        for (int j = 0; j < W; j++) { T[j] = 0.0f; U[j] = 0.0f; }
-        int sp = 0;
+        for (int j = 1; j < W; j++) T[j] = (t0c && j == 1) ? 1.0f : 0.0f;
+#pragma unroll
+        for (int j = 0; j < W; j++) U[j] = 0.0f;
+        int sp = 1;
         // tokens of the low half, then (two-register tapes) the high half
         const int lo_end = HI ? min(p.len, 64) : p.len;
-        for (int t = 0; t < lo_end; t++) dual_token<0, NT>(p, m, cm, nlo, xv, lane, laneb, lanec, T, U, sp, t);
+        for (int t = 1; t < lo_end; t++) dual_token<0, NT>(p, m, cm, nlo, xv, lane, laneb, lanec, T, U, sp, t);
         if (HI)
             for (int t = 64; t < p.len; t++) dual_token<1, NT>(p, m, cm, nlo, xv, lane, laneb, lanec, T, U, sp, t);
         const int i = ch * 64 + l;

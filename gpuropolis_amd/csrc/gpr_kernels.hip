@@ -23,11 +23,10 @@
 // MALA: constant moves are Langevin proposals c' = c + h g(c) + eps z (h = eps^2/2, g the
 // gradient of the log posterior) with the Hastings ratio of the two Gaussian proposal
 // densities.  Under MALA every likelihood is evaluated at one point per lane (KC = 1),
-// with early rejection.  Programs of at most four constants and stack depth 6, or six
-// constants and depth 5, get the proposal's LL and gradient from one forward-mode
+// with early rejection.  Programs of at most three constants and stack depth 9, four and
+// depth 8, or six and depth 6 get the proposal's LL and gradient from one forward-mode
 // interpreter pass (value and tangents per stack entry); the others are pre-checked by
-// the plain interpreter, and the
-// survivors get the stack-free adjoint sweeps, whose token values sit in the wave's LDS
+// the plain interpreter, and the survivors get the stack-free adjoint sweeps, whose token values sit in the wave's LDS
 // region for programs of at most GPR_MALA_LDS_ROWS tokens and in the wave's slab in
 // global memory for longer ones (LDS rows for every token of a 128-token cap, 32 KB per
 // wave, would leave room for five waves per CU).  The current structure's sweep plan

@@ -290,6 +290,73 @@ def test_mala_forward_mode_programs_shadow_oracle(oracle, gpr, cap):
     assert abs(st.const_accepted - cnt[3]) <= 0.1 * cnt[3] + 5
 
 
+def test_mala_deep_stack_programs_shadow_oracle(oracle, gpr):
+    """Stack depth at the forward-mode pass's limits and one past them: every leaf pushed
+    before the first operator, so a program of D leaves has depth D.  (1, 2 or 3
+    constants, depth 9), (4, depth 8) and (5 or 6, depth 6) take the forward-mode pass at
+    its deepest stack; (3, depth 10), (4, depth 9) and (5 or 6, depth 7) take the adjoint
+    sweeps.  Each group fits its own dataset of 150 points (three chunks), made from its
+    program at the starting constants' centre, and its chains shadow the oracle's."""
+    rng = np.random.default_rng(53)
+    n, per, steps, eps, cap = 150, 7, 40, 2e-3, 64
+    groups = [(1, 9), (2, 9), (3, 9), (3, 10), (4, 8), (4, 9), (5, 6), (6, 6), (5, 7), (6, 7)]
+    x = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    sd = np.full(n, 0.05, dtype=np.float32)
+
+    def program(k, depth, cvals):
+        # leaves: k constants (with the second last times x when no x leaf follows), then x
+        # and exp(-x) in turn; operators + and * in turn from the innermost
+        toks = []
+        for j in range(k):
+            toks += [("CONST", float(cvals[j]))] + (["X", "MUL"] if depth == k and j == k - 2 else [])
+        for j in range(depth - k):
+            toks += ["X"] if j % 2 == 0 else ["X", "NEG", "EXP"]
+        toks += ["ADD" if j % 2 == 0 else "MUL" for j in range(depth - 1)]
+        return toks
+
+    O = np.zeros((cap, per * len(groups)), dtype=np.uint8)
+    V = np.zeros((cap, per * len(groups)), dtype=np.float32)
+    L = np.zeros(per * len(groups), dtype=np.int32)
+    data = []
+    for g, (k, depth) in enumerate(groups):
+        centre = rng.uniform(0.3, 0.8, k) * rng.choice([-1.0, 1.0], k)
+        o, v = oracle.tape_arrays(program(k, depth, centre.astype(np.float32)))
+        assert oracle.lib().gpo_tape_depth(oracle.u8(o), len(o)) == depth
+        f = np.array([oracle.eval_tape(o, v, xi) for xi in x])
+        data.append((x, (f + rng.normal(0.0, 0.05, n)).astype(np.float32), sd))
+        for c in range(g * per, (g + 1) * per):
+            o, v = oracle.tape_arrays(program(k, depth, (centre * (1 + 0.005 * rng.normal(size=k))).astype(np.float32)))
+            O[:len(o), c] = o
+            V[:len(o), c] = v
+            L[c] = len(o)
+    cfg = oracle.make_cfg(cap=cap, proposal=1, eps=eps, fixed_structure=1, seed=59)
+    o_cs = V.copy()
+    cnt = np.zeros(4, dtype=np.int64)
+    for g in range(len(groups)):
+        s = slice(g * per, (g + 1) * per)
+        o_ops, cs_g, o_ln = O[:, s].copy(), V[:, s].copy(), L[s].copy()
+        _, _, c_g = oracle.cfg_run_block(cfg, o_ops, cs_g, o_ln, *data[g], 0, steps, chain0=g * per)
+        o_cs[:, s] = cs_g
+        cnt += c_g
+    with gpr.Context(cap=cap, seed=59, proposal=gpr.PROP_MALA, eps=eps, fixed_structure=1,
+                     resample=gpr.RS_NONE) as ctx:
+        ctx.set_data(data, [per] * len(groups))
+        ctx.write_state(O, V, L)
+        say(f"mala deep stacks: {len(groups)} groups (constants, depth) {groups}, {per} chains each, {steps} steps")
+        ctx.run_block(steps)
+        g_cs = ctx.read_state()[1]
+        st = ctx.stats()
+    rel = np.max(np.abs(g_cs.astype(np.float64) - o_cs) / np.maximum(np.abs(o_cs.astype(np.float64)), 1e-3), axis=0)
+    close = rel < 1e-3
+    for g, kd in enumerate(groups):
+        s = slice(g * per, (g + 1) * per)
+        say(f"  {kd}: {int(close[s].sum())} of {per} within 1e-3")
+        assert close[s].sum() >= per - 1
+    say("acceptance gpu", st.const_accepted / st.const_proposed, "oracle", cnt[3] / max(cnt[2], 1))
+    assert cnt[3] > 0.2 * cnt[2]
+    assert abs(st.const_accepted - cnt[3]) <= 0.1 * cnt[3] + 5
+
+
 def test_mala_zero_tangent_meets_infinite_partial(oracle, gpr):
     """c * x^(1/2) with x = 0 among the data: at x = 0 the partial of x^(1/2) in x is
     infinite and its partial in the exponent is NaN, but neither operand carries a
