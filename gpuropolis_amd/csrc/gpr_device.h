@@ -565,18 +565,21 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // GPR_DUAL_NT = 3, and NT = 2 in the same layout), one 8-byte store (NT = 1, programs of a
 // single constant), for programs of four constants (NT = GPR_DUAL_NT4) a 16-byte store of
 // the value and the first three tangents and a 4-byte store of the fourth into a second
-// plane, and for five or six (NT = GPR_DUAL_NTMAX) a 16-byte, an 8-byte and a 4-byte store
-// into three planes.  The values take the same
+// plane, and for five or six (NT = GPR_DUAL_NT6) a 16-byte, an 8-byte and a 4-byte store
+// into three planes.  Programs of up to GPR_DUAL_NTMAX constants that fit none of these take
+// two passes of four tangents each (constants of rank 0-3, then 4-7: the second pass only
+// when the first completes).  The values take the same
 // operations on the same operands as wave_eval_ls, so the LL is
 // bit-identical to it (and early rejection stops a pass at the same chunk); each lane
 // adds the per-point gradient (-2 r w) f' of its points, and one wave sum per constant
 // ends the pass.  Programs qualify when they use only the config grammar (x, constant,
 // one, + - * / pow exp log neg), hold no constant-operand token, and hold at most
-// GPR_DUAL_NT constant tokens within stack depth GPR_DUAL_DEPTH, GPR_DUAL_NT4 within
-// GPR_DUAL_DEPTH4 or GPR_DUAL_NTMAX within GPR_DUAL_DEPTH6.
+// GPR_DUAL_NT constant tokens within stack depth GPR_DUAL_DEPTH, GPR_DUAL_NT6 within
+// GPR_DUAL_DEPTH6 or GPR_DUAL_NTMAX within GPR_DUAL_DEPTH4.
 #define GPR_DUAL_NT 3
 #define GPR_DUAL_NT4 4
-#define GPR_DUAL_NTMAX 6
+#define GPR_DUAL_NT6 6
+#define GPR_DUAL_NTMAX 8
 #define GPR_DUAL_DEPTH 9
 #define GPR_DUAL_DEPTH4 8
 #define GPR_DUAL_DEPTH6 6
@@ -588,7 +591,7 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // stack entry (the store at sp = 1 and the load at sp = 2 move the empty U).
 #define GPR_DUAL_PACKED_FLOATS ((GPR_DUAL_DEPTH - 1) * 64 * GPR_DUAL_W)
 #define GPR_DUAL_SPLIT_FLOATS ((GPR_DUAL_DEPTH4 - 1) * 64 * (GPR_DUAL_NT4 + 1))
-#define GPR_DUAL_SPLIT6_FLOATS ((GPR_DUAL_DEPTH6 - 1) * 64 * (GPR_DUAL_NTMAX + 1))
+#define GPR_DUAL_SPLIT6_FLOATS ((GPR_DUAL_DEPTH6 - 1) * 64 * (GPR_DUAL_NT6 + 1))
 // floats of the dual stack of one wave
 #define GPR_DUAL_LDS_FLOATS \
     (GPR_DUAL_PACKED_FLOATS > GPR_DUAL_SPLIT_FLOATS                                      \
@@ -600,7 +603,7 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // first plane and the last tangent in laneb's; for NT = 6 also two in lanec's.
 template <int NT>
 __device__ __forceinline__ void dual_st(float *lane, float *laneb, float *lanec, int lv, const float (&v)[1 + NT]) {
-    if constexpr (NT == GPR_DUAL_NTMAX) {
+    if constexpr (NT == GPR_DUAL_NT6) {
         *(float4 *)(lane + lv * 256) = make_float4(v[0], v[1], v[2], v[3]);
         *(float2 *)(lanec + lv * 128) = make_float2(v[4], v[5]);
         laneb[lv * 64] = v[NT];
@@ -616,7 +619,7 @@ __device__ __forceinline__ void dual_st(float *lane, float *laneb, float *lanec,
 template <int NT>
 __device__ __forceinline__ void dual_ld(const float *lane, const float *laneb, const float *lanec, int lv,
                                         float (&v)[1 + NT]) {
-    if constexpr (NT == GPR_DUAL_NTMAX) {
+    if constexpr (NT == GPR_DUAL_NT6) {
         const float4 f = *(const float4 *)(lane + lv * 256);
         const float2 e = *(const float2 *)(lanec + lv * 128);
         v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
@@ -652,9 +655,14 @@ __device__ __forceinline__ float tmul(float t, float partial) {
     return r;
 }
 
-// tangents of the forward-mode pass of a program of k constant tokens
-__device__ __forceinline__ int dual_nt(int k) {
-    return k <= 2 ? (k <= 1 ? 1 : 2) : (k <= GPR_DUAL_NT ? GPR_DUAL_NT : (k <= GPR_DUAL_NT4 ? GPR_DUAL_NT4 : GPR_DUAL_NTMAX));
+// tangents of the forward-mode pass of a program of k constant tokens and stack depth dmax
+// (GPR_DUAL_NTMAX: two passes of GPR_DUAL_NT4)
+__device__ __forceinline__ int dual_nt(int k, int dmax) {
+    return k <= 2 ? (k <= 1 ? 1 : 2)
+                  : (k <= GPR_DUAL_NT ? GPR_DUAL_NT
+                                      : (k <= GPR_DUAL_NT4 ? GPR_DUAL_NT4
+                                                           : (k <= GPR_DUAL_NT6 && dmax <= GPR_DUAL_DEPTH6 ? GPR_DUAL_NT6
+                                                                                                            : GPR_DUAL_NTMAX)));
 }
 
 template <bool HI>
@@ -666,28 +674,27 @@ __device__ __forceinline__ bool dual_ok(const WaveProg &p, int dmax) {
     const bool v0 = l < p.len, v1 = HI && l + 64 < p.len;
     const bool bad = (v0 && !((GRAMMAR >> (p.o0 & 31)) & 1u)) || (v1 && !((GRAMMAR >> (p.o1 & 31)) & 1u));
     const int k = const_count<HI>(p);
-    return ballot(bad) == 0 && ((k <= GPR_DUAL_NT && dmax <= GPR_DUAL_DEPTH) || (k <= GPR_DUAL_NT4 && dmax <= GPR_DUAL_DEPTH4) ||
-                                (k <= GPR_DUAL_NTMAX && dmax <= GPR_DUAL_DEPTH6));
+    return ballot(bad) == 0 && ((k <= GPR_DUAL_NT && dmax <= GPR_DUAL_DEPTH) || (k <= GPR_DUAL_NTMAX && dmax <= GPR_DUAL_DEPTH4));
 }
 
 // One token (two for a leaf fused with the next +, - or *) of the forward-mode
 // interpreter, tokens of half H of the tape (t < 64 for H = 0): the half is a
 // compile-time constant, so the token-class masks and the tape words are read without
-// per-token selects.  NT tangents (1, 2, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NTMAX: at least the
-// program's constant count).  lane, laneb: this lane's level 0 of the stack (dual_st).
+// per-token selects.  NT tangents (1, 2, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NT6) for the
+// constants of rank j0 .. j0 + NT - 1.  lane, laneb: this lane's level 0 of the stack (dual_st).
 template <int H, int NT>
 __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, const uint64_t (&cm)[2], int nlo,
-                                           float xv, float *lane, float *laneb, float *lanec, float (&T)[1 + NT],
+                                           int j0, float xv, float *lane, float *laneb, float *lanec, float (&T)[1 + NT],
                                            float (&U)[1 + NT], int &sp, int &t) {
     constexpr int W = 1 + NT;
     const int li = t & 63;
     constexpr int h = H;
     const uint64_t bit = 1ull << li;
     if (m.leaf[h] & bit) {
-        // the leaf's value and tangent: x (0), or constant of rank rk (e_rk)
+        // the leaf's value and tangent: x (0), or constant of rank j0 + rk (e_rk)
         const bool isx = (m.x[h] & bit) != 0;
         const float cv = h ? rl_f(p.c1, li) : rl_f(p.c0, li);
-        const int rk = isx ? -1 : (h ? nlo : 0) + __popcll(cm[h] & (bit - 1));
+        const int rk = isx ? -1 : (h ? nlo : 0) + __popcll(cm[h] & (bit - 1)) - j0;
         const float b = isx ? xv : cv;
         if (m.cheap[h] & (bit << 1)) {
             // leaf followed by +, - or *: T = T op leaf, stack untouched
@@ -774,25 +781,26 @@ __device__ __forceinline__ void dual_token(const WaveProg &p, const OpMasks &m, 
 }
 
 // LL of the wave's program (as wave_loglik<1, HI, true>) and, when the pass completes,
-// gs[j] = sum over points of (-2 r w) df/dc_j for the constant token of rank j (token
-// order; cm = the constant-token masks).  ls: the wave's dual stack region.
-// NT: tangents carried, at least the program's constant count (1, 2, GPR_DUAL_NT,
-// GPR_DUAL_NT4 or GPR_DUAL_NTMAX); gs[j] for j >= NT is 0.
+// gs[j] = sum over points of (-2 r w) df/dc_{j0+j} for the constant token of rank j0 + j
+// (token order; cm = the constant-token masks).  ls: the wave's dual stack region.
+// NT: tangents carried (1, 2, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NT6), at least the
+// program's constant count less j0 unless a second pass follows; gs[j] is 0 for j >= NT
+// and for ranks past the program's constants.
 template <bool HI, int NT>
 __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
                                    const float *dw, const Chunk<1> &c0, int nchunks, float *ls, const OpMasks &m,
-                                   const uint64_t (&cm)[2], double reject_below, int *points,
+                                   const uint64_t (&cm)[2], double reject_below, int *points, int j0,
                                    float (&gs)[GPR_DUAL_NTMAX]) {
-    static_assert(NT == 1 || NT == 2 || NT == GPR_DUAL_NT || NT == GPR_DUAL_NT4 || NT == GPR_DUAL_NTMAX, "tangent count");
+    static_assert(NT == 1 || NT == 2 || NT == GPR_DUAL_NT || NT == GPR_DUAL_NT4 || NT == GPR_DUAL_NT6, "tangent count");
     constexpr int W = 1 + NT;
     const int l = lane_id();
     // this lane's level 0 of the stack below T, U (dual_st); level -1 lies below it
     // NT = 4: planes of 4 and 1 floats, GPR_DUAL_DEPTH4 - 1 levels each; NT = 6: planes of
     // 4, 2 and 1 floats, GPR_DUAL_DEPTH6 - 1 levels each
-    constexpr int LV = NT == GPR_DUAL_NTMAX ? GPR_DUAL_DEPTH6 - 1 : GPR_DUAL_DEPTH4 - 1;
+    constexpr int LV = NT == GPR_DUAL_NT6 ? GPR_DUAL_DEPTH6 - 1 : GPR_DUAL_DEPTH4 - 1;
     float *lane = (NT >= GPR_DUAL_NT4 || NT == 2) ? ls + 256 + l * 4 : ls + 64 * W + l * W;
     float *lanec = ls + LV * 256 + 128 + l * 2;
-    float *laneb = ls + LV * (NT == GPR_DUAL_NTMAX ? 384 : 256) + 64 + l;
+    float *laneb = ls + LV * (NT == GPR_DUAL_NT6 ? 384 : 256) + 64 + l;
     float acc = 0.0f, comp = 0.0f;
     float ga[NT];
 #pragma unroll
@@ -810,15 +818,15 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
         const bool t0x = (m.x[0] & 1ull) != 0, t0c = (cm[0] & 1ull) != 0;
         T[0] = t0x ? xv : (t0c ? rl_f(p.c0, 0) : 1.0f);
 #pragma unroll
-        for (int j = 1; j < W; j++) T[j] = (t0c && j == 1) ? 1.0f : 0.0f;
+        for (int j = 1; j < W; j++) T[j] = (t0c && j0 == 0 && j == 1) ? 1.0f : 0.0f;
 #pragma unroll
         for (int j = 0; j < W; j++) U[j] = 0.0f;
         int sp = 1;
         // tokens of the low half, then (two-register tapes) the high half
         const int lo_end = HI ? min(p.len, 64) : p.len;
-        for (int t = 1; t < lo_end; t++) dual_token<0, NT>(p, m, cm, nlo, xv, lane, laneb, lanec, T, U, sp, t);
+        for (int t = 1; t < lo_end; t++) dual_token<0, NT>(p, m, cm, nlo, j0, xv, lane, laneb, lanec, T, U, sp, t);
         if (HI)
-            for (int t = 64; t < p.len; t++) dual_token<1, NT>(p, m, cm, nlo, xv, lane, laneb, lanec, T, U, sp, t);
+            for (int t = 64; t < p.len; t++) dual_token<1, NT>(p, m, cm, nlo, j0, xv, lane, laneb, lanec, T, U, sp, t);
         const int i = ch * 64 + l;
         const bool valid = i < d.n;
         const float r = T[0] - c.y[0];
@@ -840,10 +848,31 @@ __device__ __forceinline__ double wave_loglik_dual(const WaveProg &p, const GprD
         }
     }
 #pragma unroll
-    for (int j = 0; j < GPR_DUAL_NTMAX; j++) gs[j] = (j < NT && j < nlo + (HI ? __popcll(cm[1]) : 0)) ? wave_sum_f(ga[j < NT ? j : 0]) : 0.0f;
+    for (int j = 0; j < GPR_DUAL_NTMAX; j++)
+        gs[j] = (j < NT && j0 + j < nlo + (HI ? __popcll(cm[1]) : 0)) ? wave_sum_f(ga[j < NT ? j : 0]) : 0.0f;
     const double s = wave_sum_d((double)acc - (double)comp);
     if (!__builtin_isfinite(s)) return -GPR_INF;
     return d.lconst - (d.wconst > 0.0 ? d.wconst * s : s);
+}
+
+// npass (1 or 2) passes of GPR_DUAL_NT4 tangents: the constants of rank 0-3, then, when the
+// first pass completes, those of rank 4-7 (its LL is the first's: same values, same
+// order).
+template <bool HI>
+__device__ __forceinline__ double dual_passes4(const WaveProg &p, const GprDataset &d, const float *dx, const float *dy,
+                                               const float *dw, const Chunk<1> &c0, int nchunks, float *ls,
+                                               const OpMasks &m, const uint64_t (&cm)[2], double reject_below,
+                                               int *points, int npass, float (&gs)[GPR_DUAL_NTMAX]) {
+    const double ll =
+        wave_loglik_dual<HI, GPR_DUAL_NT4>(p, d, dx, dy, dw, c0, nchunks, ls, m, cm, reject_below, points, 0, gs);
+    if (npass > 1 && __builtin_isfinite(ll)) {
+        float g[GPR_DUAL_NTMAX];
+        (void)wave_loglik_dual<HI, GPR_DUAL_NT4>(p, d, dx, dy, dw, c0, nchunks, ls, m, cm, -GPR_INF, nullptr,
+                                                 GPR_DUAL_NT4, g);
+#pragma unroll
+        for (int j = 0; j < GPR_DUAL_NT4; j++) gs[GPR_DUAL_NT4 + j] = g[j];
+    }
+    return ll;
 }
 
 // The gradient of the log posterior in the lane layout of wave_loglik_grad: lane t of g0

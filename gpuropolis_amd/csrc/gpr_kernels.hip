@@ -23,8 +23,8 @@
 // MALA: constant moves are Langevin proposals c' = c + h g(c) + eps z (h = eps^2/2, g the
 // gradient of the log posterior) with the Hastings ratio of the two Gaussian proposal
 // densities.  Under MALA every likelihood is evaluated at one point per lane (KC = 1),
-// with early rejection.  Programs of at most three constants and stack depth 9, four and
-// depth 8, or six and depth 6 get the proposal's LL and gradient from one forward-mode
+// with early rejection.  Programs of at most three constants and stack depth 9, or eight
+// and depth 8, get the proposal's LL and gradient from one forward-mode
 // interpreter pass (value and tangents per stack entry); the others are pre-checked by
 // the plain interpreter, and the survivors get the stack-free adjoint sweeps, whose token values sit in the wave's LDS
 // region for programs of at most GPR_MALA_LDS_ROWS tokens and in the wave's slab in
@@ -177,7 +177,8 @@ k_mh_config_wave(GprMHArgs a) {
         // random-walk moves can be batched (see BATCH below)
         // MALA: whether the current program's gradient comes from the forward-mode
         // interpreter (dual_ok), with its constant-token masks
-        // (cur_nt: tangents of its pass, 1, 2, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NTMAX)
+        // (cur_nt: tangents of its pass, 1, 2, GPR_DUAL_NT, GPR_DUAL_NT4 or GPR_DUAL_NT6, or
+        // GPR_DUAL_NTMAX for two passes of GPR_DUAL_NT4)
         bool cur_dual = false;
         int cur_nt = GPR_DUAL_NT;
         uint64_t ccm[2] = {0, 0};
@@ -187,7 +188,7 @@ k_mh_config_wave(GprMHArgs a) {
             cur_dual = !MET && mk == 0 && dual_ok<HI>(cur, dm);
             ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
             ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
-            cur_nt = dual_nt(__popcll(ccm[0]) + __popcll(ccm[1]));
+            cur_nt = dual_nt(__popcll(ccm[0]) + __popcll(ccm[1]), dm);
             // the sweeps' plan serves only programs without the forward-mode pass
             if (!cur_dual) gplan = grad_plan<HI>(cur);
         }
@@ -452,17 +453,17 @@ k_mh_config_wave(GprMHArgs a) {
                             const Chunk<1> &k0c = *(const Chunk<1> *)&c0;
                             const double cl =
                                 cur_nt == 1 ? wave_loglik_dual<HI, 1>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm,
-                                                                      -GPR_INF, &np_, gs)
+                                                                      -GPR_INF, &np_, 0, gs)
                                 : cur_nt == 2 ? wave_loglik_dual<HI, 2>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm,
-                                                                        -GPR_INF, &np_, gs)
+                                                                        -GPR_INF, &np_, 0, gs)
                                 : cur_nt == GPR_DUAL_NT
                                     ? wave_loglik_dual<HI, GPR_DUAL_NT>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm,
-                                                                        -GPR_INF, &np_, gs)
-                                : cur_nt == GPR_DUAL_NT4
-                                    ? wave_loglik_dual<HI, GPR_DUAL_NT4>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask,
-                                                                         ccm, -GPR_INF, &np_, gs)
-                                    : wave_loglik_dual<HI, GPR_DUAL_NTMAX>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask,
-                                                                           ccm, -GPR_INF, &np_, gs);
+                                                                        -GPR_INF, &np_, 0, gs)
+                                : cur_nt == GPR_DUAL_NT6
+                                    ? wave_loglik_dual<HI, GPR_DUAL_NT6>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask,
+                                                                         ccm, -GPR_INF, &np_, 0, gs)
+                                    : dual_passes4<HI>(cur, d, dx, dy, dw, k0c, nchunks, ls, cmask, ccm, -GPR_INF,
+                                                       &np_, cur_nt == GPR_DUAL_NTMAX ? 2 : 1, gs);
                             dual_grad_lanes<HI>(cur, ccm, gs, h0, h1, gok);
                             gok = gok && __builtin_isfinite(cl);
                         } else {
@@ -522,19 +523,19 @@ k_mh_config_wave(GprMHArgs a) {
                     const bool precheck = !MET && !cur_dual && nchunks > 1;
                     if (cur_dual && cur_nt == 1)
                         pll_m = wave_loglik_dual<HI, 1>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls,
-                                                        cmask, ccm, rb, &mala_pts, dgs);
+                                                        cmask, ccm, rb, &mala_pts, 0, dgs);
                     else if (cur_dual && cur_nt == 2)
                         pll_m = wave_loglik_dual<HI, 2>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls,
-                                                        cmask, ccm, rb, &mala_pts, dgs);
+                                                        cmask, ccm, rb, &mala_pts, 0, dgs);
                     else if (cur_dual && cur_nt == GPR_DUAL_NT)
                         pll_m = wave_loglik_dual<HI, GPR_DUAL_NT>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks,
-                                                                  ls, cmask, ccm, rb, &mala_pts, dgs);
-                    else if (cur_dual && cur_nt == GPR_DUAL_NT4)
-                        pll_m = wave_loglik_dual<HI, GPR_DUAL_NT4>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0,
-                                                                   nchunks, ls, cmask, ccm, rb, &mala_pts, dgs);
+                                                                  ls, cmask, ccm, rb, &mala_pts, 0, dgs);
+                    else if (cur_dual && cur_nt == GPR_DUAL_NT6)
+                        pll_m = wave_loglik_dual<HI, GPR_DUAL_NT6>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0,
+                                                                   nchunks, ls, cmask, ccm, rb, &mala_pts, 0, dgs);
                     else if (cur_dual)
-                        pll_m = wave_loglik_dual<HI, GPR_DUAL_NTMAX>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0,
-                                                                     nchunks, ls, cmask, ccm, rb, &mala_pts, dgs);
+                        pll_m = dual_passes4<HI>(prop, d, dx, dy, dw, *(const Chunk<1> *)&c0, nchunks, ls, cmask, ccm,
+                                                 rb, &mala_pts, cur_nt == GPR_DUAL_NTMAX ? 2 : 1, dgs);
                     else if (precheck && HI && prop.len <= 64)
                         pll_m = wave_loglik<KC, false, true>(prop, d, dx, dy, dw, c0, nchunks, ls, &cmask, rb,
                                                              &mala_pts);
@@ -639,7 +640,7 @@ k_mh_config_wave(GprMHArgs a) {
                         cur_dual = !MET && mk == 0 && dual_ok<HI>(cur, dm);
                         ccm[0] = ballot(l < cur.len && cur.o0 == GPR_OP_CONST);
                         ccm[1] = HI ? ballot(l + 64 < cur.len && cur.o1 == GPR_OP_CONST) : 0;
-                        cur_nt = dual_nt(__popcll(ccm[0]) + __popcll(ccm[1]));
+                        cur_nt = dual_nt(__popcll(ccm[0]) + __popcll(ccm[1]), dm);
                         if (!cur_dual) gplan = grad_plan<HI>(cur);
                     }
                     cur_b16 = cur_bok && const_count<HI>(cur) <= 4;

@@ -229,10 +229,10 @@ def test_mala_long_programs_shadow_oracle(oracle, gpr):
 
 @pytest.mark.parametrize("cap", [64, 128])
 def test_mala_forward_mode_programs_shadow_oracle(oracle, gpr, cap):
-    """Programs of one to six constants take the forward-mode pass (one tangent for one
-    constant, three for two or three, four in the two-plane stack for four, six in the
-    three-plane stack for five or six within stack depth 5); seven constants take the
-    adjoint sweeps.  Sums of c_j f_j(x) over 150 points (three chunks), at cap 128 half of
+    """Programs of one to seven constants (stack depth at most 5) take the forward-mode
+    pass: one tangent for one constant, two for two, three for three, four in the
+    two-plane stack for four, six in the three-plane stack for five or six, and two
+    passes of four tangents for seven.  Sums of c_j f_j(x) over 150 points (three chunks), at cap 128 half of
     them padded past 64 tokens with (x - x) terms, shadow the oracle's MALA chains in
     every group."""
     rng = np.random.default_rng(29)
@@ -293,23 +293,29 @@ def test_mala_forward_mode_programs_shadow_oracle(oracle, gpr, cap):
 def test_mala_deep_stack_programs_shadow_oracle(oracle, gpr):
     """Stack depth at the forward-mode pass's limits and one past them: every leaf pushed
     before the first operator, so a program of D leaves has depth D.  (1, 2 or 3
-    constants, depth 9), (4, depth 8) and (5 or 6, depth 6) take the forward-mode pass at
-    its deepest stack; (3, depth 10), (4, depth 9) and (5 or 6, depth 7) take the adjoint
-    sweeps.  Each group fits its own dataset of 150 points (three chunks), made from its
-    program at the starting constants' centre, and its chains shadow the oracle's."""
+    constants, depth 9), (4, depth 8) and (5 or 6, depth 6) take one forward-mode pass at
+    its deepest stack; (5, depth 7), (6, depth 8), (7, depth 5) and (8, depth 8) take two
+    passes of four tangents; (3, depth 10), (4 or 5, depth 9), (8, depth 9) and (9, depth
+    5) take the adjoint sweeps.  Each group fits its own dataset of 150 points (three
+    chunks), made from its program at the starting constants' centre, and its chains
+    shadow the oracle's."""
     rng = np.random.default_rng(53)
     n, per, steps, eps, cap = 150, 7, 40, 2e-3, 64
-    groups = [(1, 9), (2, 9), (3, 9), (3, 10), (4, 8), (4, 9), (5, 6), (6, 6), (5, 7), (6, 7)]
+    groups = [(1, 9), (2, 9), (3, 9), (4, 8), (5, 6), (6, 6), (5, 7), (6, 8), (7, 5), (8, 8), (3, 10), (4, 9),
+              (5, 9), (8, 9), (9, 5)]
     x = rng.uniform(0.5, 1.5, n).astype(np.float32)
     sd = np.full(n, 0.05, dtype=np.float32)
 
     def program(k, depth, cvals):
-        # leaves: k constants (with the second last times x when no x leaf follows), then x
-        # and exp(-x) in turn; operators + and * in turn from the innermost
-        toks = []
-        for j in range(k):
-            toks += [("CONST", float(cvals[j]))] + (["X", "MUL"] if depth == k and j == k - 2 else [])
-        for j in range(depth - k):
+        # leaves: the first m constants folded by + and * in turn (at least one x leaf
+        # follows), the other constants, then x and exp(-x) in turn; operators + and * in
+        # turn from the innermost
+        m = max(1, k - depth + 2)
+        toks = [("CONST", float(cvals[0]))]
+        for j in range(1, m):
+            toks += [("CONST", float(cvals[j])), "ADD" if j % 2 else "MUL"]
+        toks += [("CONST", float(cvals[j])) for j in range(m, k)]
+        for j in range(depth - 1 - k + m):
             toks += ["X"] if j % 2 == 0 else ["X", "NEG", "EXP"]
         toks += ["ADD" if j % 2 == 0 else "MUL" for j in range(depth - 1)]
         return toks
