@@ -47,7 +47,7 @@ HBM_PEAK = 8.0e12
 # were measured on.  For the batched workload a "launch" is one block's six concurrent
 # size-class launches.
 MEASURED_TRAFFIC = {
-    ("full", 1048576, 1000, 0): 2_617_144_928,
+    ("full", 1048576, 1000, 0): 1_323_373_952,
     ("large", 131072, 1000, 0): 163_308_256,
     ("batched", 262144, 1000, 0): 384_401_216,
 }
@@ -56,7 +56,7 @@ MEASURED_TRAFFIC = {
 # the share of issue slots each unit is busy: one wave64 instruction per SIMD every 4
 # cycles, 1024 SIMDs at 2.4 GHz.
 MEASURED_ISSUE = {
-    ("full", 1048576, 1000, 0): (1110.94, 852.87),
+    ("full", 1048576, 1000, 0): (1116.28, 758.48),
     ("large", 131072, 1000, 0): (281.12, 288.25),
     ("batched", 262144, 1000, 0): (319.92, 296.77),
 }
