@@ -574,8 +574,9 @@ __device__ __forceinline__ double wave_loglik(const WaveProg &p, const GprDatase
 // adds the per-point gradient (-2 r w) f' of its points, and one wave sum per constant
 // ends the pass.  Programs qualify when they use only the config grammar (x, constant,
 // one, + - * / pow exp log neg), hold no constant-operand token, and hold at most
-// GPR_DUAL_NT constant tokens within stack depth GPR_DUAL_DEPTH, GPR_DUAL_NT6 within
-// GPR_DUAL_DEPTH6 or GPR_DUAL_NTMAX within GPR_DUAL_DEPTH4.
+// GPR_DUAL_NT constant tokens within stack depth GPR_DUAL_DEPTH or at most GPR_DUAL_NTMAX
+// within GPR_DUAL_DEPTH4 (dual_nt: one pass of GPR_DUAL_NT4 tangents for four constants,
+// of GPR_DUAL_NT6 for five or six within GPR_DUAL_DEPTH6, else two passes).
 #define GPR_DUAL_NT 3
 #define GPR_DUAL_NT4 4
 #define GPR_DUAL_NT6 6
